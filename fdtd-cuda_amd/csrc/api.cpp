@@ -1,0 +1,662 @@
+// C ABI implementation (include/fdtd.h): owns device memory, the solver
+// stream and the step sequencing.  Replaces the reference's main() time loop
+// (main.cu:2724-3175) and its globals with one handle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd.h"
+#include "host.h"
+#include "internal.h"
+
+using fdtd::Setup;
+
+namespace {
+thread_local std::string g_create_err = "no error";
+}
+
+struct fdtd_sim {
+    Setup S;
+    std::string err = "no error";
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    Geom g{};
+    // fields: [parity][component]
+    float *E[2][3] = {}, *H[2][3] = {};
+    int p = 0;                      // current field parity
+    float *psiH[2][6] = {}, *psiE[6] = {};
+    int pp = 0;                     // current psi_H parity
+    // CPML tables (float)
+    float *ik_e[3] = {}, *ik_h[3] = {}, *be[3] = {}, *ce[3] = {}, *bh[3] = {}, *ch[3] = {};
+    // 1D line
+    double *line_e[2] = {}, *line_h = nullptr, *mur = nullptr;
+    int le = 0;
+    // dispersive box
+    int box_lo[3] = {0, 0, 0}, box_n[3] = {0, 0, 0};
+    long nbox = 0;
+    uint8_t *disp_id[3] = {};
+    float *disp_st[3][8] = {};
+    int n_disp[3] = {0, 0, 0};
+    // DFT
+    FaceDev fd{};
+    long mtot = 0;
+    float *X = nullptr;             // [K][mtot]
+    double *acc = nullptr;          // [2F][mtot]
+    double *W = nullptr;            // [2][K][2F]
+    double *freq_d = nullptr;
+    double *flux_d = nullptr;       // [12][F]
+    int K = 64, slot = 0;
+    std::vector<int> slot_t;
+    int samples = 0;
+    // misc
+    float *dense = nullptr;         // NX*NY*NZ staging for field copies
+    int t = 0;
+    int tj = 8, ichunk = 32;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+int set_err(fdtd_sim *s, int code, const std::string &m) {
+    if (s) s->err = m;
+    else g_create_err = m;
+    return code;
+}
+
+#define HIPCHK(sim, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return set_err(sim, FDTD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(fdtd_sim *s, T **ptr, size_t count) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        s->err = std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e);
+        return FDTD_ERR_ALLOC;
+    }
+    s->allocs.push_back(p);
+    e = hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(T), s->stream);
+    if (e != hipSuccess) { s->err = hipGetErrorString(e); return FDTD_ERR_DEVICE; }
+    *ptr = (T *)p;
+    return FDTD_OK;
+}
+
+template <typename T>
+int upload(fdtd_sim *s, T **ptr, const std::vector<T> &v) {
+    int rc = dalloc(s, ptr, v.size());
+    if (rc) return rc;
+    HIPCHK(s, hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FDTD_OK;
+}
+
+std::vector<float> to_f(const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); }
+
+StepDev make_step(fdtd_sim *s, bool fused) {
+    const Setup &S = s->S;
+    StepDev d{};
+    d.g = s->g;
+    d.c.Chh = (float)S.Chh; d.c.Che = (float)S.Che; d.c.Cee = (float)S.Cee; d.c.Ceh = (float)S.Ceh;
+    for (int a = 0; a < 3; a++) {
+        d.cp[a].ik_e = s->ik_e[a]; d.cp[a].ik_h = s->ik_h[a];
+        d.cp[a].be = s->be[a]; d.cp[a].ce = s->ce[a]; d.cp[a].bh = s->bh[a]; d.cp[a].ch = s->ch[a];
+        d.cp[a].Nc = S.cp[a].Nc; d.cp[a].F = S.cp[a].F;
+    }
+    TfsfDev &t = d.tf;
+    for (int a = 0; a < 3; a++) { t.t0[a] = S.tf0[a]; t.t1[a] = S.tf1[a]; t.inv_d[a] = (float)(1.0 / S.P.dx); }
+    const double st = std::sin(S.theta), ct = std::cos(S.theta), sp = std::sin(S.phi), cph = std::cos(S.phi);
+    const double sps = std::sin(S.psi), cps = std::cos(S.psi);
+    t.s[0] = (float)(st * cph); t.s[1] = (float)(st * sp); t.s[2] = (float)ct;
+    // polarisation factors of the incident E and H components (main.cu:395-470)
+    t.ef[0] = (float)(cps * sp - sps * ct * cph);
+    t.ef[1] = (float)(-cps * cph - sps * ct * sp);
+    t.ef[2] = (float)(sps * st);
+    t.hf[0] = (float)(sps * sp + cps * ct * cph);
+    t.hf[1] = (float)(-sps * cph + cps * ct * sp);
+    t.hf[2] = (float)(-cps * st);
+    t.cH = (float)S.Che; t.cE = (float)S.Ceh;
+    t.dscale = 1.0f;  // d_1D = dx (tfsf2.cu:19)
+    t.m0 = S.m0;
+    t.e_inc = s->line_e[s->le];
+    t.h_inc = s->line_h;
+    d.tfsf_on = fused ? 1 : 0;
+    d.fd = s->fd;
+    d.sample_row = nullptr;
+    return d;
+}
+
+LineDev make_line(fdtd_sim *s) {
+    const Setup &S = s->S;
+    LineDev l{};
+    l.e_old = s->line_e[s->le];
+    l.e_new = s->line_e[1 - s->le];
+    l.h = s->line_h;
+    l.mur = s->mur;
+    l.L = S.L;
+    l.src = S.src;
+    const double d1 = S.P.dx;
+    l.ch = fdtd::kZ0 * S.dt / (fdtd::kMU0 * d1);
+    l.ce = S.dt / (fdtd::kZ0 * fdtd::kEP0 * d1);
+    const double factor = fdtd::kC0 * S.dt;
+    l.mur_coef = (factor - d1) / (factor + d1);
+    l.amp = S.P.MAX_AMP;
+    l.delay = S.delay;
+    l.width = S.width;
+    l.w0dt = 2 * fdtd::kPI * S.f0 * S.dt;
+    l.static_field = S.P.StaticField == 1;
+    l.t = s->t;
+    return l;
+}
+
+DispDev make_disp(fdtd_sim *s, int parity) {
+    const Setup &S = s->S;
+    DispDev d{};
+    d.g = s->g;
+    for (int a = 0; a < 3; a++) { d.lo[a] = s->box_lo[a]; d.n[a] = s->box_n[a]; d.ik_e[a] = s->ik_e[a]; }
+    for (int c = 0; c < 3; c++) {
+        d.id[c] = s->disp_id[c];
+        d.rho[c] = nullptr;
+        for (int q = 0; q < 8; q++) d.st[c][q] = s->disp_st[c][q];
+        d.H[c] = s->H[parity][c];
+        d.E[c] = s->E[parity][c];
+    }
+    // normalised ADE: P~ = P/eps0, 1/C_E folded into every coefficient
+    const fdtd::Ade &A = S.ade;
+    const double CE = A.C_E, e0 = fdtd::kEP0;
+    d.a0 = (float)(S.dt / (fdtd::kZ0 * CE));
+    d.a1 = (float)(A.C_E1 / CE);
+    d.a2 = (float)(A.C_E2 / CE);
+    const double *k[3] = {A.d, A.c[0], A.c[1]};
+    for (int q = 0; q < 3; q++) {
+        d.b1[q] = (float)((k[q][0] - 1) * e0 / CE);
+        d.b2[q] = (float)(k[q][1] * e0 / CE);
+        d.p1[q] = (float)k[q][0];
+        d.p2[q] = (float)k[q][1];
+        d.p3[q] = (float)(k[q][2] / e0);
+        d.p4[q] = (float)(k[q][3] / e0);
+        d.p5[q] = (float)(k[q][4] / e0);
+    }
+    return d;
+}
+
+// Builds the dispersive box from the masks and uploads ids; state zeroed.
+int build_dispersive(fdtd_sim *s) {
+    const Setup &S = s->S;
+    const int NX = S.N[0], NY = S.N[1], NZ = S.N[2];
+    int lo[3] = {NX, NY, NZ}, hi[3] = {-1, -1, -1};
+    for (int c = 0; c < 3; c++) s->n_disp[c] = 0;
+    for (int c = 0; c < 3; c++) {
+        const std::vector<uint8_t> &m = S.mat[c + 1];
+        for (int i = 0; i < NX; i++)
+            for (int j = 0; j < NY; j++)
+                for (int k = 0; k < NZ; k++) {
+                    const uint8_t v = m[((size_t)i * NY + j) * NZ + k];
+                    if (v > 1 && v < 6) {
+                        s->n_disp[c]++;
+                        lo[0] = std::min(lo[0], i); lo[1] = std::min(lo[1], j); lo[2] = std::min(lo[2], k);
+                        hi[0] = std::max(hi[0], i); hi[1] = std::max(hi[1], j); hi[2] = std::max(hi[2], k);
+                    }
+                }
+    }
+    const long total = (long)s->n_disp[0] + s->n_disp[1] + s->n_disp[2];
+    if (total == 0) {
+        for (int a = 0; a < 3; a++) { s->box_lo[a] = 0; s->box_n[a] = 0; }
+        s->nbox = 0;
+        return FDTD_OK;
+    }
+    for (int a = 0; a < 3; a++) {
+        // dispersive cells must sit strictly inside the absorption monitor box, which keeps
+        // them off the CPML, the TFSF faces and every monitor face
+        if (lo[a] <= S.abs_s[a] || hi[a] >= S.abs_e[a] - 1)
+            return set_err(s, FDTD_ERR_INVALID, "dispersive cells must lie strictly inside the absorption box");
+    }
+    if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "hydrodynamic model not available in this build");
+    for (int a = 0; a < 3; a++) { s->box_lo[a] = lo[a]; s->box_n[a] = hi[a] - lo[a] + 1; }
+    s->nbox = (long)s->box_n[0] * s->box_n[1] * s->box_n[2];
+    for (int c = 0; c < 3; c++) {
+        std::vector<uint8_t> ids(s->nbox, 0);
+        const std::vector<uint8_t> &m = S.mat[c + 1];
+        for (int bi = 0; bi < s->box_n[0]; bi++)
+            for (int bj = 0; bj < s->box_n[1]; bj++)
+                for (int bk = 0; bk < s->box_n[2]; bk++) {
+                    const uint8_t v = m[((size_t)(lo[0] + bi) * NY + lo[1] + bj) * NZ + lo[2] + bk];
+                    ids[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk] = (v > 1 && v < 6) ? 1 : 0;
+                }
+        int rc = upload(s, &s->disp_id[c], ids);
+        if (rc) return rc;
+        for (int q = 0; q < 8; q++) {
+            rc = dalloc(s, &s->disp_st[c][q], s->nbox);
+            if (rc) return rc;
+        }
+    }
+    return FDTD_OK;
+}
+
+void build_faces(fdtd_sim *s) {
+    const Setup &S = s->S;
+    FaceDev &f = s->fd;
+    long off = 0;
+    for (int q = 0; q < 12; q++) {
+        const int box = q / 6, axis = (q / 2) % 3, side = q % 2;
+        const int *st = box ? S.abs_s : S.sca_s, *en = box ? S.abs_e : S.sca_e;
+        const int outer = axis == 0 ? 1 : 0, inner = axis == 2 ? 1 : 2;
+        f.plane[q] = side ? en[axis] - 1 : st[axis];
+        f.lo_a[q] = st[outer];
+        f.na[q] = en[outer] - st[outer];
+        f.lo_b[q] = st[inner];
+        f.nb[q] = en[inner] - st[inner];
+        f.off[q] = off;
+        off += 2L * f.na[q] * f.nb[q];
+    }
+    f.mE = off;
+    f.mE_pad = (off + 255) / 256 * 256;
+    s->mtot = f.mE_pad * 2;
+}
+
+int contract(fdtd_sim *s) {
+    if (s->slot == 0) return FDTD_OK;
+    const int F = s->S.P.NUM_freq;
+    HIPCHK(s, launch_twiddle(s->W, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
+    HIPCHK(s, launch_dft_contract(s->X, s->W, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
+    s->slot = 0;
+    return FDTD_OK;
+}
+
+bool sampling_step(const fdtd_sim *s) { return s->S.P.StaticField == 0 && s->t % s->S.P.dft_stride == 0; }
+
+int after_sample(fdtd_sim *s) {
+    s->slot_t[s->slot] = s->t;
+    s->slot++;
+    s->samples++;
+    if (s->slot == s->K) return contract(s);
+    return FDTD_OK;
+}
+
+hipEvent_t next_event(fdtd_sim *s, int which) {
+    if (s->ev_used >= s->events.size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+        s->events.push_back({a, b});
+    }
+    return which == 0 ? s->events[s->ev_used].first : s->events[s->ev_used].second;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fdtd_default_params(fdtd_params *p) { fdtd::default_params(p); }
+
+const char *fdtd_last_error(const fdtd_sim *sim) { return sim ? sim->err.c_str() : g_create_err.c_str(); }
+
+int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
+    if (!p || !out) return set_err(nullptr, FDTD_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, FDTD_ERR_DEVICE, "no HIP device");
+    fdtd_sim *s = new (std::nothrow) fdtd_sim();
+    if (!s) return set_err(nullptr, FDTD_ERR_ALLOC, "host allocation failed");
+    std::string err;
+    if (!fdtd::build_setup(*p, s->S, err)) {
+        delete s;
+        return set_err(nullptr, FDTD_ERR_INVALID, err);
+    }
+    const Setup &S = s->S;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return set_err(nullptr, FDTD_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    if (const char *v = std::getenv("FDTD_TJ")) s->tj = std::atoi(v);
+    if (const char *v = std::getenv("FDTD_ICHUNK")) s->ichunk = std::max(1, std::atoi(v));
+    auto fail = [&](int rc) {
+        std::string m = s->err;
+        fdtd_destroy(s);
+        return set_err(nullptr, rc, m);
+    };
+    int rc;
+    Geom &g = s->g;
+    g.NX = S.N[0]; g.NY = S.N[1]; g.NZ = S.N[2];
+    g.sz = (g.NZ + 2 + 31) / 32 * 32;
+    g.sxy = (long)g.sz * (g.NY + 2);
+    const size_t fsz = (size_t)g.sxy * (g.NX + 2);
+    const long base = g.sxy + g.sz + 1;
+    for (int q = 0; q < 2; q++)
+        for (int c = 0; c < 3; c++) {
+            float *a, *b;
+            if ((rc = dalloc(s, &a, fsz)) || (rc = dalloc(s, &b, fsz))) return fail(rc);
+            s->E[q][c] = a + base;
+            s->H[q][c] = b + base;
+        }
+    // psi slabs (ordering in internal.h)
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+    const int ncx = S.cp[0].Nc, ncy = S.cp[1].Nc, ncz = S.cp[2].Nc;
+    const size_t hz_ = (size_t)NX * NY * 2 * ncz, hy_ = (size_t)NX * 2 * ncy * NZ, hx_ = (size_t)2 * ncx * NY * NZ;
+    const size_t ez_ = (size_t)NX * NY * 2 * (ncz + 1), ey_ = (size_t)NX * 2 * (ncy + 1) * NZ, ex_ = (size_t)2 * (ncx + 1) * NY * NZ;
+    const size_t hsz[6] = {hz_, hy_, hz_, hx_, hx_, hy_};
+    const size_t esz[6] = {ez_, ey_, ez_, ex_, ey_, ex_};
+    for (int q = 0; q < 2; q++)
+        for (int a = 0; a < 6; a++)
+            if ((rc = dalloc(s, &s->psiH[q][a], hsz[a]))) return fail(rc);
+    for (int a = 0; a < 6; a++)
+        if ((rc = dalloc(s, &s->psiE[a], esz[a]))) return fail(rc);
+    // CPML tables: 1/(kappa d), recursion b and c/d, near slab then far slab
+    for (int a = 0; a < 3; a++) {
+        const fdtd::CpmlAxis &c = S.cp[a];
+        const double d = S.P.dx;
+        std::vector<float> ike(c.ked.size()), ikh(c.khd.size());
+        for (size_t i = 0; i < c.ked.size(); i++) { ike[i] = (float)(1.0 / c.ked[i]); ikh[i] = (float)(1.0 / c.khd[i]); }
+        std::vector<float> be, ce, bh, ch;
+        for (int i = 0; i <= c.Nc; i++) { be.push_back((float)c.beN[i]); ce.push_back((float)(c.ceN[i] / d)); }
+        for (int i = 0; i <= c.Nc; i++) { be.push_back((float)c.beF[i]); ce.push_back((float)(c.ceF[i] / d)); }
+        for (int i = 0; i < c.Nc; i++) { bh.push_back((float)c.bhN[i]); ch.push_back((float)(c.chN[i] / d)); }
+        for (int i = 0; i < c.Nc; i++) { bh.push_back((float)c.bhF[i]); ch.push_back((float)(c.chF[i] / d)); }
+        if ((rc = upload(s, &s->ik_e[a], ike)) || (rc = upload(s, &s->ik_h[a], ikh)) || (rc = upload(s, &s->be[a], be)) ||
+            (rc = upload(s, &s->ce[a], ce)) || (rc = upload(s, &s->bh[a], bh)) || (rc = upload(s, &s->ch[a], ch)))
+            return fail(rc);
+    }
+    // 1D line
+    if ((rc = dalloc(s, &s->line_e[0], S.L)) || (rc = dalloc(s, &s->line_e[1], S.L)) || (rc = dalloc(s, &s->line_h, S.L)) ||
+        (rc = dalloc(s, &s->mur, 2)))
+        return fail(rc);
+    // dispersive box
+    if ((rc = build_dispersive(s))) return fail(rc);
+    // DFT buffers
+    build_faces(s);
+    s->K = S.P.dft_block;
+    s->slot_t.assign(s->K, 0);
+    const int F = S.P.NUM_freq;
+    if ((rc = dalloc(s, &s->X, (size_t)s->K * s->mtot)) || (rc = dalloc(s, &s->acc, (size_t)2 * F * s->mtot)) ||
+        (rc = dalloc(s, &s->W, (size_t)2 * s->K * 2 * F)) || (rc = upload(s, &s->freq_d, S.freq)) ||
+        (rc = dalloc(s, &s->flux_d, (size_t)12 * F)))
+        return fail(rc);
+    if ((rc = dalloc(s, &s->dense, (size_t)NX * NY * NZ))) return fail(rc);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(FDTD_ERR_DEVICE);
+    *out = s;
+    return FDTD_OK;
+}
+
+int fdtd_destroy(fdtd_sim *s) {
+    if (!s) return FDTD_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &e : s->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (void *p : s->allocs) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return FDTD_OK;
+}
+
+int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
+    if (!s || !g) return FDTD_ERR_INVALID;
+    const Setup &S = s->S;
+    std::memset(g, 0, sizeof(*g));
+    g->NX = S.N[0]; g->NY = S.N[1]; g->NZ = S.N[2];
+    for (int a = 0; a < 3; a++) {
+        g->Ncpml[a] = S.cp[a].Nc;
+        g->tfsf[a] = S.tf0[a];
+        g->abs_start[a] = S.abs_s[a]; g->abs_end[a] = S.abs_e[a];
+        g->sca_start[a] = S.sca_s[a]; g->sca_end[a] = S.sca_e[a];
+        g->n_dispersive[a] = s->n_disp[a];
+        g->box_lo[a] = s->box_lo[a];
+        g->box_hi[a] = s->box_lo[a] + s->box_n[a];
+    }
+    g->dt = S.dt;
+    g->inc_length = S.L; g->m0 = S.m0;
+    g->width = S.width; g->delay = S.delay; g->f_0 = S.f0;
+    g->num_freq = S.P.NUM_freq;
+    g->t = s->t;
+    g->hydro = S.hydro;
+    g->sample_count = s->samples;
+    return FDTD_OK;
+}
+
+int fdtd_set_materials(fdtd_sim *s, const uint8_t *mx, const uint8_t *my, const uint8_t *mz) {
+    if (!s || !mx || !my || !mz) return FDTD_ERR_INVALID;
+    const size_t N = (size_t)s->g.NX * s->g.NY * s->g.NZ;
+    Setup &S = s->S;
+    std::vector<uint8_t> old[3] = {S.mat[1], S.mat[2], S.mat[3]};
+    S.mat[1].assign(mx, mx + N);
+    S.mat[2].assign(my, my + N);
+    S.mat[3].assign(mz, mz + N);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    int rc = build_dispersive(s);
+    if (rc) {
+        for (int c = 0; c < 3; c++) S.mat[c + 1] = old[c];
+        std::string m = s->err;
+        build_dispersive(s);
+        s->err = m;
+        return rc;
+    }
+    // the new dispersive cells start from the current field as E^n
+    if (s->nbox) {
+        DispDev d = make_disp(s, s->p);
+        HIPCHK(s, launch_disp_sync_state(d, 1, s->stream));
+    }
+    return FDTD_OK;
+}
+
+int fdtd_get_materials(const fdtd_sim *s, int comp, uint8_t *out) {
+    if (!s || !out || comp < 0 || comp > 3) return FDTD_ERR_INVALID;
+    std::memcpy(out, s->S.mat[comp].data(), s->S.mat[comp].size());
+    return FDTD_OK;
+}
+
+int fdtd_update_h(fdtd_sim *s) {
+    if (!s) return FDTD_ERR_INVALID;
+    StepDev d = make_step(s, false);
+    for (int c = 0; c < 3; c++) {
+        d.Ein[c] = s->E[s->p][c]; d.Hin[c] = s->H[s->p][c];
+        d.Eout[c] = s->E[s->p][c]; d.Hout[c] = s->H[s->p][c];
+    }
+    for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = d.psiH_out.p[a] = s->psiH[s->pp][a]; d.psiE.p[a] = s->psiE[a]; }
+    HIPCHK(s, launch_split_h(d, s->stream));
+    return FDTD_OK;
+}
+
+int fdtd_tfsf(fdtd_sim *s) {
+    if (!s) return FDTD_ERR_INVALID;
+    LineDev l = make_line(s);
+    HIPCHK(s, launch_line(l, s->stream));
+    StepDev d = make_step(s, false);
+    for (int c = 0; c < 3; c++) { d.Eout[c] = s->E[s->p][c]; d.Hout[c] = s->H[s->p][c]; }
+    HIPCHK(s, launch_tfsf_split(d, s->stream));
+    s->le ^= 1;
+    return FDTD_OK;
+}
+
+int fdtd_update_e(fdtd_sim *s) {
+    if (!s) return FDTD_ERR_INVALID;
+    StepDev d = make_step(s, false);
+    for (int c = 0; c < 3; c++) {
+        d.Ein[c] = s->E[s->p][c]; d.Hin[c] = s->H[s->p][c];
+        d.Eout[c] = s->E[s->p][c]; d.Hout[c] = s->H[s->p][c];
+    }
+    for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = d.psiH_out.p[a] = s->psiH[s->pp][a]; d.psiE.p[a] = s->psiE[a]; }
+    HIPCHK(s, launch_split_e(d, s->stream));
+    if (s->nbox) {
+        DispDev dd = make_disp(s, s->p);
+        HIPCHK(s, launch_dispersive(dd, s->stream));
+    }
+    return FDTD_OK;
+}
+
+int fdtd_dft_accumulate(fdtd_sim *s) {
+    if (!s) return FDTD_ERR_INVALID;
+    int rc = FDTD_OK;
+    if (sampling_step(s)) {
+        StepDev d = make_step(s, false);
+        for (int c = 0; c < 3; c++) { d.Ein[c] = s->E[s->p][c]; d.Hin[c] = s->H[s->p][c]; }
+        d.sample_row = s->X + (size_t)s->slot * s->mtot;
+        HIPCHK(s, launch_dft_sample_split(d, s->stream));
+        rc = after_sample(s);
+    }
+    s->t++;
+    return rc;
+}
+
+int fdtd_step(fdtd_sim *s, int nsteps) {
+    if (!s || nsteps < 0) return FDTD_ERR_INVALID;
+    for (int n = 0; n < nsteps; n++) {
+        LineDev l = make_line(s);
+        HIPCHK(s, launch_line(l, s->stream));
+        StepDev d = make_step(s, true);
+        const int q = s->p, pp = s->pp;
+        for (int c = 0; c < 3; c++) {
+            d.Ein[c] = s->E[q][c]; d.Hin[c] = s->H[q][c];
+            d.Eout[c] = s->E[1 - q][c]; d.Hout[c] = s->H[1 - q][c];
+        }
+        for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
+        const bool samp = sampling_step(s);
+        if (samp) d.sample_row = s->X + (size_t)s->slot * s->mtot;
+        if (s->timing) {
+            hipEvent_t a = next_event(s, 0), b = next_event(s, 1);
+            if (!a || !b) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
+            HIPCHK(s, hipEventRecord(a, s->stream));
+            HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, s->stream));
+            HIPCHK(s, hipEventRecord(b, s->stream));
+            s->ev_used++;
+        } else {
+            HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, s->stream));
+        }
+        if (s->nbox) {
+            DispDev dd = make_disp(s, 1 - q);
+            HIPCHK(s, launch_dispersive(dd, s->stream));
+        }
+        s->p ^= 1;
+        s->pp ^= 1;
+        s->le ^= 1;
+        if (samp) {
+            int rc = after_sample(s);
+            if (rc) return rc;
+        }
+        s->t++;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_sync(fdtd_sim *s) {
+    if (!s) return FDTD_ERR_INVALID;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+int fdtd_get_field(fdtd_sim *s, int comp, float *out) {
+    if (!s || !out || comp < 0 || comp > 5) return FDTD_ERR_INVALID;
+    const float *f = comp < 3 ? s->E[s->p][comp] : s->H[s->p][comp - 3];
+    HIPCHK(s, launch_pack_field(f, s->g, s->dense, 1, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(out, s->dense, (size_t)s->g.NX * s->g.NY * s->g.NZ * sizeof(float), hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+int fdtd_set_field(fdtd_sim *s, int comp, const float *in) {
+    if (!s || !in || comp < 0 || comp > 5) return FDTD_ERR_INVALID;
+    float *f = comp < 3 ? s->E[s->p][comp] : s->H[s->p][comp - 3];
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(s->dense, in, (size_t)s->g.NX * s->g.NY * s->g.NZ * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(s, launch_pack_field(f, s->g, s->dense, 0, s->stream));
+    if (comp < 3 && s->nbox) {
+        DispDev d = make_disp(s, s->p);
+        HIPCHK(s, launch_disp_sync_state(d, 1, s->stream));
+    }
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+int fdtd_get_line(fdtd_sim *s, double *e, double *h) {
+    if (!s) return FDTD_ERR_INVALID;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (e) HIPCHK(s, hipMemcpy(e, s->line_e[s->le], s->S.L * sizeof(double), hipMemcpyDeviceToHost));
+    if (h) HIPCHK(s, hipMemcpy(h, s->line_h, s->S.L * sizeof(double), hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+int fdtd_get_spectrum_setup(const fdtd_sim *s, double *freq, double *einc) {
+    if (!s) return FDTD_ERR_INVALID;
+    for (int m = 0; m < s->S.P.NUM_freq; m++) {
+        if (freq) freq[m] = s->S.freq[m];
+        if (einc) { einc[2 * m] = s->S.Einc[m].real(); einc[2 * m + 1] = s->S.Einc[m].imag(); }
+    }
+    return FDTD_OK;
+}
+
+int fdtd_get_dft(fdtd_sim *s, int face, int comp, double *out) {
+    if (!s || !out || face < 0 || face > 11 || comp < 0 || comp > 3) return FDTD_ERR_INVALID;
+    int rc = contract(s);
+    if (rc) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const FaceDev &f = s->fd;
+    const long blk = (long)f.na[face] * f.nb[face];
+    const long col = (comp < 2 ? 0 : f.mE_pad) + f.off[face] + (comp % 2) * blk;
+    const int F = s->S.P.NUM_freq;
+    std::vector<double> re(blk), im(blk);
+    for (int m = 0; m < F; m++) {
+        HIPCHK(s, hipMemcpy(re.data(), s->acc + (size_t)m * s->mtot + col, blk * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(s, hipMemcpy(im.data(), s->acc + (size_t)(F + m) * s->mtot + col, blk * sizeof(double), hipMemcpyDeviceToHost));
+        for (long c = 0; c < blk; c++) {
+            out[((size_t)m * blk + c) * 2] = re[c];
+            out[((size_t)m * blk + c) * 2 + 1] = im[c];
+        }
+    }
+    return FDTD_OK;
+}
+
+int fdtd_flux(fdtd_sim *s, double *sca, double *abs_, double *ext, double *raw) {
+    if (!s) return FDTD_ERR_INVALID;
+    int rc = contract(s);
+    if (rc) return rc;
+    const Setup &S = s->S;
+    const int F = S.P.NUM_freq;
+    HIPCHK(s, launch_flux_reduce(s->acc, s->fd, s->mtot, s->fd.mE_pad, F, S.P.dx * S.P.dy, s->flux_d, s->stream));
+    std::vector<double> Q(12 * F);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(Q.data(), s->flux_d, Q.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // normalise by |E_incident|^2 and combine the faces (Spectral.cu:1162-1168)
+    for (int m = 0; m < F; m++) {
+        const double a = std::abs(S.Einc[m]);
+        for (int q = 0; q < 12; q++) Q[q * F + m] /= a * a;
+    }
+    const double area = S.P.xs_area > 0 ? S.P.xs_area : fdtd::kPI * S.P.nano_sphere_radius * S.P.nano_sphere_radius;
+    for (int m = 0; m < F; m++) {
+        double sc = 0, ab = 0;
+        for (int ax = 0; ax < 3; ax++) {
+            sc += -Q[(ax * 2 + 0) * F + m] + Q[(ax * 2 + 1) * F + m];
+            ab += -Q[(6 + ax * 2 + 0) * F + m] + Q[(6 + ax * 2 + 1) * F + m];
+        }
+        sc /= area;
+        ab = -ab / area;
+        if (sca) sca[m] = sc;
+        if (abs_) abs_[m] = ab;
+        if (ext) ext[m] = sc + ab;
+    }
+    if (raw) std::memcpy(raw, Q.data(), Q.size() * sizeof(double));
+    return FDTD_OK;
+}
+
+int fdtd_kernel_timing(fdtd_sim *s, int enable, double *total_ms, long *launches) {
+    if (!s) return FDTD_ERR_INVALID;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    double tot = 0;
+    for (size_t i = 0; i < s->ev_used; i++) {
+        float ms = 0;
+        HIPCHK(s, hipEventElapsedTime(&ms, s->events[i].first, s->events[i].second));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (long)s->ev_used;
+    s->ev_used = 0;
+    s->timing = enable != 0;
+    return FDTD_OK;
+}
+
+}  // extern "C"

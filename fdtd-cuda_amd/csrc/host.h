@@ -1,0 +1,69 @@
+// Host-side FP64 setup of the FDTD solver (no HIP types).
+#ifndef FDTD_HOST_H_
+#define FDTD_HOST_H_
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd.h"
+
+namespace fdtd {
+
+// Physical constants as the reference defines them (define.h:22-29).
+constexpr double kPI = 3.1415926535897932384626433832795;
+constexpr double kEP0 = 8.85418782E-12;
+constexpr double kMU0 = 1.25663706E-06;
+constexpr double kC0 = 2.99792458E+08;
+constexpr double kZ0 = 376.730313462;
+constexpr double kE0 = 1.60217662E-19;
+constexpr double kME = 9.10938356E-31;
+
+// One CPML axis: recursion coefficients (b, c) at E positions (Nc+1 per
+// slab) and H positions (Nc per slab), and the kappa-scaled spacings.
+struct CpmlAxis {
+    int Nc = 0;
+    int F = 0;                      // first cell of the far slab (cpml_F_*)
+    std::vector<double> beN, ceN, beF, ceF;   // Nc+1
+    std::vector<double> bhN, chN, bhF, chF;   // Nc
+    std::vector<double> ked, khd;             // N
+};
+
+// Drude + 2 critical-point ADE coefficients (setup.cu:1282-1373).
+struct Ade {
+    double eps_inf = 0, N_EQ = 0;
+    double C_E = 0, C_E1 = 0, C_E2 = 0;
+    double d[5] = {0, 0, 0, 0, 0};  // Drude pole d1..d5
+    double dNL = 0;
+    double c[2][5] = {{0}};         // CP poles C1..C5
+};
+
+struct Setup {
+    fdtd_params P;
+    int N[3] = {0, 0, 0};           // augmented grid
+    double dt = 0;
+    CpmlAxis cp[3];
+    int tf0[3] = {0, 0, 0}, tf1[3] = {0, 0, 0};
+    int abs_s[3], abs_e[3], sca_s[3], sca_e[3];
+    int L = 0, m0 = 0, src = 0;
+    double width = 0, delay = 0, f0 = 0;
+    double theta = 0, phi = 0, psi = 0;
+    double Chh = 1, Che = 0, Cee = 1, Ceh = 0;
+    bool hydro = false;
+    Ade ade;
+    std::vector<double> freq;
+    std::vector<std::complex<double>> Einc;
+    std::vector<uint8_t> mat[4];    // mat_matrix, X, Y, Z
+};
+
+void default_params(fdtd_params *p);
+// Builds everything fdtd_create needs; returns false with a message on invalid input.
+bool build_setup(const fdtd_params &p, Setup &s, std::string &err);
+// Staircased sphere masks (setup.cu:219-248).
+void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]);
+double source_pulse(const Setup &s, int t);
+
+}  // namespace fdtd
+
+#endif

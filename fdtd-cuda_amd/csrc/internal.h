@@ -1,0 +1,144 @@
+// Internal structures shared by the host side (api.cpp, setup.cpp) and the
+// HIP kernels (kernels.hip).  Not part of the public ABI.
+#ifndef FDTD_INTERNAL_H_
+#define FDTD_INTERNAL_H_
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+// ---------------------------------------------------------------------------
+// Field layout in HBM.  Every 3D field is FP32, z fastest, with one ghost
+// layer on each side of every axis so stencil reads at -1 and N never need a
+// bounds test.  The stored pointer is pre-offset to element (0,0,0):
+//     f[i*sxy + j*sz + k],  i in [-1, NX], j in [-1, NY], k in [-1, NZ].
+// sz is NZ+2 rounded up to 32 floats (128 B rows); ghosts are zero forever.
+// ---------------------------------------------------------------------------
+struct Geom {
+    int NX, NY, NZ;
+    int sz;        // row pitch (floats)
+    long sxy;      // plane pitch (floats) = sz * (NY + 2)
+};
+
+__host__ __device__ inline long gidx(const Geom &g, int i, int j, int k) {
+    return (long)i * g.sxy + (long)j * g.sz + k;
+}
+
+// CPML tables per axis, FP32 copies of the FP64 host profiles.
+//   ik_e[N], ik_h[N]   : 1/(kappa*d) at E and H positions (ked/khd, setup.cu:518-529)
+//   be[2*(Nc+1)], ce[2*(Nc+1)]: E recursion, near slab at [0, Nc], far at Nc+1+k2
+//   bh[2*Nc],     ch[2*Nc]    : H recursion, near at [0, Nc), far at Nc+k2
+//   ce/ch already divided by the cell size.
+struct CpmlDev {
+    const float *ik_e, *ik_h;
+    const float *be, *ce, *bh, *ch;
+    int Nc;     // slab thickness
+    int F;      // first cell of the far slab (cpml_F_* in the reference)
+};
+
+// psi storage.  Slab arrays hold the near slab then the far slab along the
+// slab axis (H: 2*Nc layers, E: 2*(Nc+1) layers), z fastest.
+//   z-slabs : [NX][NY][layers]
+//   y-slabs : [NX][layers][NZ]
+//   x-slabs : [layers][NY][NZ]
+// Order of the six arrays per field type (component, slab axis):
+//   H: 0 hx_z, 1 hx_y, 2 hy_z, 3 hy_x, 4 hz_x, 5 hz_y
+//   E: 0 ex_z, 1 ex_y, 2 ey_z, 3 ey_x, 4 ez_y, 5 ez_x
+struct PsiSet {
+    float *p[6];
+};
+
+// TFSF plane-wave source constants (CORRECT_X/Y/Z, main.cu:369-758).
+struct TfsfDev {
+    int t0[3], t1[3];        // face planes Ntfsf and N - Ntfsf per axis
+    float s[3];              // propagation direction (sin th cos ph, sin th sin ph, cos th)
+    float ef[3], hf[3];      // polarisation factors of E_inc and H_inc components
+    float cH, cE;            // Che, Ceh
+    float inv_d[3];          // 1/dx, 1/dy, 1/dz
+    float dscale;            // dx / d_1D
+    int m0;
+    const double *e_inc;     // e_inc^n
+    const double *h_inc;     // h_inc^{n+1/2}
+};
+
+// 12 DFT monitor faces; columns of the sample matrix (see dft_* in kernels.hip).
+struct FaceDev {
+    int plane[12];           // index along the normal axis
+    int lo_a[12], lo_b[12];  // face range start along outer/inner in-plane axes
+    int na[12], nb[12];      // face extents
+    long off[12];            // first column of component 0 of this face (E half)
+    long mE;                 // columns in the E half (H half starts at mE_pad)
+    long mE_pad;
+};
+
+// Uniform update coefficients (setup.cu:2015-2062 with uniform media).
+struct CoefDev {
+    float Chh, Che, Cee, Ceh;
+};
+
+// Arguments of the fused and split field kernels.
+struct StepDev {
+    Geom g;
+    CoefDev c;
+    CpmlDev cp[3];
+    TfsfDev tf;
+    int tfsf_on;
+    const float *Ein[3], *Hin[3];   // E^n, H^{n-1/2}
+    float *Eout[3], *Hout[3];       // E^{n+1}, H^{n+1/2}
+    PsiSet psiH_in, psiH_out, psiE;
+    // DFT sampling
+    FaceDev fd;
+    float *sample_row;              // row of the sample ring buffer, or null (no sample this step)
+};
+
+// Dispersive (ADE) region: a dense box [lo, lo+n) of the grid.
+struct DispDev {
+    Geom g;
+    int lo[3], n[3];
+    const uint8_t *id[3];           // per E component: 1 = dispersive, 0 = not
+    const float *rho[3];            // design density per E component, or null (rho = 1)
+    // state per component, box-dense SoA: E^n, E^{n-1}, Drude P^n, P^{n-1}, CP0 P^n, P^{n-1}, CP1 P^n, P^{n-1}
+    float *st[3][8];
+    const float *H[3];              // H^{n+1/2}
+    float *E[3];                    // output field (E^{n+1} written at dispersive cells)
+    const float *ik_e[3];
+    // ADE coefficients with P stored as P/eps0 and 1/C_E folded in
+    float a0, a1, a2;               // E^{n+1} = a0*curl + a1*E^n - a2*E^{n-1} - sum(b1 P^n + b2 P^{n-1})
+    float b1[3], b2[3];             // poles: 0 Drude, 1-2 CP
+    float p1[3], p2[3], p3[3], p4[3], p5[3];   // P^{n+1} = p1 P^n + p2 P^{n-1} + p3 E^{n+1} + p4 E^n + p5 E^{n-1}
+    // design-density variant: vacuum/metal blend (rho in [0,1])
+    float ce_vac, ce_met_minus_vac; // eps0*eps_inf(rho) + rho*sum(c3)  split into parts (scaled)
+};
+
+// 1D incident line (main.cu:2280-2337, 2831-2859), FP64.
+struct LineDev {
+    const double *e_old;
+    double *e_new;
+    double *h;
+    double *mur;                    // e1, e2
+    int L, src;
+    double ch, ce, mur_coef;
+    double amp, delay, width, w0dt; // pulse: amp*exp(-((t-delay)/width)^2/2)*sin(w0dt*t)
+    int static_field;
+    int t;
+};
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (kernels.hip).  All return hipError_t of the launch.
+// ---------------------------------------------------------------------------
+hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, hipStream_t st);
+hipError_t launch_split_h(const StepDev &s, hipStream_t st);
+hipError_t launch_split_e(const StepDev &s, hipStream_t st);
+hipError_t launch_tfsf_split(const StepDev &s, hipStream_t st);
+hipError_t launch_line(const LineDev &l, hipStream_t st);
+hipError_t launch_dispersive(const DispDev &d, hipStream_t st);
+hipError_t launch_dft_sample_split(const StepDev &s, hipStream_t st);
+hipError_t launch_twiddle(double *W, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
+                          double dt, hipStream_t st);
+hipError_t launch_dft_contract(const float *X, const double *W, double *acc, long mtot, long mE_pad, int K,
+                               int F, hipStream_t st);
+hipError_t launch_flux_reduce(const double *acc, const FaceDev &fd, long mtot, long mE_pad, int F, double dA,
+                              double *out, hipStream_t st);
+hipError_t launch_pack_field(const float *f, const Geom &g, float *dense, int to_dense, hipStream_t st);
+hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t st);
+
+#endif

@@ -1,0 +1,693 @@
+// HIP kernels of the FDTD hot path for gfx950 (MI355X).
+//
+// Fused fast path, one launch per time step:
+//   k_fused_step   H update + CPML psi_H + TFSF + E update + CPML psi_E +
+//                  DFT face sampling, marching along x through a (j,k) tile.
+//                  E and H are ping-ponged so every field is read once from
+//                  buffer A and written once to buffer B (48 B per vacuum cell).
+//   k_dispersive   Drude + 2 CP ADE on the dense box holding the metal; runs
+//                  after k_fused_step and overwrites E at dispersive cells.
+//   k_line         1D incident line (h_inc, e_inc, Mur, source) on device.
+//   k_dft_*        twiddle build, buffered DFT contraction, flux reduction.
+// Split path (reference call structure): k_split_h, k_tfsf_split, k_split_e,
+// k_dft_sample_split, sharing the per-cell device functions below.
+//
+// Update equations follow the reference kernels (main.cu:762-1044 H,
+// main.cu:1048-1912 E, main.cu:369-758 TFSF, main.cu:2280-2337 line,
+// main.cu:35-366 DFT); see each function.
+
+#include <hip/hip_runtime.h>
+#include "internal.h"
+
+namespace {
+
+constexpr double kPI = 3.1415926535897932384626433832795;
+
+// ----------------------------------------------------------------- CPML index helpers
+
+// H slabs: 2*Nc layers (near [0,Nc), far Nc+k2); E slabs: 2*(Nc+1) layers (near [0,Nc], far Nc+1+k2)
+__device__ inline long zslab(const Geom &g, int layers, int i, int j, int kk) {
+    return ((long)i * g.NY + j) * layers + kk;
+}
+__device__ inline long yslab(const Geom &g, int layers, int i, int jj, int k) {
+    return ((long)i * layers + jj) * g.NZ + k;
+}
+__device__ inline long xslab(const Geom &g, int ii, int j, int k) {
+    return ((long)ii * g.NY + j) * g.NZ + k;
+}
+
+__device__ inline float psi_step(const float *in, float *out, long idx, float b, float c, float dE, bool wr) {
+    float p = b * in[idx] + c * dE;
+    if (wr) out[idx] = p;
+    return p;
+}
+
+// ----------------------------------------------------------------- TFSF (main.cu:369-758)
+
+// Linear interpolation of the 1D line at projected distance d from the TFSF
+// corner: e_inc at integer offsets (main.cu:387-391), h_inc half-cell shifted
+// (main.cu:420-424).
+__device__ inline float inc_interp(const TfsfDev &t, bool hline, float di, float dj, float dk) {
+    float d = (di * t.s[0] + dj * t.s[1] + dk * t.s[2]) * t.dscale;
+    if (hline) d += 0.5f;
+    const int id = (int)d;
+    const float w = d - (float)id;
+    const double *line = hline ? t.h_inc + (t.m0 - 1) : t.e_inc + t.m0;
+    return (1.0f - w) * (float)line[id] + w * (float)line[id + 1];
+}
+
+// Sum of the TFSF corrections to field component c (E if efield, else H) at
+// (i,j,k), accumulated in the reference's CORRECT_X, _Y, _Z order.
+//   H component c on face normal a uses e_inc component b = 3-a-c; it sits at
+//   plane t0[a]-1 (near) or t1[a] (far), spans [t0,t1] along c and [t0,t1)
+//   along b; sign +/- (near/far) when a follows c cyclically, else -/+.
+//   E component c on face normal a uses h_inc component b; it sits on planes
+//   t0[a] and t1[a], spans [t0,t1) along c and [t0,t1] along b; sign -/+ when a
+//   follows c cyclically, else +/-.
+// Applied to H after the H update and to E^n before the E update, as the
+// reference's CORRECT kernels run between UPDATE_h* and UPDATE_e*.
+__device__ inline float tfsf_corr(const TfsfDev &t, bool efield, int c, int i, int j, int k) {
+    const int x[3] = {i, j, k};
+    float r = 0.f;
+#pragma unroll 1
+    for (int a = 0; a < 3; a++) {
+        if (a == c) continue;
+        const int b = 3 - a - c;
+        const int near_plane = efield ? t.t0[a] : t.t0[a] - 1;
+        const bool nr = x[a] == near_plane, fr = x[a] == t.t1[a];
+        if (!nr && !fr) continue;
+        const bool c_ok = efield ? (x[c] >= t.t0[c] && x[c] < t.t1[c]) : (x[c] >= t.t0[c] && x[c] <= t.t1[c]);
+        const bool b_ok = efield ? (x[b] >= t.t0[b] && x[b] <= t.t1[b]) : (x[b] >= t.t0[b] && x[b] < t.t1[b]);
+        if (!c_ok || !b_ok) continue;
+        float dd[3];
+        dd[c] = (float)(x[c] - t.t0[c]) + (efield ? 0.5f : 0.f);
+        dd[b] = (float)(x[b] - t.t0[b]) + (efield ? 0.f : 0.5f);
+        const float span = (float)(t.t1[a] - t.t0[a]);
+        dd[a] = efield ? (nr ? -0.5f : span + 0.5f) : (nr ? 0.f : span);
+        const bool cyc = a == (c + 1) % 3;
+        float sg = (cyc != efield) ? 1.f : -1.f;
+        if (fr) sg = -sg;
+        const float v = inc_interp(t, efield, dd[0], dd[1], dd[2]) * (efield ? t.hf[b] : t.ef[b]);
+        r += sg * (efield ? t.cE : t.cH) * v * t.inv_d[a];
+    }
+    return r;
+}
+
+__device__ inline float tfsf_h(const TfsfDev &t, int c, int i, int j, int k) {
+    if (i < t.t0[0] - 1 || i > t.t1[0] || j < t.t0[1] - 1 || j > t.t1[1] || k < t.t0[2] - 1 || k > t.t1[2]) return 0.f;
+    return tfsf_corr(t, false, c, i, j, k);
+}
+__device__ inline float tfsf_e(const TfsfDev &t, int c, int i, int j, int k) {
+    if (i < t.t0[0] || i > t.t1[0] || j < t.t0[1] || j > t.t1[1] || k < t.t0[2] || k > t.t1[2]) return 0.f;
+    return tfsf_corr(t, true, c, i, j, k);
+}
+
+// ----------------------------------------------------------------- per-cell H update (main.cu:762-1044)
+
+// Neighbour values of E^n needed by the H update at (i,j,k).
+struct ENbr {
+    float ex, ey, ez;        // (i,j,k)
+    float ex_j1, ez_j1;      // (i,j+1,k)
+    float ex_k1, ey_k1;      // (i,j,k+1)
+    float ey_i1, ez_i1;      // (i+1,j,k)
+};
+
+// CPML slab layer of H position x along one axis (near [0,Nc), far Nc+x-F), or -1.
+__device__ inline int h_layer(const CpmlDev &c, int x) {
+    return x < c.Nc ? x : (x >= c.F ? c.Nc + x - c.F : -1);
+}
+// CPML slab layer of E position x (near [0,Nc], far Nc+1+x-F), or -1.  The
+// reference's ez near-y test is j < Ncpml (main.cu:1877); at j == Ncpml the
+// recursion has ce == 0, so psi stays zero and both tests give the same field.
+__device__ inline int e_layer(const CpmlDev &c, int x) {
+    return x <= c.Nc ? x : (x >= c.F ? c.Nc + 1 + x - c.F : -1);
+}
+
+// New H at (i,j,k).  wr: this thread owns the cell and writes psi_H.
+__device__ inline void h_cell(const StepDev &s, int i, int j, int k, const ENbr &e, float hxo, float hyo,
+                              float hzo, bool wr, float &hx, float &hy, float &hz) {
+    const Geom &g = s.g;
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+    const float Chh = s.c.Chh, Che = s.c.Che;
+    const CpmlDev &cx = s.cp[0], &cy = s.cp[1], &cz = s.cp[2];
+    const bool in = i >= 0 && j >= 0 && k >= 0;
+    // psi_H is updated only on the interior i < NX-1, j < NY-1, k < NZ-1 (main.cu:834-1037)
+    const bool pin = in && i < NX - 1 && j < NY - 1 && k < NZ - 1;
+    const int lx = pin ? h_layer(cx, i) : -1, ly = pin ? h_layer(cy, j) : -1, lz = pin ? h_layer(cz, k) : -1;
+    const int Lz = 2 * cz.Nc, Ly = 2 * cy.Nc;
+    // hx: i < NX, j < NY-1, k < NZ-1 (main.cu:829)
+    hx = hxo;
+    if (in && i < NX && j < NY - 1 && k < NZ - 1) {
+        const float dEy = e.ey_k1 - e.ey, dEz = e.ez_j1 - e.ez;
+        float h = Chh * hxo + Che * (dEy * cz.ik_h[k] - dEz * cy.ik_h[j]);
+        if (lz >= 0) h += Che * psi_step(s.psiH_in.p[0], s.psiH_out.p[0], zslab(g, Lz, i, j, lz), cz.bh[lz], cz.ch[lz], dEy, wr);
+        if (ly >= 0) h -= Che * psi_step(s.psiH_in.p[1], s.psiH_out.p[1], yslab(g, Ly, i, ly, k), cy.bh[ly], cy.ch[ly], dEz, wr);
+        if (s.tfsf_on) h += tfsf_h(s.tf, 0, i, j, k);
+        hx = h;
+    }
+    // hy: i < NX-1, j < NY, k < NZ-1 (main.cu:918)
+    hy = hyo;
+    if (in && i < NX - 1 && j < NY && k < NZ - 1) {
+        const float dEz = e.ez_i1 - e.ez, dEx = e.ex_k1 - e.ex;
+        float h = Chh * hyo + Che * (dEz * cx.ik_h[i] - dEx * cz.ik_h[k]);
+        if (lz >= 0) h -= Che * psi_step(s.psiH_in.p[2], s.psiH_out.p[2], zslab(g, Lz, i, j, lz), cz.bh[lz], cz.ch[lz], dEx, wr);
+        if (lx >= 0) h += Che * psi_step(s.psiH_in.p[3], s.psiH_out.p[3], xslab(g, lx, j, k), cx.bh[lx], cx.ch[lx], dEz, wr);
+        if (s.tfsf_on) h += tfsf_h(s.tf, 1, i, j, k);
+        hy = h;
+    }
+    // hz: i < NX-1, j < NY-1, k < NZ (main.cu:1014)
+    hz = hzo;
+    if (in && i < NX - 1 && j < NY - 1 && k < NZ) {
+        const float dEx = e.ex_j1 - e.ex, dEy = e.ey_i1 - e.ey;
+        float h = Chh * hzo + Che * (dEx * cy.ik_h[j] - dEy * cx.ik_h[i]);
+        if (lx >= 0) h -= Che * psi_step(s.psiH_in.p[4], s.psiH_out.p[4], xslab(g, lx, j, k), cx.bh[lx], cx.ch[lx], dEy, wr);
+        if (ly >= 0) h += Che * psi_step(s.psiH_in.p[5], s.psiH_out.p[5], yslab(g, Ly, i, ly, k), cy.bh[ly], cy.ch[ly], dEx, wr);
+        if (s.tfsf_on) h += tfsf_h(s.tf, 2, i, j, k);
+        hz = h;
+    }
+}
+
+// ----------------------------------------------------------------- per-cell E update (main.cu:1048-1912)
+
+// H^{n+1/2} values around (i,j,k) needed by the E update.
+struct HNbr {
+    float hx, hy, hz;        // (i,j,k)
+    float hz_jm, hx_jm;      // (i,j-1,k)
+    float hy_km, hx_km;      // (i,j,k-1)
+    float hz_im, hy_im;      // (i-1,j,k)
+};
+
+__device__ inline void e_cell(const StepDev &s, int i, int j, int k, const HNbr &h, float exo, float eyo,
+                              float ezo, float &ex, float &ey, float &ez) {
+    const Geom &g = s.g;
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+    const float Cee = s.c.Cee, Ceh = s.c.Ceh;
+    const CpmlDev &cx = s.cp[0], &cy = s.cp[1], &cz = s.cp[2];
+    const bool in = i >= 0 && j >= 0 && k >= 0;
+    // psi_E runs on i < NX-1, j < NY-1, k < NZ-1 intersected with each component's domain
+    const bool pin = in && i < NX - 1 && j < NY - 1 && k < NZ - 1;
+    const int lx = pin ? e_layer(cx, i) : -1, ly = pin ? e_layer(cy, j) : -1, lz = pin ? e_layer(cz, k) : -1;
+    const int Lz = 2 * (cz.Nc + 1), Ly = 2 * (cy.Nc + 1);
+    // ex: i < NX-1, 0 < j < NY-1, 0 < k < NZ-1 (main.cu:1172)
+    ex = exo;
+    if (in && i < NX - 1 && j > 0 && j < NY - 1 && k > 0 && k < NZ - 1) {
+        float eo = exo;
+        if (s.tfsf_on) eo += tfsf_e(s.tf, 0, i, j, k);
+        const float dHz = h.hz - h.hz_jm, dHy = h.hy - h.hy_km;
+        float e = Cee * eo + Ceh * (dHz * cy.ik_e[j] - dHy * cz.ik_e[k]);
+        if (lz >= 0) e -= Ceh * psi_step(s.psiE.p[0], s.psiE.p[0], zslab(g, Lz, i, j, lz), cz.be[lz], cz.ce[lz], dHy, true);
+        if (ly >= 0) e += Ceh * psi_step(s.psiE.p[1], s.psiE.p[1], yslab(g, Ly, i, ly, k), cy.be[ly], cy.ce[ly], dHz, true);
+        ex = e;
+    }
+    // ey: 0 < i < NX-1, j < NY-1, 0 < k < NZ-1 (main.cu:1476)
+    ey = eyo;
+    if (in && i > 0 && i < NX - 1 && j < NY - 1 && k > 0 && k < NZ - 1) {
+        float eo = eyo;
+        if (s.tfsf_on) eo += tfsf_e(s.tf, 1, i, j, k);
+        const float dHx = h.hx - h.hx_km, dHz = h.hz - h.hz_im;
+        float e = Cee * eo + Ceh * (dHx * cz.ik_e[k] - dHz * cx.ik_e[i]);
+        if (lz >= 0) e += Ceh * psi_step(s.psiE.p[2], s.psiE.p[2], zslab(g, Lz, i, j, lz), cz.be[lz], cz.ce[lz], dHx, true);
+        if (lx >= 0) e -= Ceh * psi_step(s.psiE.p[3], s.psiE.p[3], xslab(g, lx, j, k), cx.be[lx], cx.ce[lx], dHz, true);
+        ey = e;
+    }
+    // ez: 0 < i < NX-1, 0 < j < NY-1, k < NZ-1 (main.cu:1767)
+    ez = ezo;
+    if (in && i > 0 && i < NX - 1 && j > 0 && j < NY - 1 && k < NZ - 1) {
+        float eo = ezo;
+        if (s.tfsf_on) eo += tfsf_e(s.tf, 2, i, j, k);
+        const float dHy = h.hy - h.hy_im, dHx = h.hx - h.hx_jm;
+        float e = Cee * eo + Ceh * (dHy * cx.ik_e[i] - dHx * cy.ik_e[j]);
+        if (ly >= 0) e -= Ceh * psi_step(s.psiE.p[4], s.psiE.p[4], yslab(g, Ly, i, ly, k), cy.be[ly], cy.ce[ly], dHx, true);
+        if (lx >= 0) e += Ceh * psi_step(s.psiE.p[5], s.psiE.p[5], xslab(g, lx, j, k), cx.be[lx], cx.ce[lx], dHy, true);
+        ez = e;
+    }
+}
+
+// ----------------------------------------------------------------- DFT face sampling (main.cu:35-366)
+
+// Writes the tangential E and H of (i,j,k) into the sample row for every
+// monitor face containing the cell.  Column layout: E half [0, mE), H half at
+// mE_pad + the same offset; per face: component a block then component b.
+__device__ inline void dft_sample_cell(const FaceDev &f, float *row, int i, int j, int k, float ex, float ey,
+                                       float ez, float hx, float hy, float hz) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+        const int axis = (q >> 1) % 3;
+        int n, a, b;
+        if (axis == 0) { n = i; a = j; b = k; }
+        else if (axis == 1) { n = j; a = i; b = k; }
+        else { n = k; a = i; b = j; }
+        if (n != f.plane[q]) continue;
+        a -= f.lo_a[q];
+        b -= f.lo_b[q];
+        if (a < 0 || a >= f.na[q] || b < 0 || b >= f.nb[q]) continue;
+        const long col = f.off[q] + (long)a * f.nb[q] + b;
+        const long blk = (long)f.na[q] * f.nb[q];
+        float ea, eb, ha, hb;
+        if (axis == 0) { ea = ey; eb = ez; ha = hy; hb = hz; }
+        else if (axis == 1) { ea = ez; eb = ex; ha = hz; hb = hx; }
+        else { ea = ex; eb = ey; ha = hx; hb = hy; }
+        row[col] = ea;
+        row[col + blk] = eb;
+        row[f.mE_pad + col] = ha;
+        row[f.mE_pad + col + blk] = hb;
+    }
+}
+
+// ----------------------------------------------------------------- fused step kernel
+
+// Tile: 64 lanes along z (k = kt*63 - 1 + lane) by TJ wave rows along y
+// (j = jt*(TJ-1) - 1 + row); a block marches x over [i0, i1).  The first lane
+// and first row are halo: they recompute H there (from the read-only A
+// buffers) so E at the tile's low faces has its j-1 and k-1 neighbours, and
+// own nothing.  H planes are exchanged through LDS, double buffered, one
+// barrier per x step.
+template <int TJ>
+__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk) {
+    __shared__ float sH[2][3][TJ][65];
+    const Geom &g = s.g;
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int k = blockIdx.x * 63 - 1 + lane;
+    const int j = blockIdx.y * (TJ - 1) - 1 + row;
+    const int i0 = blockIdx.z * ichunk;
+    const int i1 = min(i0 + ichunk, g.NX);
+    const bool own = lane > 0 && row > 0 && k < g.NZ && j < g.NY;
+    // lanes past the grid read a clamped, in-bounds cell; their results are discarded
+    const int kc = min(k, g.NZ - 1), jc = min(j, g.NY - 1);
+    const long sz = g.sz, sxy = g.sxy;
+    const long cidx = (long)jc * sz + kc;   // in-plane offset
+
+    const float *Ex = s.Ein[0], *Ey = s.Ein[1], *Ez = s.Ein[2];
+    const float *Hx = s.Hin[0], *Hy = s.Hin[1], *Hz = s.Hin[2];
+
+    // prologue: H^{n+1/2} at plane i0-1 (values only) and E^n at plane i0
+    float hyP = 0.f, hzP = 0.f;
+    float exc, eyc, ezc;
+    {
+        long p = (long)i0 * sxy + cidx;
+        exc = Ex[p]; eyc = Ey[p]; ezc = Ez[p];
+    }
+    if (i0 > 0) {
+        const int ip = i0 - 1;
+        const long p = (long)ip * sxy + cidx;
+        ENbr e;
+        e.ex = Ex[p]; e.ey = Ey[p]; e.ez = Ez[p];
+        e.ex_j1 = Ex[p + sz]; e.ez_j1 = Ez[p + sz];
+        e.ex_k1 = Ex[p + 1]; e.ey_k1 = Ey[p + 1];
+        e.ey_i1 = eyc; e.ez_i1 = ezc;
+        float hx, hy, hz;
+        h_cell(s, ip, jc == j ? j : g.NY, kc == k ? k : g.NZ, e, Hx[p], Hy[p], Hz[p], false, hx, hy, hz);
+        hyP = hy; hzP = hz;
+    }
+    int buf = 0;
+    for (int i = i0; i < i1; i++) {
+        const long p = (long)i * sxy + cidx;
+        ENbr e;
+        e.ex = exc; e.ey = eyc; e.ez = ezc;
+        const float exn = Ex[p + sxy], eyn = Ey[p + sxy], ezn = Ez[p + sxy];
+        e.ey_i1 = eyn; e.ez_i1 = ezn;
+        e.ex_j1 = Ex[p + sz]; e.ez_j1 = Ez[p + sz];
+        e.ex_k1 = Ex[p + 1]; e.ey_k1 = Ey[p + 1];
+        float hx, hy, hz;
+        h_cell(s, i, jc == j ? j : g.NY, kc == k ? k : g.NZ, e, Hx[p], Hy[p], Hz[p], own, hx, hy, hz);
+        if (own) { s.Hout[0][p] = hx; s.Hout[1][p] = hy; s.Hout[2][p] = hz; }
+        sH[buf][0][row][lane] = hx;
+        sH[buf][1][row][lane] = hy;
+        sH[buf][2][row][lane] = hz;
+        __syncthreads();
+        HNbr h;
+        h.hx = hx; h.hy = hy; h.hz = hz;
+        const int rm = row > 0 ? row - 1 : 0, lm = lane > 0 ? lane - 1 : 0;
+        h.hz_jm = sH[buf][2][rm][lane];
+        h.hx_jm = sH[buf][0][rm][lane];
+        h.hy_km = sH[buf][1][row][lm];
+        h.hx_km = sH[buf][0][row][lm];
+        h.hz_im = hzP; h.hy_im = hyP;
+        if (own) {
+            float ex, ey, ez;
+            e_cell(s, i, j, k, h, exc, eyc, ezc, ex, ey, ez);
+            s.Eout[0][p] = ex; s.Eout[1][p] = ey; s.Eout[2][p] = ez;
+        }
+        hyP = hy; hzP = hz;
+        exc = exn; eyc = eyn; ezc = ezn;
+        buf ^= 1;
+    }
+}
+
+// ----------------------------------------------------------------- split path kernels
+
+__global__ void k_split_h(StepDev s) {
+    const Geom &g = s.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.NZ) return;
+    const long p = gidx(g, i, j, k);
+    ENbr e;
+    e.ex = s.Ein[0][p]; e.ey = s.Ein[1][p]; e.ez = s.Ein[2][p];
+    e.ex_j1 = s.Ein[0][p + g.sz]; e.ez_j1 = s.Ein[2][p + g.sz];
+    e.ex_k1 = s.Ein[0][p + 1]; e.ey_k1 = s.Ein[1][p + 1];
+    e.ey_i1 = s.Ein[1][p + g.sxy]; e.ez_i1 = s.Ein[2][p + g.sxy];
+    float hx, hy, hz;
+    h_cell(s, i, j, k, e, s.Hin[0][p], s.Hin[1][p], s.Hin[2][p], true, hx, hy, hz);
+    s.Hout[0][p] = hx; s.Hout[1][p] = hy; s.Hout[2][p] = hz;
+}
+
+__global__ void k_split_e(StepDev s) {
+    const Geom &g = s.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.NZ) return;
+    const long p = gidx(g, i, j, k);
+    HNbr h;
+    h.hx = s.Hin[0][p]; h.hy = s.Hin[1][p]; h.hz = s.Hin[2][p];
+    h.hz_jm = s.Hin[2][p - g.sz]; h.hx_jm = s.Hin[0][p - g.sz];
+    h.hy_km = s.Hin[1][p - 1]; h.hx_km = s.Hin[0][p - 1];
+    h.hz_im = s.Hin[2][p - g.sxy]; h.hy_im = s.Hin[1][p - g.sxy];
+    float ex, ey, ez;
+    e_cell(s, i, j, k, h, s.Ein[0][p], s.Ein[1][p], s.Ein[2][p], ex, ey, ez);
+    s.Eout[0][p] = ex; s.Eout[1][p] = ey; s.Eout[2][p] = ez;
+}
+
+// CORRECT_X/Y/Z in place over the bounding box of the TFSF planes.
+__global__ void k_tfsf_split(StepDev s) {
+    const TfsfDev &t = s.tf;
+    const int k = t.t0[2] - 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = t.t0[1] - 1 + blockIdx.y;
+    const int i = t.t0[0] - 1 + blockIdx.z;
+    if (k > t.t1[2]) return;
+    const bool onx = i == t.t0[0] - 1 || i == t.t0[0] || i == t.t1[0];
+    const bool ony = j == t.t0[1] - 1 || j == t.t0[1] || j == t.t1[1];
+    const bool onz = k == t.t0[2] - 1 || k == t.t0[2] || k == t.t1[2];
+    if (!(onx || ony || onz)) return;
+    const long p = gidx(s.g, i, j, k);
+    for (int c = 0; c < 3; c++) {
+        float dh = tfsf_h(t, c, i, j, k);
+        if (dh != 0.f) s.Hout[c][p] += dh;
+        float de = tfsf_e(t, c, i, j, k);
+        if (de != 0.f) s.Eout[c][p] += de;
+    }
+}
+
+// One thread per sample column of the E half (H taken at the same cell).
+__global__ void k_dft_sample_split(StepDev s) {
+    const FaceDev &f = s.fd;
+    long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= f.mE) return;
+    int q = 0;
+    while (q < 11 && col >= f.off[q + 1]) q++;
+    const long blk = (long)f.na[q] * f.nb[q];
+    long r = col - f.off[q];
+    const int comp = r >= blk ? 1 : 0;
+    r -= comp * blk;
+    const int a = (int)(r / f.nb[q]) + f.lo_a[q], b = (int)(r % f.nb[q]) + f.lo_b[q];
+    const int axis = (q >> 1) % 3;
+    int i, j, k;
+    if (axis == 0) { i = f.plane[q]; j = a; k = b; }
+    else if (axis == 1) { j = f.plane[q]; i = a; k = b; }
+    else { k = f.plane[q]; i = a; j = b; }
+    const long p = gidx(s.g, i, j, k);
+    // E_a/E_b and H_a/H_b components per normal axis: x (y,z), y (z,x), z (x,y)
+    const int ca = (axis + 1) % 3, cb = (axis + 2) % 3;
+    const int c = comp ? cb : ca;
+    s.sample_row[col] = s.Ein[c][p];
+    s.sample_row[f.mE_pad + col] = s.Hin[c][p];
+}
+
+// ----------------------------------------------------------------- dispersive box (main.cu:1178-1219)
+
+__global__ void k_dispersive(DispDev d) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    const int bk = (int)(idx % d.n[2]);
+    const int bj = (int)((idx / d.n[2]) % d.n[1]);
+    const int bi = (int)(idx / ((long)d.n[2] * d.n[1]));
+    const int i = d.lo[0] + bi, j = d.lo[1] + bj, k = d.lo[2] + bk;
+    const Geom &g = d.g;
+    const long p = gidx(g, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (!d.id[c][idx]) continue;
+        float curl;
+        if (c == 0) curl = (d.H[2][p] - d.H[2][p - g.sz]) * d.ik_e[1][j] - (d.H[1][p] - d.H[1][p - 1]) * d.ik_e[2][k];
+        else if (c == 1) curl = (d.H[0][p] - d.H[0][p - 1]) * d.ik_e[2][k] - (d.H[2][p] - d.H[2][p - g.sxy]) * d.ik_e[0][i];
+        else curl = (d.H[1][p] - d.H[1][p - g.sxy]) * d.ik_e[0][i] - (d.H[0][p] - d.H[0][p - g.sz]) * d.ik_e[1][j];
+        float *const *st = d.st[c];
+        const float en = st[0][idx], en1 = st[1][idx];
+        const float pd = st[2][idx], pd1 = st[3][idx];
+        const float pa = st[4][idx], pa1 = st[5][idx];
+        const float pb = st[6][idx], pb1 = st[7][idx];
+        const float e = d.a0 * curl + d.a1 * en - d.a2 * en1
+                      - (d.b1[0] * pd + d.b2[0] * pd1) - (d.b1[1] * pa + d.b2[1] * pa1) - (d.b1[2] * pb + d.b2[2] * pb1);
+        st[2][idx] = d.p1[0] * pd + d.p2[0] * pd1 + d.p3[0] * e + d.p4[0] * en + d.p5[0] * en1;
+        st[3][idx] = pd;
+        st[4][idx] = d.p1[1] * pa + d.p2[1] * pa1 + d.p3[1] * e + d.p4[1] * en + d.p5[1] * en1;
+        st[5][idx] = pa;
+        st[6][idx] = d.p1[2] * pb + d.p2[2] * pb1 + d.p3[2] * e + d.p4[2] * en + d.p5[2] * en1;
+        st[7][idx] = pb;
+        st[1][idx] = en;
+        st[0][idx] = e;
+        d.E[c][p] = e;
+    }
+}
+
+// Copies E^n from the field into the box state (after a field overwrite) or
+// back (from_field = 0).
+__global__ void k_disp_sync(DispDev d, int from_field) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    const int bk = (int)(idx % d.n[2]);
+    const int bj = (int)((idx / d.n[2]) % d.n[1]);
+    const int bi = (int)(idx / ((long)d.n[2] * d.n[1]));
+    const long p = gidx(d.g, d.lo[0] + bi, d.lo[1] + bj, d.lo[2] + bk);
+    for (int c = 0; c < 3; c++) {
+        if (!d.id[c][idx]) continue;
+        if (from_field) d.st[c][0][idx] = d.E[c][p];
+        else d.E[c][p] = d.st[c][0][idx];
+    }
+}
+
+// ----------------------------------------------------------------- 1D incident line
+
+// One block: h_inc (main.cu:2328-2336), e_inc (main.cu:2280-2326) into the
+// other buffer, first-order Mur ends and the additive source (main.cu:2831-2859).
+__global__ __launch_bounds__(1024) void k_line(LineDev l) {
+    const int L = l.L;
+    for (int i = threadIdx.x; i < L - 1; i += blockDim.x) l.h[i] = l.h[i] - l.ch * (l.e_old[i + 1] - l.e_old[i]);
+    __syncthreads();
+    for (int i = threadIdx.x + 1; i < L - 1; i += blockDim.x) l.e_new[i] = l.e_old[i] - l.ce * (l.h[i] - l.h[i - 1]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double e1 = l.mur[0], e2 = l.mur[1];
+        l.e_new[0] = e1 + l.mur_coef * (l.e_new[1] - l.e_old[0]);
+        l.e_new[L - 1] = e2 + l.mur_coef * (l.e_new[L - 2] - l.e_old[L - 1]);
+        l.mur[0] = l.e_new[1];
+        l.mur[1] = l.e_new[L - 2];
+        double pulse;
+        if (l.static_field) {
+            pulse = (l.t < 1000) ? l.amp * l.t / 1000.0 : l.amp;  // main.cu:2852-2857
+        } else {
+            // powf on a float-converted argument, as the reference (main.cu:2848)
+            const float b = (float)(((double)l.t - l.delay) / l.width);
+            const float b2 = b * b;
+            pulse = l.amp * exp(-(double)b2 / 2.0) * sin(l.w0dt * (double)l.t);
+        }
+        l.e_new[l.src] += pulse;
+    }
+}
+
+// ----------------------------------------------------------------- DFT contraction
+
+struct SlotTimes {
+    int t[256];
+};
+
+// W[half][s][r]: half 0 = E (phase (t+1)dt), 1 = H (phase (t+1/2)dt); r < F cos, r >= F sin.
+__global__ void k_twiddle(double *W, SlotTimes ts, int K, int nvalid, int F, const double *freq, double dt) {
+    const int n = 2 * K * 2 * F;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) {
+        const int r = x % (2 * F);
+        const int sidx = (x / (2 * F)) % K;
+        const int half = x / (2 * F * K);
+        double v = 0.0;
+        if (sidx < nvalid) {
+            const int f = r % F;
+            const double tt = ts.t[sidx] + (half == 0 ? 1.0 : 0.5);
+            const double ph = 2.0 * kPI * tt * dt * freq[f];
+            v = r < F ? cos(ph) : sin(ph);
+        }
+        W[x] = v;
+    }
+}
+
+// acc[r][col] += sum_s X[s][col] * W[half(col)][s][r]; FP64 accumulate.
+// Block: 256 columns x one chunk of 16 rows r.
+constexpr int kRC = 16;
+__global__ __launch_bounds__(256) void k_dft_contract(const float *X, const double *W, double *acc, long mtot,
+                                                      long mE_pad, int K, int F) {
+    __shared__ double sW[256 * kRC];
+    const long col0 = (long)blockIdx.x * 256;
+    const int r0 = blockIdx.y * kRC;
+    const int half = col0 >= mE_pad ? 1 : 0;
+    const double *Wh = W + (long)half * K * 2 * F;
+    for (int x = threadIdx.x; x < K * kRC; x += 256) {
+        const int sidx = x / kRC, rr = x % kRC;
+        sW[x] = (r0 + rr < 2 * F) ? Wh[(long)sidx * 2 * F + r0 + rr] : 0.0;
+    }
+    __syncthreads();
+    const long col = col0 + threadIdx.x;
+    if (col >= mtot) return;
+    double a[kRC];
+#pragma unroll
+    for (int rr = 0; rr < kRC; rr++) a[rr] = 0.0;
+    for (int sidx = 0; sidx < K; sidx++) {
+        const double x = (double)X[(long)sidx * mtot + col];
+#pragma unroll
+        for (int rr = 0; rr < kRC; rr++) a[rr] += x * sW[sidx * kRC + rr];
+    }
+#pragma unroll
+    for (int rr = 0; rr < kRC; rr++)
+        if (r0 + rr < 2 * F) acc[(long)(r0 + rr) * mtot + col] += a[rr];
+}
+
+// Flux through one face at one frequency: sum Re(Ea conj(Hb) - Eb conj(Ha)) * dA (Spectral.cu:737-1132).
+__global__ __launch_bounds__(256) void k_flux(const double *acc, FaceDev fd, long mtot, long mE_pad, int F,
+                                              double dA, double *out) {
+    __shared__ double red[256];
+    const int q = blockIdx.x, f = blockIdx.y;
+    const long blk = (long)fd.na[q] * fd.nb[q];
+    const double *reR = acc + (long)f * mtot, *imR = acc + (long)(F + f) * mtot;
+    double sum = 0.0;
+    for (long c = threadIdx.x; c < blk; c += 256) {
+        const long ca = fd.off[q] + c, cb = ca + blk;
+        const double ear = reR[ca], eai = imR[ca], ebr = reR[cb], ebi = imR[cb];
+        const double har = reR[mE_pad + ca], hai = imR[mE_pad + ca], hbr = reR[mE_pad + cb], hbi = imR[mE_pad + cb];
+        // Re(Ea * conj(Hb)) - Re(Eb * conj(Ha))
+        sum += (ear * hbr + eai * hbi) - (ebr * har + ebi * hai);
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q * F + f] = red[0] * dA;
+}
+
+// ----------------------------------------------------------------- dense <-> padded field copies
+
+__global__ void k_pack(const float *f, Geom g, float *dense, int to_dense) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.NZ) return;
+    const long d = ((long)i * g.NY + j) * g.NZ + k;
+    const long p = gidx(g, i, j, k);
+    if (to_dense) dense[d] = f[p];
+    else const_cast<float *>(f)[p] = dense[d];
+}
+
+}  // namespace
+
+// ================================================================ launch wrappers
+
+hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, hipStream_t st) {
+    const int nkt = (s.g.NZ + 62) / 63;
+    const int nchunk = (s.g.NX + ichunk - 1) / ichunk;
+    switch (tj) {
+        case 4: {
+            dim3 grid(nkt, (s.g.NY + 2) / 3, nchunk), block(64, 4);
+            hipLaunchKernelGGL(k_fused_step<4>, grid, block, 0, st, s, ichunk);
+            break;
+        }
+        case 8: {
+            dim3 grid(nkt, (s.g.NY + 6) / 7, nchunk), block(64, 8);
+            hipLaunchKernelGGL(k_fused_step<8>, grid, block, 0, st, s, ichunk);
+            break;
+        }
+        case 16: {
+            dim3 grid(nkt, (s.g.NY + 14) / 15, nchunk), block(64, 16);
+            hipLaunchKernelGGL(k_fused_step<16>, grid, block, 0, st, s, ichunk);
+            break;
+        }
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_split_h(const StepDev &s, hipStream_t st) {
+    dim3 grid((s.g.NZ + 63) / 64, s.g.NY, s.g.NX), block(64);
+    hipLaunchKernelGGL(k_split_h, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_e(const StepDev &s, hipStream_t st) {
+    dim3 grid((s.g.NZ + 63) / 64, s.g.NY, s.g.NX), block(64);
+    hipLaunchKernelGGL(k_split_e, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_tfsf_split(const StepDev &s, hipStream_t st) {
+    const TfsfDev &t = s.tf;
+    const int nk = t.t1[2] - t.t0[2] + 2, nj = t.t1[1] - t.t0[1] + 2, ni = t.t1[0] - t.t0[0] + 2;
+    dim3 grid((nk + 63) / 64, nj, ni), block(64);
+    hipLaunchKernelGGL(k_tfsf_split, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_line(const LineDev &l, hipStream_t st) {
+    hipLaunchKernelGGL(k_line, dim3(1), dim3(1024), 0, st, l);
+    return hipGetLastError();
+}
+
+hipError_t launch_dispersive(const DispDev &d, hipStream_t st) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dispersive, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t st) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_disp_sync, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, d, from_field);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_sample_split(const StepDev &s, hipStream_t st) {
+    const long n = s.fd.mE;
+    hipLaunchKernelGGL(k_dft_sample_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddle(double *W, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
+                          double dt, hipStream_t st) {
+    if (K > 256) return hipErrorInvalidValue;
+    SlotTimes ts;
+    for (int i = 0; i < 256; i++) ts.t[i] = i < K ? t_slot_host[i] : 0;
+    const int n = 2 * K * 2 * F;
+    hipLaunchKernelGGL(k_twiddle, dim3((n + 255) / 256), dim3(256), 0, st, W, ts, K, nvalid, F, freq, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_contract(const float *X, const double *W, double *acc, long mtot, long mE_pad, int K,
+                               int F, hipStream_t st) {
+    dim3 grid((unsigned)((mtot + 255) / 256), (2 * F + kRC - 1) / kRC), block(256);
+    hipLaunchKernelGGL(k_dft_contract, grid, block, 0, st, X, W, acc, mtot, mE_pad, K, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_reduce(const double *acc, const FaceDev &fd, long mtot, long mE_pad, int F, double dA,
+                              double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_flux, dim3(12, F), dim3(256), 0, st, acc, fd, mtot, mE_pad, F, dA, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_field(const float *f, const Geom &g, float *dense, int to_dense, hipStream_t st) {
+    dim3 grid((g.NZ + 63) / 64, g.NY, g.NX), block(64);
+    hipLaunchKernelGGL(k_pack, grid, block, 0, st, f, g, dense, to_dense);
+    return hipGetLastError();
+}

@@ -1,0 +1,186 @@
+"""Python binding of the MI355X FDTD solver over its C ABI (include/fdtd.h).
+
+The compute path is the HIP library libfdtd_hip.so built in-tree by
+`make -C fdtd-cuda_amd`; importing this module never falls back to any other
+implementation: a missing library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (EX, EY, EZ, HX, HY, HZ, FDTD_OK, GridInfo, Params, face_shape)  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfdtd_hip.so")
+
+EXPORTS = [
+    "fdtd_default_params", "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_get_info",
+    "fdtd_set_materials", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
+    "fdtd_dft_accumulate", "fdtd_step", "fdtd_sync", "fdtd_get_field", "fdtd_set_field",
+    "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libfdtd_hip.so and declares the ABI signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run `make -C fdtd-cuda_amd`)")
+    lib = C.CDLL(path)
+    vp, ip, dp, fp, u8p = C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+    sig = {
+        "fdtd_default_params": (None, [C.POINTER(Params)]),
+        "fdtd_create": (ip, [C.POINTER(Params), C.POINTER(vp)]),
+        "fdtd_destroy": (ip, [vp]),
+        "fdtd_last_error": (C.c_char_p, [vp]),
+        "fdtd_get_info": (ip, [vp, C.POINTER(GridInfo)]),
+        "fdtd_set_materials": (ip, [vp, u8p, u8p, u8p]),
+        "fdtd_get_materials": (ip, [vp, ip, u8p]),
+        "fdtd_update_h": (ip, [vp]),
+        "fdtd_tfsf": (ip, [vp]),
+        "fdtd_update_e": (ip, [vp]),
+        "fdtd_dft_accumulate": (ip, [vp]),
+        "fdtd_step": (ip, [vp, ip]),
+        "fdtd_sync": (ip, [vp]),
+        "fdtd_get_field": (ip, [vp, ip, fp]),
+        "fdtd_set_field": (ip, [vp, ip, fp]),
+        "fdtd_get_line": (ip, [vp, dp, dp]),
+        "fdtd_get_spectrum_setup": (ip, [vp, dp, dp]),
+        "fdtd_get_dft": (ip, [vp, ip, ip, dp]),
+        "fdtd_flux": (ip, [vp, dp, dp, dp, dp]),
+        "fdtd_kernel_timing": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def default_params():
+    p = Params()
+    load_library().fdtd_default_params(C.byref(p))
+    return p
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class FDTDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fdtd error {code}: {msg}")
+        self.code = code
+
+
+class Simulation:
+    """One solver handle (FP32 state on the GPU)."""
+
+    def __init__(self, params):
+        self.lib = load_library()
+        self.h = C.c_void_p()
+        rc = self.lib.fdtd_create(C.byref(params), C.byref(self.h))
+        if rc != FDTD_OK:
+            raise FDTDError(rc, self.lib.fdtd_last_error(None).decode())
+        self.params = params
+        self.info = self.get_info()
+        self.shape = (self.info.NX, self.info.NY, self.info.NZ)
+
+    def _check(self, rc):
+        if rc != FDTD_OK:
+            raise FDTDError(rc, self.lib.fdtd_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.fdtd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_info(self):
+        g = GridInfo()
+        self._check(self.lib.fdtd_get_info(self.h, C.byref(g)))
+        return g
+
+    def set_materials(self, mx, my, mz):
+        arrs = [np.ascontiguousarray(m, dtype=np.uint8).reshape(-1) for m in (mx, my, mz)]
+        self._check(self.lib.fdtd_set_materials(self.h, *[_ptr(a, C.c_uint8) for a in arrs]))
+        self.info = self.get_info()
+
+    def get_materials(self, comp):
+        out = np.empty(self.shape, dtype=np.uint8)
+        self._check(self.lib.fdtd_get_materials(self.h, comp, _ptr(out, C.c_uint8)))
+        return out
+
+    def update_h(self):
+        self._check(self.lib.fdtd_update_h(self.h))
+
+    def tfsf(self):
+        self._check(self.lib.fdtd_tfsf(self.h))
+
+    def update_e(self):
+        self._check(self.lib.fdtd_update_e(self.h))
+
+    def dft_accumulate(self):
+        self._check(self.lib.fdtd_dft_accumulate(self.h))
+
+    def split_step(self, n=1):
+        for _ in range(n):
+            self.update_h()
+            self.tfsf()
+            self.update_e()
+            self.dft_accumulate()
+
+    def step(self, n=1):
+        self._check(self.lib.fdtd_step(self.h, int(n)))
+
+    def sync(self):
+        self._check(self.lib.fdtd_sync(self.h))
+
+    def get_field(self, comp):
+        out = np.empty(self.shape, dtype=np.float32)
+        self._check(self.lib.fdtd_get_field(self.h, comp, _ptr(out, C.c_float)))
+        return out
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(self.shape)
+        self._check(self.lib.fdtd_set_field(self.h, comp, _ptr(a, C.c_float)))
+
+    def get_line(self):
+        L = self.info.inc_length
+        e, h = np.empty(L), np.empty(L)
+        self._check(self.lib.fdtd_get_line(self.h, _ptr(e, C.c_double), _ptr(h, C.c_double)))
+        return e, h
+
+    def spectrum_setup(self):
+        F = self.info.num_freq
+        f, ei = np.empty(F), np.empty(2 * F)
+        self._check(self.lib.fdtd_get_spectrum_setup(self.h, _ptr(f, C.c_double), _ptr(ei, C.c_double)))
+        return f, ei[0::2] + 1j * ei[1::2]
+
+    def get_dft(self, face, comp):
+        na, nb = face_shape(self.info, face)
+        out = np.empty((self.info.num_freq, na, nb, 2))
+        self._check(self.lib.fdtd_get_dft(self.h, face, comp, _ptr(out, C.c_double)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def flux(self):
+        F = self.info.num_freq
+        sca, ab, ext, raw = np.empty(F), np.empty(F), np.empty(F), np.empty(12 * F)
+        self._check(self.lib.fdtd_flux(self.h, *[_ptr(a, C.c_double) for a in (sca, ab, ext, raw)]))
+        return {"sca": sca, "abs": ab, "ext": ext, "raw": raw.reshape(12, F)}
+
+    def kernel_timing(self, enable):
+        ms, n = C.c_double(), C.c_long()
+        self._check(self.lib.fdtd_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,179 @@
+/*
+ * fdtd.h — C ABI of the MI355X-native 3D FDTD plasmonics solver.
+ *
+ * The reference (jbaxt089/FDTD-CUDA) has no library boundary: its time loop
+ * in main.cu:2724-3175 launches 15 kernels with 30-70 raw arguments each and
+ * keeps all state in ~700 process globals (global_var.h:32-815).  This header
+ * replaces that loop body with an opaque handle that keeps the same call
+ * structure:
+ *
+ *     setup                      -> fdtd_create            (SETUP_CONST, MATERIAL_MATRIX,
+ *                                                          SETUP_Drude_CP, SETUP_CPML,
+ *                                                          SETUP_TFSF, ALLOCATE_MEM)
+ *     UPDATE_hx/hy/hz            -> fdtd_update_h          (main.cu:2735-2769)
+ *     UPDATE_h_inc, CORRECT_X/Y/Z,
+ *     UPDATE_e_inc, Mur, source  -> fdtd_tfsf              (main.cu:2791-2859)
+ *     UpdateHydroP*, UPDATE_e*   -> fdtd_update_e          (main.cu:2869-2901)
+ *     ScattAbs (t % 4 == 0), t++ -> fdtd_dft_accumulate    (main.cu:2920-2934)
+ *     the four above, fused      -> fdtd_step
+ *     CalculateAbsScatt          -> fdtd_flux              (Spectral.cu:411-1177)
+ *
+ * Conventions: every entry point returns FDTD_OK (0) or a negative code and
+ * never exits the process.  Host buffers belong to the caller and are copied
+ * in or out; the handle owns all device memory and one HIP stream.  Calls on
+ * one handle must not overlap; different handles are independent.  Arrays
+ * crossing the boundary use the reference's layout: index k + NZ*(j + NY*i)
+ * with z fastest (main.cu:16-19), over the augmented grid NX*NY*NZ.
+ * Device state is FP32; coefficients are computed on the host in FP64.
+ */
+#ifndef FDTD_H_
+#define FDTD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_OK               0
+#define FDTD_ERR_INVALID     -1   /* bad argument or parameter */
+#define FDTD_ERR_ALLOC       -2   /* device or host allocation failed */
+#define FDTD_ERR_DEVICE      -3   /* HIP runtime error */
+#define FDTD_ERR_NONFINITE   -4   /* NaN/Inf found in the fields (divergence) */
+#define FDTD_ERR_UNSUPPORTED -5   /* feature not available for this setup */
+
+/* Field component ids for fdtd_get_field / fdtd_set_field. */
+enum { FDTD_EX = 0, FDTD_EY, FDTD_EZ, FDTD_HX, FDTD_HY, FDTD_HZ };
+
+/*
+ * Simulation parameters.  The first block mirrors the Input.txt tokens that
+ * the hot path consumes (setup.cu:17-58); the second exposes values the
+ * reference hard-codes (defaults equal the reference's hard-coded values).
+ * fdtd_default_params fills every field.
+ */
+typedef struct fdtd_params {
+    /* --- Input.txt mirror --- */
+    int    Tend;               /* steps; the reference loop runs t = 0..Tend (main.cu:2724)    */
+    double dx, dy, dz;         /* cell size in metres; the solver requires dx == dy == dz      */
+    int    NCELLX, NCELLY, NCELLZ;   /* interior cells, before CPML and +1 (setup.cu:377-393)  */
+    int    NcpmlX, NcpmlY, NcpmlZ;   /* CPML thickness in cells                                */
+    int    nano_sphere;        /* 0: none, 1: staircased sphere at the grid centre (setup.cu:219) */
+    double nano_sphere_radius; /* metres                                                       */
+    int    material;           /* 2: silver, 3: gold Drude + 2 CP (setup.cu:1164-1211)         */
+    int    first_medium;       /* background material id; must be <= 1                        */
+    int    NUM_freq;           /* DFT frequencies                                              */
+    int    WL_or_freq;         /* 1: linear in wavelength, 0: linear in frequency (main.cu:2448) */
+    double BandWidth;          /* pulse bandwidth (Hz) (tfsf2.cu:37)                           */
+    int    NONLOCAL;           /* 3: hydrodynamic nonlocal model (setup.cu:995)                */
+    double MAX_AMP;            /* source amplitude (main.cu:2848)                              */
+    double TimeFactor;         /* dt = 0.5*dy/c0/TimeFactor (setup.cu:359)                     */
+    double lam_0;              /* pulse centre wavelength (m); f_0 = c0/lam_0 (setup.cu:62)    */
+    int    StaticField;        /* 1: ramped static source, no DFT (main.cu:2852-2857)          */
+    /* --- values the reference hard-codes --- */
+    int    tfsf_inset;         /* TFSF box inset inside the CPML, 10 (setup.cu:428)            */
+    int    abs_offset;         /* absorption box inside the TFSF box, 5 (setup.cu:435)         */
+    int    sca_offset;         /* scattering box outside the TFSF box, 5 (setup.cu:451)        */
+    double inc_theta, inc_phi, polar_psi;  /* incidence angles in degrees, 0 (tfsf2.cu:11-13)  */
+    int    cpml_exp;           /* polynomial grading order, 3 (setup.cu:481)                   */
+    double kappa_max;          /* 5 (setup.cu:484)                                              */
+    double alpha_max;          /* 0.05 (setup.cu:496)                                           */
+    double sigma_factor;       /* sigma_max = sigma_factor*(m+1)/(Z0*d), 0.8 (setup.cu:491)     */
+    double lam_min, lam_max;   /* DFT wavelength range, 200 nm .. 700 nm (main.cu:2446-2447)    */
+    int    dft_stride;         /* sample every dft_stride steps, 4 (main.cu:2920)               */
+    int    dft_block;          /* K: samples buffered per MFMA contraction                      */
+    int    source_offset;      /* source cell m0 - source_offset, 50 (main.cu:2851)             */
+    double hydro_beta;         /* nonlocal velocity beta (m/s), 1.39e6*sqrt(3/4) (setup.cu:980-983) */
+    double xs_area;            /* cross-section normaliser (m^2); 0 -> pi*r^2 (Spectral.cu:1166) */
+} fdtd_params;
+
+/* Derived grid facts (read-only). */
+typedef struct fdtd_grid_info {
+    int    NX, NY, NZ;         /* augmented grid, interior + 2*Ncpml + 1                       */
+    int    Ncpml[3];
+    double dt;
+    int    tfsf[3];            /* Ntfsf{X,Y,Z}: TFSF box spans [tfsf, N - tfsf]               */
+    int    abs_start[3], abs_end[3];   /* absorption monitor box [start, end)                  */
+    int    sca_start[3], sca_end[3];   /* scattering monitor box [start, end)                  */
+    int    inc_length, m0;     /* 1D incident line (setup.cu:350, tfsf2.cu:29)                */
+    double width, delay, f_0;  /* pulse parameters (tfsf2.cu:37-40)                           */
+    int    num_freq;
+    int    t;                  /* index of the next step                                      */
+    int    hydro;              /* 1 when the hydrodynamic model is on                          */
+    int    n_dispersive[3];    /* E-component cells on the dispersive (ADE) branch             */
+    int    box_lo[3], box_hi[3];       /* bounding box of dispersive cells, [lo, hi)          */
+    int    sample_count;       /* DFT samples taken so far                                     */
+} fdtd_grid_info;
+
+typedef struct fdtd_sim fdtd_sim;
+
+/* Fills *p with the reference's defaults (a 64^3 gold-sphere run). */
+void fdtd_default_params(fdtd_params *p);
+
+/* Setup: SETUP_CONST (setup.cu:347), MATERIAL_MATRIX (setup.cu:135), the
+ * frequency list (main.cu:2446-2463), SETUP_Drude_CP (setup.cu:979),
+ * ALLOCATE_MEM (setup.cu:2396), SETUP_TFSF (tfsf2.cu:6), the update
+ * coefficients (setup.cu:2015-2062), SETUP_CPML (setup.cu:513) and the source
+ * spectrum (main.cu:2629-2644).  Fields start at zero. */
+int  fdtd_create(const fdtd_params *p, fdtd_sim **out);
+int  fdtd_destroy(fdtd_sim *sim);
+
+/* Message for the last failure on this handle (or of the last failed
+ * fdtd_create when sim is NULL).  Never NULL. */
+const char *fdtd_last_error(const fdtd_sim *sim);
+
+int  fdtd_get_info(const fdtd_sim *sim, fdtd_grid_info *info);
+
+/* Replaces the per-component material masks mat_matrix{X,Y,Z}
+ * (setup.cu:135-342).  Each array is NX*NY*NZ ids; an E component takes the
+ * dispersive branch where 1 < id < 6 (main.cu:1175).  Dispersive cells must
+ * lie strictly inside the absorption box. */
+int  fdtd_set_materials(fdtd_sim *sim, const uint8_t *id_x, const uint8_t *id_y,
+                        const uint8_t *id_z);
+/* Copies a mask out (comp 0: mat_matrix, 1..3: X, Y, Z). */
+int  fdtd_get_materials(const fdtd_sim *sim, int comp, uint8_t *out);
+
+/* Split step, same order as the reference loop body.  fdtd_dft_accumulate
+ * ends the step: it samples the DFT monitors when t % dft_stride == 0 and
+ * advances t. */
+int  fdtd_update_h(fdtd_sim *sim);
+int  fdtd_tfsf(fdtd_sim *sim);
+int  fdtd_update_e(fdtd_sim *sim);
+int  fdtd_dft_accumulate(fdtd_sim *sim);
+
+/* Fused fast path: nsteps full steps (H + TFSF + E + DFT). */
+int  fdtd_step(fdtd_sim *sim, int nsteps);
+
+/* Blocks until queued work finished; reports device errors. */
+int  fdtd_sync(fdtd_sim *sim);
+
+/* Field copies, NX*NY*NZ floats in the reference layout. */
+int  fdtd_get_field(fdtd_sim *sim, int comp, float *out);
+int  fdtd_set_field(fdtd_sim *sim, int comp, const float *in);
+
+/* 1D incident line (e_inc, h_inc), inc_length values each. */
+int  fdtd_get_line(fdtd_sim *sim, double *e_inc, double *h_inc);
+
+/* Frequencies (Hz) and the source spectrum E_incident (re, im pairs). */
+int  fdtd_get_spectrum_setup(const fdtd_sim *sim, double *freq, double *e_incident);
+
+/* Accumulated DFT of one monitor face (0..11: box*6 + axis*2 + side, box 0
+ * = scattering, 1 = absorption; axis 0..2 = x..z; side 0 near, 1 far) and
+ * one tangential component (0..3 = E_a, E_b, H_a, H_b with (a,b) the cyclic
+ * pair after the normal axis).  out holds NUM_freq*na*nb (re, im) pairs,
+ * b fastest.  Pending samples are contracted first. */
+int  fdtd_get_dft(fdtd_sim *sim, int face, int comp, double *out);
+
+/* CalculateAbsScatt (Spectral.cu:411-1177) in FP64: per frequency the
+ * scattering, absorption and extinction cross sections (normalised by
+ * xs_area) and the 12 raw face fluxes raw[face*NUM_freq + m].  Any pointer may
+ * be NULL. */
+int  fdtd_flux(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *raw);
+
+/* Device time of the fused step kernel accumulated since the last reset
+ * (HIP events on the solver stream); enable=1 starts recording. */
+int  fdtd_kernel_timing(fdtd_sim *sim, int enable, double *total_ms, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_H_ */

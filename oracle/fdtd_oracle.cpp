@@ -1,0 +1,1071 @@
+// FP64 OpenMP oracle for the FDTD hot path — TEST INFRASTRUCTURE ONLY.
+//
+// This file restates the reference's update equations (jbaxt089/FDTD-CUDA)
+// in double precision on the CPU, following the non-periodic branches of the
+// reference kernels line by line.  It is the checker for the HIP solver and
+// the CPU baseline timed by bench.py; nothing in the product links it.
+//
+// Followed (file:line in the reference):
+//   setup     setup.cu:347-510 (SETUP_CONST), setup.cu:135-342 (MATERIAL_MATRIX),
+//             setup.cu:513-977 (SETUP_CPML_*), setup.cu:979-1378 (SETUP_Drude_CP),
+//             setup.cu:2015-2062 (DEF_UPDATE_COEFF_*), tfsf2.cu:6-56 (SETUP_TFSF),
+//             main.cu:2446-2463 (frequency list), main.cu:2629-2644 (E_incident)
+//   H update  main.cu:762-1044  (UPDATE_hx/hy/hz, non-periodic branch)
+//   1D line   main.cu:2280-2337 (UPDATE_e_inc/h_inc), main.cu:2831-2859 (Mur + source)
+//   TFSF      main.cu:369-758   (CORRECT_Y/X/Z)
+//   hydro     main.cu:1916-2276 (UpdateHydroPx/Py/Pz)
+//   E update  main.cu:1048-1912 (UPDATE_ex/ey/ez, non-periodic branch)
+//   DFT       main.cu:35-366    (ScattAbs)
+//   order     main.cu:2724-2934
+//   flux      Spectral.cu:411-1177 (CalculateAbsScatt)
+//
+// Decisions where the reference is ambiguous or racy (the Q-list):
+//   * Periodic branches are dead in the reference (Periodic_* = 0,
+//     main.cu:2398-2400) and are not restated.
+//   * TFSF trig is evaluated in double; the reference casts angles to float
+//     (main.cu:387), which differs by <= 1e-7 relative.
+//   * The DFT accumulates in FP64 (the reference uses float accumulators,
+//     main.cu:35) and the flux reduction runs in FP64 (the reference uses
+//     float complex, Spectral.cu:415-419).
+//   * bh_x_F uses its own profile; the reference copies bh_z_F into it
+//     (setup.cu:681), identical for cubic cells and equal Ncpml.
+//   * Hydro: every neighbour read inside one E-component update sees the
+//     values from before that update (the reference races on Px_d_n and
+//     NDx_prev, main.cu:1224-1269); the x -> y -> z component order is kept,
+//     so ey and ez see the advanced Px/Py and NDx/NDy (main.cu:1544-1545).
+//     pow(N, 1/3) is evaluated in double (the reference calls powf).
+//   * The source envelope squares (t - delay)/width in float, exactly as the
+//     reference's powf call on a converted argument (main.cu:2848).
+//   * Hydro continuity for ND_z keeps the reference's 0.5*(Vz1 - Vz2) term
+//     without the Nz factors (main.cu:1837); ex/ey keep 0.5*(N*V) (main.cu:1239, 1554).
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+#include "../include/fdtd.h"
+
+namespace {
+
+// Physical constants exactly as define.h:22-29.
+const double PI = 3.1415926535897932384626433832795;
+const double EP0 = 8.85418782E-12;
+const double MU0 = 1.25663706E-06;
+const double C0 = 2.99792458E+08;
+const double Z0 = 376.730313462;
+const double E0 = 1.60217662E-19;
+const double ME = 9.10938356E-31;
+
+typedef std::vector<double> vec;
+
+struct Oracle {
+    fdtd_params P;
+    int NX, NY, NZ;             // augmented
+    int NcX, NcY, NcZ;
+    int cpml_N_X, cpml_F_X, cpml_N_Y, cpml_F_Y, cpml_N_Z, cpml_F_Z;
+    int cpml_x_lim, cpml_y_lim, cpml_z_lim;
+    int NtfsfX, NtfsfY, NtfsfZ;
+    int abs_s[3], abs_e[3], sca_s[3], sca_e[3];
+    double dt, dx, dy, dz;
+    int inc_Length, m0;
+    double width, delay, f_0, d_1D;
+    double inc_theta, inc_phi, polar_psi;
+    int i_0, j_0, k_0;
+    int t;
+    int hydro;
+    double C_E, C_E_1, C_E_2, N_EQ, eps_inf;
+    double d1, d2, d3, d4, d5, dNL;   // one Drude pole
+    double c1[2], c2[2], c3[2], c4[2], c5[2];
+    double Chh, Che, Cee, Ceh;        // uniform media (setup.cu:2015-2062)
+    vec ex, ey, ez, hx, hy, hz;
+    std::vector<int> mat, matX, matY, matZ;
+    vec kedx, kedy, kedz, khdx, khdy, khdz;
+    vec be_x_N, be_x_F, ce_x_N, ce_x_F, bh_x_N, bh_x_F, ch_x_N, ch_x_F;
+    vec be_y_N, be_y_F, ce_y_N, ce_y_F, bh_y_N, bh_y_F, ch_y_N, ch_y_F;
+    vec be_z_N, be_z_F, ce_z_N, ce_z_F, bh_z_N, bh_z_F, ch_z_N, ch_z_F;
+    vec psi_Hx_z_N, psi_Hx_z_F, psi_Hx_y_N, psi_Hx_y_F;
+    vec psi_Hy_z_N, psi_Hy_z_F, psi_Hy_x_N, psi_Hy_x_F;
+    vec psi_Hz_x_N, psi_Hz_x_F, psi_Hz_y_N, psi_Hz_y_F;
+    vec psi_Ex_z_N, psi_Ex_z_F, psi_Ex_y_N, psi_Ex_y_F;
+    vec psi_Ey_z_N, psi_Ey_z_F, psi_Ey_x_N, psi_Ey_x_F;
+    vec psi_Ez_y_N, psi_Ez_y_F, psi_Ez_x_N, psi_Ez_x_F;
+    // Dispersive state (full grid, as the reference: setup.cu:1073-1158)
+    vec e_n[3], e_n_1[3];
+    vec Pcp[3], Pcp_n[3], Pcp_n_1[3];          // 2 poles interleaved, pole index innermost
+    vec Pd[3], Pd_n[3], Pd_n_1[3], Pd_n_2[3];  // 1 Drude pole
+    vec ND[3], ND_prev[3];                      // hydro charge
+    vec e_inc, h_inc;
+    double e1, e2;
+    vec freq;
+    std::vector<std::complex<double>> E_incident;
+    // DFT accumulators: [face][comp][f][a][b] (re, im)
+    std::vector<vec> dft;  // 12*4 arrays of size F*na*nb*2
+    int face_na[12], face_nb[12];
+    long long samples;
+
+    inline size_t I3(int i, int j, int k) const { return (size_t)k + (size_t)NZ * ((size_t)j + (size_t)NY * i); }
+};
+
+thread_local std::string g_err;
+
+int fail(const std::string &m) { g_err = m; return FDTD_ERR_INVALID; }
+
+// ---------------------------------------------------------------- setup
+
+void setup_cpml_axis(int Nc, double d, double dtv, const fdtd_params &P, int F_idx,
+                     vec &be_N, vec &ce_N, vec &be_F, vec &ce_F,
+                     vec &bh_N, vec &ch_N, vec &bh_F, vec &ch_F, vec &ked, vec &khd) {
+    // setup.cu:540-681 (x), 683-830 (y), 832-977 (z)
+    const double m = P.cpml_exp, kmax = P.kappa_max, amax = P.alpha_max;
+    const double smax = P.sigma_factor * (m + 1) / (Z0 * d);
+    be_N.assign(Nc + 1, 0); ce_N.assign(Nc + 1, 0); be_F.assign(Nc + 1, 0); ce_F.assign(Nc + 1, 0);
+    bh_N.assign(Nc, 0); ch_N.assign(Nc, 0); bh_F.assign(Nc, 0); ch_F.assign(Nc, 0);
+    for (int i = 0; i < Nc + 1; i++) {
+        double S = pow((double)(Nc - i) / (double)Nc, m) * smax;
+        double A = amax * pow((double)i / (double)Nc, 1.0);
+        double K = 1 + (kmax - 1) * pow((double)(Nc - i) / (double)Nc, m);
+        be_N[i] = exp(-(S / K + A) * (dtv / EP0));
+        ce_N[i] = (K * A + S == 0.0) ? 0.0 : S * (be_N[i] - 1) / (K * (S + K * A));
+        ked[i] = ked[i] * K;
+    }
+    for (int i = 0; i < Nc + 1; i++) {
+        double S = pow((double)i / (double)Nc, m) * smax;
+        double A = amax * pow((double)(Nc - i) / (double)Nc, 1.0);
+        double K = 1 + (kmax - 1) * pow((double)i / (double)Nc, m);
+        be_F[i] = exp(-(S / K + A) * (dtv / EP0));
+        ce_F[i] = (K * A + S == 0.0) ? 0.0 : S * (be_F[i] - 1) / (K * (S + K * A));
+        ked[i + F_idx] = ked[i + F_idx] * K;
+    }
+    for (int i = 0; i < Nc; i++) {
+        double S = pow(((double)Nc - (double)i - 0.5) / (double)Nc, m) * smax;
+        double A = amax * pow(((double)i + 0.5) / (double)Nc, 1.0);
+        double K = 1 + (kmax - 1) * pow(((double)Nc - (double)i - 0.5) / (double)Nc, m);
+        bh_N[i] = exp(-(S / K + A) * (dtv / EP0));
+        ch_N[i] = (K * A + S == 0.0) ? 0.0 : S * (bh_N[i] - 1) / (K * (S + K * A));
+        khd[i] = khd[i] * K;
+    }
+    for (int i = 0; i < Nc; i++) {
+        double S = pow(((double)i + 0.5) / (double)Nc, m) * smax;
+        double A = amax * pow(((double)Nc - (double)i - 0.5) / (double)Nc, 1.0);
+        double K = 1 + (kmax - 1) * pow(((double)i + 0.5) / (double)Nc, m);
+        bh_F[i] = exp(-(S / K + A) * (dtv / EP0));
+        ch_F[i] = (K * A + S == 0.0) ? 0.0 : S * (bh_F[i] - 1) / (K * (S + K * A));
+        khd[i + F_idx] = khd[i + F_idx] * K;
+    }
+}
+
+double pulse_at(const Oracle &o, int t) {
+    // main.cu:2633-2636 / 2843-2848: MAX_AMP*exp(-powf((t-delay)/width,2)/2)*sin(2 pi f0 t dt)
+    if (o.P.StaticField == 1) {
+        const double StaticBuild = 1000.0;  // main.cu:2344
+        return (t < StaticBuild) ? o.P.MAX_AMP * t / StaticBuild : o.P.MAX_AMP;
+    }
+    float b = (float)(((double)t - o.delay) / o.width);
+    float b2 = b * b;
+    return o.P.MAX_AMP * exp(-(double)b2 / 2.0) * sin(2 * PI * o.f_0 * t * o.dt);
+}
+
+int setup(Oracle &o, const fdtd_params &p) {
+    o.P = p;
+    if (p.dx != p.dy || p.dx != p.dz) return fail("oracle requires cubic cells (dx == dy == dz)");
+    if (p.NCELLX < 4 || p.NCELLY < 4 || p.NCELLZ < 18) return fail("grid too small (NCELLZ > 17 needed for the source cell)");
+    if (p.NcpmlX < 1 || p.NcpmlY < 1 || p.NcpmlZ < 1) return fail("Ncpml must be >= 1");
+    if (p.NUM_freq < 1) return fail("NUM_freq must be >= 1");
+    if (p.first_medium > 1) return fail("first_medium must be <= 1");
+    o.dx = p.dx; o.dy = p.dy; o.dz = p.dz;
+    // SETUP_CONST, setup.cu:347-510
+    o.inc_Length = p.NCELLZ * 6;
+    o.dt = 0.5 * p.dy / C0 / p.TimeFactor;
+    o.NcX = p.NcpmlX; o.NcY = p.NcpmlY; o.NcZ = p.NcpmlZ;
+    int nx = p.NCELLX + 2 * o.NcX, ny = p.NCELLY + 2 * o.NcY, nz = p.NCELLZ + 2 * o.NcZ;
+    o.cpml_N_X = o.NcX; o.cpml_F_X = nx - o.NcX;
+    o.cpml_N_Y = o.NcY; o.cpml_F_Y = ny - o.NcY;
+    o.cpml_N_Z = o.NcZ; o.cpml_F_Z = nz - o.NcZ;
+    o.NX = nx + 1; o.NY = ny + 1; o.NZ = nz + 1;
+    o.cpml_x_lim = o.NX - 1; o.cpml_y_lim = o.NY - 1; o.cpml_z_lim = o.NZ - 1;
+    o.NtfsfX = p.tfsf_inset + o.NcX; o.NtfsfY = p.tfsf_inset + o.NcY; o.NtfsfZ = p.tfsf_inset + o.NcZ;
+    int Nt[3] = {o.NtfsfX, o.NtfsfY, o.NtfsfZ}, Nn[3] = {o.NX, o.NY, o.NZ};
+    for (int a = 0; a < 3; a++) {
+        o.abs_s[a] = Nt[a] + p.abs_offset; o.abs_e[a] = Nn[a] - Nt[a] - p.abs_offset;
+        o.sca_s[a] = Nt[a] - p.sca_offset; o.sca_e[a] = Nn[a] - Nt[a] + p.sca_offset;
+        if (o.sca_s[a] < 1 || o.abs_e[a] - o.abs_s[a] < 2) return fail("monitor boxes do not fit");
+    }
+    size_t N = (size_t)o.NX * o.NY * o.NZ;
+    // MATERIAL_MATRIX, setup.cu:135-248 (masks use the augmented size)
+    o.mat.assign(N, p.first_medium); o.matX.assign(N, p.first_medium);
+    o.matY.assign(N, p.first_medium); o.matZ.assign(N, p.first_medium);
+    if (p.nano_sphere == 1) {
+        int r = (int)(p.nano_sphere_radius / p.dx);
+        double cz = o.NZ / 2, cx = o.NX / 2, cy = o.NY / 2;
+        for (int i = 0; i < o.NX; i++)
+            for (int j = 0; j < o.NY; j++)
+                for (int k = 0; k < o.NZ; k++) {
+                    size_t id = o.I3(i, j, k);
+                    if (pow((i - cx) * (i - cx) + (j - cy) * (j - cy) + (k - cz) * (k - cz), 0.5) < r) o.mat[id] = p.material;
+                    if (pow((i + 0.5 - cx) * (i + 0.5 - cx) + (j - cy) * (j - cy) + (k - cz) * (k - cz), 0.5) < r) o.matX[id] = p.material;
+                    if (pow((i - cx) * (i - cx) + (j + 0.5 - cy) * (j + 0.5 - cy) + (k - cz) * (k - cz), 0.5) < r) o.matY[id] = p.material;
+                    if (pow((i - cx) * (i - cx) + (j - cy) * (j - cy) + (k + 0.5 - cz) * (k + 0.5 - cz), 0.5) < r) o.matZ[id] = p.material;
+                }
+    }
+    // frequency list, main.cu:2446-2463
+    int F = p.NUM_freq;
+    o.freq.assign(F, 0);
+    for (int i = 0; i < F; i++) {
+        if (p.WL_or_freq == 1) o.freq[i] = C0 / (p.lam_min + i * (p.lam_max - p.lam_min) / (F - 1.0));
+        else o.freq[i] = C0 / p.lam_max + i * (C0 / p.lam_min - C0 / p.lam_max) / (F - 1.0);
+    }
+    // SETUP_Drude_CP, setup.cu:979-1378
+    o.hydro = (p.NONLOCAL == 3);
+    double w_D, gamma_d, A_cp[2], phi_cp[2], OM[2], GA[2];
+    if (p.material == 2) {
+        o.eps_inf = 1.4447; w_D = 1.3280e16; gamma_d = 9.1269e13;
+        A_cp[0] = -1.5951; phi_cp[0] = 3.1288; OM[0] = 8.2749e15; GA[0] = 5.1770e15;
+        A_cp[1] = 0.25261; phi_cp[1] = -1.5066; OM[1] = 6.1998e15; GA[1] = 5.4126e14;
+        o.N_EQ = 5.541318196761572e28;
+    } else if (p.material == 3) {
+        o.eps_inf = 1.1431; w_D = 1.3202e16; gamma_d = 1.0805e14;
+        A_cp[0] = 0.26698; phi_cp[0] = -1.2371; OM[0] = 3.8711e15; GA[0] = 4.4642e14;
+        A_cp[1] = 3.0834; phi_cp[1] = -1.0968; OM[1] = 4.1684e15; GA[1] = 2.3555e15;
+        o.N_EQ = 5.476415562682574e28;
+    } else {
+        return fail("material must be 2 (silver) or 3 (gold)");
+    }
+    double dt = o.dt;
+    double sum3 = 0, sum4 = 0, sum5 = 0;
+    for (int i = 0; i < 2; i++) {
+        double a0 = 2.0 * EP0 * A_cp[i] * OM[i] * (OM[i] * cos(phi_cp[i]) - GA[i] * sin(phi_cp[i]));
+        double a1 = -2.0 * EP0 * A_cp[i] * OM[i] * sin(phi_cp[i]);
+        double b0 = GA[i] * GA[i] + OM[i] * OM[i], b1 = 2.0 * GA[i], b2 = 1.0;
+        double C = b2 / (dt * dt) + b1 / (2.0 * dt) + b0 / 4.0;
+        o.c1[i] = (2 * b2 / (dt * dt) - b0 / 2.0) / C;
+        o.c2[i] = (b1 / (2.0 * dt) - b2 / (dt * dt) - b0 / 4.0) / C;
+        o.c3[i] = (a0 / 4.0 + a1 / (2.0 * dt)) / C;
+        o.c4[i] = a0 / (2.0 * C);
+        o.c5[i] = (a0 / 4.0 - a1 / (2.0 * dt)) / C;
+    }
+    double dd = 1.0 / (dt * dt) + gamma_d / (2.0 * dt);
+    o.d1 = 2.0 / (dd * dt * dt);
+    o.d2 = (gamma_d / (2.0 * dt) - 1.0 / (dt * dt)) / dd;
+    if (!o.hydro) {
+        o.d3 = o.d5 = EP0 * w_D * w_D / (4.0 * dd);
+        o.d4 = EP0 * w_D * w_D / (2.0 * dd);
+        o.dNL = 0.0;  // NONLOCAL != 1 in every configuration restated here
+        sum3 = o.d3 + o.c3[0] + o.c3[1];
+        sum4 = o.d4 + o.c4[0] + o.c4[1];
+        sum5 = o.d5 + o.c5[0] + o.c5[1];
+    } else {
+        o.d3 = -1.0 * E0 / ME / dd / dt;
+        o.d4 = 1.0 * E0 / ME / dd / dt;
+        o.d5 = -1.0 / dd;
+        o.dNL = pow(o.N_EQ, 1.0 / 3.0) * p.hydro_beta * p.hydro_beta / dd;
+        sum3 = o.c3[0] + o.c3[1];
+        sum4 = o.c4[0] + o.c4[1];
+        sum5 = o.c5[0] + o.c5[1];
+    }
+    o.C_E = EP0 * o.eps_inf + sum3;
+    o.C_E_1 = -sum4 + EP0 * o.eps_inf;
+    o.C_E_2 = sum5;
+    // SETUP_TFSF, tfsf2.cu:6-56
+    o.inc_phi = p.inc_phi * PI / 180.0; o.inc_theta = p.inc_theta * PI / 180.0; o.polar_psi = p.polar_psi * PI / 180.0;
+    o.d_1D = p.dx;
+    o.m0 = (int)floor(o.inc_Length / 2);
+    o.i_0 = o.NtfsfX; o.j_0 = o.NtfsfY; o.k_0 = o.NtfsfZ;
+    o.width = ceil((6 / (p.BandWidth * 2 * PI)) / dt);
+    o.delay = 6 * o.width;
+    o.f_0 = C0 / p.lam_0;
+    o.e1 = o.e2 = 0;
+    if (o.m0 - p.source_offset < 1) return fail("source cell outside the incident line");
+    // uniform media coefficients, setup.cu:2015-2062 with eps=EP0, mu=MU0, sigma=0
+    o.Chh = 1.0; o.Che = Z0 * dt / MU0; o.Cee = 1.0; o.Ceh = dt / (Z0 * EP0);
+    // allocation, zero fields
+    o.ex.assign(N, 0); o.ey.assign(N, 0); o.ez.assign(N, 0);
+    o.hx.assign(N, 0); o.hy.assign(N, 0); o.hz.assign(N, 0);
+    for (int c = 0; c < 3; c++) {
+        o.e_n[c].assign(N, 0); o.e_n_1[c].assign(N, 0);
+        o.Pcp[c].assign(2 * N, 0); o.Pcp_n[c].assign(2 * N, 0); o.Pcp_n_1[c].assign(2 * N, 0);
+        o.Pd[c].assign(N, 0); o.Pd_n[c].assign(N, 0); o.Pd_n_1[c].assign(N, 0); o.Pd_n_2[c].assign(N, 0);
+        if (o.hydro) { o.ND[c].assign(N, 0); o.ND_prev[c].assign(N, 0); }
+    }
+    o.e_inc.assign(o.inc_Length, 0); o.h_inc.assign(o.inc_Length, 0);
+    // SETUP_CPML, setup.cu:513-977
+    o.kedx.assign(o.NX, p.dx); o.khdx.assign(o.NX, p.dx);
+    o.kedy.assign(o.NY, p.dy); o.khdy.assign(o.NY, p.dy);
+    o.kedz.assign(o.NZ, p.dz); o.khdz.assign(o.NZ, p.dz);
+    setup_cpml_axis(o.NcX, p.dx, dt, p, o.cpml_F_X, o.be_x_N, o.ce_x_N, o.be_x_F, o.ce_x_F, o.bh_x_N, o.ch_x_N, o.bh_x_F, o.ch_x_F, o.kedx, o.khdx);
+    setup_cpml_axis(o.NcY, p.dy, dt, p, o.cpml_F_Y, o.be_y_N, o.ce_y_N, o.be_y_F, o.ce_y_F, o.bh_y_N, o.ch_y_N, o.bh_y_F, o.ch_y_F, o.kedy, o.khdy);
+    setup_cpml_axis(o.NcZ, p.dz, dt, p, o.cpml_F_Z, o.be_z_N, o.ce_z_N, o.be_z_F, o.ce_z_F, o.bh_z_N, o.ch_z_N, o.bh_z_F, o.ch_z_F, o.kedz, o.khdz);
+    size_t zH = (size_t)o.NX * o.NY * o.NcZ, yH = (size_t)o.NX * o.NcY * o.NZ, xH = (size_t)o.NcX * o.NY * o.NZ;
+    size_t zE = (size_t)o.NX * o.NY * (o.NcZ + 1), yE = (size_t)o.NX * (o.NcY + 1) * o.NZ, xE = (size_t)(o.NcX + 1) * o.NY * o.NZ;
+    o.psi_Hx_z_N.assign(zH, 0); o.psi_Hx_z_F.assign(zH, 0); o.psi_Hx_y_N.assign(yH, 0); o.psi_Hx_y_F.assign(yH, 0);
+    o.psi_Hy_z_N.assign(zH, 0); o.psi_Hy_z_F.assign(zH, 0); o.psi_Hy_x_N.assign(xH, 0); o.psi_Hy_x_F.assign(xH, 0);
+    o.psi_Hz_x_N.assign(xH, 0); o.psi_Hz_x_F.assign(xH, 0); o.psi_Hz_y_N.assign(yH, 0); o.psi_Hz_y_F.assign(yH, 0);
+    o.psi_Ex_z_N.assign(zE, 0); o.psi_Ex_z_F.assign(zE, 0); o.psi_Ex_y_N.assign(yE, 0); o.psi_Ex_y_F.assign(yE, 0);
+    o.psi_Ey_z_N.assign(zE, 0); o.psi_Ey_z_F.assign(zE, 0); o.psi_Ey_x_N.assign(xE, 0); o.psi_Ey_x_F.assign(xE, 0);
+    o.psi_Ez_y_N.assign(yE, 0); o.psi_Ez_y_F.assign(yE, 0); o.psi_Ez_x_N.assign(xE, 0); o.psi_Ez_x_F.assign(xE, 0);
+    // source spectrum, main.cu:2629-2644 (t < Tend)
+    o.E_incident.assign(F, 0);
+    for (int t = 0; t < p.Tend; t++) {
+        double pl = pulse_at(o, t);
+        for (int m = 0; m < F; m++) o.E_incident[m] += pl * std::exp(std::complex<double>(0, 2 * PI * t * dt * o.freq[m]));
+    }
+    // DFT faces: q = box*6 + axis*2 + side; a/b are the in-plane axes in cyclic order
+    o.dft.assign(48, vec());
+    for (int q = 0; q < 12; q++) {
+        int box = q / 6, axis = (q / 2) % 3;
+        const int *s = box ? o.abs_s : o.sca_s, *e = box ? o.abs_e : o.sca_e;
+        int a = (axis + 1) % 3, b = (axis + 2) % 3;
+        // the in-plane (outer, inner) order follows the reference's arrays: x-face (j,k), y-face (i,k), z-face (i,j)
+        int outer = axis == 0 ? 1 : 0, inner = axis == 2 ? 1 : 2;
+        (void)a; (void)b;
+        o.face_na[q] = e[outer] - s[outer];
+        o.face_nb[q] = e[inner] - s[inner];
+        for (int c = 0; c < 4; c++) o.dft[q * 4 + c].assign((size_t)F * o.face_na[q] * o.face_nb[q] * 2, 0.0);
+    }
+    o.t = 0;
+    o.samples = 0;
+    return FDTD_OK;
+}
+
+// ---------------------------------------------------------------- H update, main.cu:762-1044
+
+void update_h(Oracle &o) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double dx = o.dx, dy = o.dy, dz = o.dz, Chh = o.Chh, Che = o.Che;
+    const int NcY = o.NcY, NcZ = o.NcZ;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+            for (int k = 0; k < NZ; k++) {
+                size_t id = o.I3(i, j, k);
+                // UPDATE_hx, main.cu:829-855
+                if (i < NX && j < NY - 1 && k < NZ - 1) {
+                    double curl = (o.ey[o.I3(i, j, k + 1)] - o.ey[id]) / o.khdz[k] - (o.ez[o.I3(i, j + 1, k)] - o.ez[id]) / o.khdy[j];
+                    o.hx[id] = Chh * o.hx[id] + Che * curl;
+                    if (k < o.cpml_N_Z && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                        size_t p = (size_t)k + NcZ * ((size_t)j + (size_t)NY * i);
+                        o.psi_Hx_z_N[p] = o.bh_z_N[k] * o.psi_Hx_z_N[p] + o.ch_z_N[k] * (o.ey[o.I3(i, j, k + 1)] - o.ey[id]) / dz;
+                        o.hx[id] += Che * o.psi_Hx_z_N[p];
+                    }
+                    if (k >= o.cpml_F_Z && j < o.cpml_y_lim && i < o.cpml_x_lim) {
+                        int k2 = k - o.cpml_F_Z;
+                        size_t p = (size_t)k2 + NcZ * ((size_t)j + (size_t)NY * i);
+                        o.psi_Hx_z_F[p] = o.bh_z_F[k2] * o.psi_Hx_z_F[p] + o.ch_z_F[k2] * (o.ey[o.I3(i, j, k + 1)] - o.ey[id]) / dz;
+                        o.hx[id] += Che * o.psi_Hx_z_F[p];
+                    }
+                    if (j < o.cpml_N_Y && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                        size_t p = (size_t)k + NZ * ((size_t)j + (size_t)NcY * i);
+                        o.psi_Hx_y_N[p] = o.bh_y_N[j] * o.psi_Hx_y_N[p] + o.ch_y_N[j] * (o.ez[o.I3(i, j + 1, k)] - o.ez[id]) / dy;
+                        o.hx[id] -= Che * o.psi_Hx_y_N[p];
+                    }
+                    if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                        int j2 = j - o.cpml_F_Y;
+                        size_t p = (size_t)k + NZ * ((size_t)j2 + (size_t)NcY * i);
+                        o.psi_Hx_y_F[p] = o.bh_y_F[j2] * o.psi_Hx_y_F[p] + o.ch_y_F[j2] * (o.ez[o.I3(i, j + 1, k)] - o.ez[id]) / dy;
+                        o.hx[id] -= Che * o.psi_Hx_y_F[p];
+                    }
+                }
+                // UPDATE_hy, main.cu:918-943
+                if (i < NX - 1 && j < NY && k < NZ - 1) {
+                    double curl = (o.ez[o.I3(i + 1, j, k)] - o.ez[id]) / o.khdx[i] - (o.ex[o.I3(i, j, k + 1)] - o.ex[id]) / o.khdz[k];
+                    o.hy[id] = Chh * o.hy[id] + Che * curl;
+                    if (k < o.cpml_N_Z && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        size_t p = (size_t)k + NcZ * ((size_t)j + (size_t)NY * i);
+                        o.psi_Hy_z_N[p] = o.bh_z_N[k] * o.psi_Hy_z_N[p] + o.ch_z_N[k] * (o.ex[o.I3(i, j, k + 1)] - o.ex[id]) / dz;
+                        o.hy[id] -= Che * o.psi_Hy_z_N[p];
+                    }
+                    if (k >= o.cpml_F_Z && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        int k2 = k - o.cpml_F_Z;
+                        size_t p = (size_t)k2 + NcZ * ((size_t)j + (size_t)NY * i);
+                        o.psi_Hy_z_F[p] = o.bh_z_F[k2] * o.psi_Hy_z_F[p] + o.ch_z_F[k2] * (o.ex[o.I3(i, j, k + 1)] - o.ex[id]) / dz;
+                        o.hy[id] -= Che * o.psi_Hy_z_F[p];
+                    }
+                    if (i < o.cpml_N_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        size_t p = o.I3(i, j, k);
+                        o.psi_Hy_x_N[p] = o.bh_x_N[i] * o.psi_Hy_x_N[p] + o.ch_x_N[i] * (o.ez[o.I3(i + 1, j, k)] - o.ez[id]) / dx;
+                        o.hy[id] += Che * o.psi_Hy_x_N[p];
+                    }
+                    if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        int i2 = i - o.cpml_F_X;
+                        size_t p = o.I3(i2, j, k);
+                        o.psi_Hy_x_F[p] = o.bh_x_F[i2] * o.psi_Hy_x_F[p] + o.ch_x_F[i2] * (o.ez[o.I3(i + 1, j, k)] - o.ez[id]) / dx;
+                        o.hy[id] += Che * o.psi_Hy_x_F[p];
+                    }
+                }
+                // UPDATE_hz, main.cu:1014-1038
+                if (i < NX - 1 && j < NY - 1 && k < NZ) {
+                    double curl = (o.ex[o.I3(i, j + 1, k)] - o.ex[id]) / o.khdy[j] - (o.ey[o.I3(i + 1, j, k)] - o.ey[id]) / o.khdx[i];
+                    o.hz[id] = Chh * o.hz[id] + Che * curl;
+                    if (i < o.cpml_N_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        size_t p = o.I3(i, j, k);
+                        o.psi_Hz_x_N[p] = o.bh_x_N[i] * o.psi_Hz_x_N[p] + o.ch_x_N[i] * (o.ey[o.I3(i + 1, j, k)] - o.ey[id]) / dx;
+                        o.hz[id] -= Che * o.psi_Hz_x_N[p];
+                    }
+                    if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        int i2 = i - o.cpml_F_X;
+                        size_t p = o.I3(i2, j, k);
+                        o.psi_Hz_x_F[p] = o.bh_x_F[i2] * o.psi_Hz_x_F[p] + o.ch_x_F[i2] * (o.ey[o.I3(i + 1, j, k)] - o.ey[id]) / dx;
+                        o.hz[id] -= Che * o.psi_Hz_x_F[p];
+                    }
+                    if (j < o.cpml_N_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                        size_t p = (size_t)k + NZ * ((size_t)j + (size_t)NcY * i);
+                        o.psi_Hz_y_N[p] = o.bh_y_N[j] * o.psi_Hz_y_N[p] + o.ch_y_N[j] * (o.ex[o.I3(i, j + 1, k)] - o.ex[id]) / dy;
+                        o.hz[id] += Che * o.psi_Hz_y_N[p];
+                    }
+                    if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                        int j2 = j - o.cpml_F_Y;
+                        size_t p = (size_t)k + NZ * ((size_t)j2 + (size_t)NcY * i);
+                        o.psi_Hz_y_F[p] = o.bh_y_F[j2] * o.psi_Hz_y_F[p] + o.ch_y_F[j2] * (o.ex[o.I3(i, j + 1, k)] - o.ex[id]) / dy;
+                        o.hz[id] += Che * o.psi_Hz_y_F[p];
+                    }
+                }
+            }
+}
+
+// ---------------------------------------------------------------- TFSF, main.cu:369-758
+
+struct IncGeom {
+    double sx, sy, sz;  // propagation direction components
+    double ex_f, ey_f, ez_f, hx_f, hy_f, hz_f;  // polarisation factors
+};
+
+IncGeom inc_geom(const Oracle &o) {
+    IncGeom g;
+    double st = sin(o.inc_theta), ct = cos(o.inc_theta), sp = sin(o.inc_phi), cp = cos(o.inc_phi);
+    double sps = sin(o.polar_psi), cps = cos(o.polar_psi);
+    g.sx = st * cp; g.sy = st * sp; g.sz = ct;
+    g.ex_f = cps * sp - sps * ct * cp;
+    g.ey_f = -cps * cp - sps * ct * sp;
+    g.ez_f = sps * st;
+    g.hx_f = sps * sp + cps * ct * cp;
+    g.hy_f = -sps * cp + cps * ct * sp;
+    g.hz_f = -cps * st;
+    return g;
+}
+
+inline double e_interp(const Oracle &o, const IncGeom &g, double di, double dj, double dk) {
+    double d = di * g.sx + dj * g.sy + dk * g.sz;
+    d = (o.dx / o.d_1D) * d;
+    double dp = d - (int)d;
+    return (1 - dp) * o.e_inc[o.m0 + (int)d] + dp * o.e_inc[o.m0 + (int)d + 1];
+}
+inline double h_interp(const Oracle &o, const IncGeom &g, double di, double dj, double dk) {
+    double d = di * g.sx + dj * g.sy + dk * g.sz;
+    d = (o.dx / o.d_1D) * d;
+    double d2 = d + 0.5;
+    double dp = d2 - (int)d2;
+    return (1 - dp) * o.h_inc[o.m0 - 1 + (int)d2] + dp * o.h_inc[o.m0 + (int)d2];
+}
+
+void correct(Oracle &o) {
+    const IncGeom g = inc_geom(o);
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ, tX = o.NtfsfX, tY = o.NtfsfY, tZ = o.NtfsfZ;
+    const double i0 = o.i_0, j0 = o.j_0, k0 = o.k_0, dx = o.dx, dy = o.dy, dz = o.dz;
+    const double Che = o.Che, Ceh = o.Ceh;
+    // CORRECT_X (main.cu:495-620)
+    for (int j = tY; j <= NY - tY; j++)
+        for (int k = tZ; k <= NZ - tZ; k++) {
+            for (int side = 0; side < 2; side++) {
+                int i = side ? NX - tX : tX;
+                double sg = side ? 1.0 : -1.0;
+                double ez_inc = e_interp(o, g, i - i0, j - j0, k + 0.5 - k0) * g.ez_f;
+                if (k != NZ - tZ) o.hy[o.I3(side ? i : i - 1, j, k)] += sg * Che * ez_inc / dx;
+                double ey_inc = e_interp(o, g, i - i0, j + 0.5 - j0, k - k0) * g.ey_f;
+                if (j != NY - tY) o.hz[o.I3(side ? i : i - 1, j, k)] -= sg * Che * ey_inc / dx;
+                double hz_inc = h_interp(o, g, i + (side ? 0.5 : -0.5) - i0, j + 0.5 - j0, k - k0) * g.hz_f;
+                if (j != NY - tY) o.ey[o.I3(i, j, k)] -= sg * Ceh * hz_inc / dx;
+                double hy_inc = h_interp(o, g, i + (side ? 0.5 : -0.5) - i0, j - j0, k + 0.5 - k0) * g.hy_f;
+                if (k != NZ - tZ) o.ez[o.I3(i, j, k)] += sg * Ceh * hy_inc / dx;
+            }
+        }
+    // CORRECT_Y (main.cu:384-492)
+    for (int i = tX; i <= NX - tX; i++)
+        for (int k = tZ; k <= NZ - tZ; k++) {
+            for (int side = 0; side < 2; side++) {
+                int j = side ? NY - tY : tY;
+                double sg = side ? 1.0 : -1.0;
+                double ez_inc = e_interp(o, g, i - i0, j - j0, k + 0.5 - k0) * g.ez_f;
+                if (k != NZ - tZ) o.hx[o.I3(i, side ? j : j - 1, k)] -= sg * Che * ez_inc / dy;
+                double ex_inc = e_interp(o, g, i - i0 + 0.5, j - j0, k - k0) * g.ex_f;
+                if (i != NX - tX) o.hz[o.I3(i, side ? j : j - 1, k)] += sg * Che * ex_inc / dy;
+                double hz_inc = h_interp(o, g, i + 0.5 - i0, j + (side ? 0.5 : -0.5) - j0, k - k0) * g.hz_f;
+                if (i != NX - tX) o.ex[o.I3(i, j, k)] += sg * Ceh * hz_inc / dy;
+                double hx_inc = h_interp(o, g, i - i0, j + (side ? 0.5 : -0.5) - j0, k + 0.5 - k0) * g.hx_f;
+                if (k != NZ - tZ) o.ez[o.I3(i, j, k)] -= sg * Ceh * hx_inc / dy;
+            }
+        }
+    // CORRECT_Z (main.cu:638-757), far face only when not Periodic_XY (always here)
+    for (int i = tX; i <= NX - tX; i++)
+        for (int j = tY; j <= NY - tY; j++) {
+            for (int side = 0; side < 2; side++) {
+                int k = side ? NZ - tZ : tZ;
+                double sg = side ? 1.0 : -1.0;
+                double ey_inc = e_interp(o, g, i - i0, j + 0.5 - j0, k - k0) * g.ey_f;
+                if (j != NY - tY) o.hx[o.I3(i, j, side ? k : k - 1)] += sg * Che * ey_inc / dz;
+                double ex_inc = e_interp(o, g, i + 0.5 - i0, j - j0, k - k0) * g.ex_f;
+                if (i != NX - tX) o.hy[o.I3(i, j, side ? k : k - 1)] -= sg * Che * ex_inc / dz;
+                double hx_inc = h_interp(o, g, i - i0, j + 0.5 - j0, k + (side ? 0.5 : -0.5) - k0) * g.hx_f;
+                if (j != NY - tY) o.ey[o.I3(i, j, k)] += sg * Ceh * hx_inc / dz;
+                double hy_inc = h_interp(o, g, i + 0.5 - i0, j - j0, k + (side ? 0.5 : -0.5) - k0) * g.hy_f;
+                if (i != NX - tX) o.ex[o.I3(i, j, k)] -= sg * Ceh * hy_inc / dz;
+            }
+        }
+}
+
+// ey uses Cexh in the reference (main.cu:546); media are uniform so Cexh == Ceyh.
+
+void tfsf(Oracle &o) {
+    const int L = o.inc_Length;
+    // UPDATE_h_inc, main.cu:2328-2336
+    for (int i = 0; i < L - 1; i++) o.h_inc[i] = o.h_inc[i] - (Z0 * o.dt / (MU0 * o.d_1D)) * (o.e_inc[i + 1] - o.e_inc[i]);
+    correct(o);
+    // UPDATE_e_inc, main.cu:2280-2326
+    for (int i = 1; i < L - 1; i++) o.e_inc[i] = o.e_inc[i] - (o.dt / (Z0 * EP0 * o.d_1D)) * (o.h_inc[i] - o.h_inc[i - 1]);
+    // Mur ABC + source on the host, main.cu:2831-2859
+    double factor = C0 * o.dt;
+    o.e_inc[0] = o.e1 + ((factor - o.d_1D) / (factor + o.d_1D)) * (o.e_inc[1] - o.e_inc[0]);
+    o.e_inc[L - 1] = o.e2 + ((factor - o.d_1D) / (factor + o.d_1D)) * (o.e_inc[L - 2] - o.e_inc[L - 1]);
+    o.e1 = o.e_inc[1];
+    o.e2 = o.e_inc[L - 2];
+    o.e_inc[o.m0 - o.P.source_offset] += pulse_at(o, o.t);
+}
+
+// ---------------------------------------------------------------- hydro P, main.cu:1916-2276
+
+inline bool hydro_metal(const Oracle &o, int id) { return id > o.P.first_medium && id < 6; }
+
+void hydro_p(Oracle &o) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double IX = 1.0 / o.dx, IY = 1.0 / o.dy, IZ = 1.0 / o.dz;
+    const vec &Px = o.Pd_n[0], &Py = o.Pd_n[1], &Pz = o.Pd_n[2];
+    const vec &Nx = o.ND_prev[0], &Ny = o.ND_prev[1], &Nz = o.ND_prev[2];
+    auto I = [&](int i, int j, int k) { return o.I3(i, j, k); };
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < NX - 1; i++)
+        for (int j = 1; j < NY - 1; j++)
+            for (int k = 1; k < NZ - 1; k++) {
+                size_t id = I(i, j, k);
+                // UpdateHydroPx general branch (main.cu:1938-2038); i == 0 cells are never metal
+                if (hydro_metal(o, o.matX[id])) {
+                    double ND1 = o.N_EQ + Nx[id];
+                    double gd2 = IX * IX * (Px[I(i + 1, j, k)] * Nx[I(i + 1, j, k)] - 2.0 * Px[id] * Nx[id] + Px[I(i - 1, j, k)] * Nx[I(i - 1, j, k)])
+                               + IX * IY * (Py[I(i + 1, j, k)] * Ny[I(i + 1, j, k)] - Py[id] * Ny[id] - Py[I(i + 1, j - 1, k)] * Ny[I(i + 1, j - 1, k)] + Py[I(i, j - 1, k)] * Ny[I(i, j - 1, k)])
+                               + IX * IZ * (Pz[I(i + 1, j, k)] * Nz[I(i + 1, j, k)] - Pz[id] * Nz[id] - Pz[I(i + 1, j, k - 1)] * Nz[I(i + 1, j, k - 1)] + Pz[I(i, j, k - 1)] * Nz[I(i, j, k - 1)]);
+                    double gd = IX * IX * (Px[I(i + 1, j, k)] - 2.0 * Px[id] + Px[I(i - 1, j, k)])
+                              + IX * IY * (Py[I(i + 1, j, k)] - Py[id] - Py[I(i + 1, j - 1, k)] + Py[I(i, j - 1, k)])
+                              + IX * IZ * (Pz[I(i + 1, j, k)] - Pz[id] - Pz[I(i + 1, j, k - 1)] + Pz[I(i, j, k - 1)]);
+                    o.Pd[0][id] = o.d1 * Px[id] + o.d2 * o.Pd_n_1[0][id] + o.d3 * o.ex[id] + o.d4 * o.e_n[0][id]
+                                + o.dNL * (gd + gd2 / o.N_EQ) / pow(ND1, 1.0 / 3.0);
+                }
+                // UpdateHydroPy general branch (main.cu:2065-2164)
+                if (hydro_metal(o, o.matY[id])) {
+                    double ND1 = o.N_EQ + Ny[id];
+                    double gd2 = IY * IY * (Py[I(i, j + 1, k)] * Ny[I(i, j + 1, k)] - 2.0 * Py[id] * Ny[id] + Py[I(i, j - 1, k)] * Ny[I(i, j - 1, k)])
+                               + IX * IY * (Px[I(i, j + 1, k)] * Nx[I(i, j + 1, k)] - Px[id] * Nx[id] - Px[I(i - 1, j + 1, k)] * Nx[I(i - 1, j + 1, k)] + Px[I(i - 1, j, k)] * Nx[I(i - 1, j, k)])
+                               + IY * IZ * (Pz[I(i, j + 1, k)] * Nz[I(i, j + 1, k)] - Pz[id] * Nz[id] - Pz[I(i, j + 1, k - 1)] * Nz[I(i, j + 1, k - 1)] + Pz[I(i, j, k - 1)] * Nz[I(i, j, k - 1)]);
+                    double gd = IY * IY * (Py[I(i, j + 1, k)] - 2.0 * Py[id] + Py[I(i, j - 1, k)])
+                              + IX * IY * (Px[I(i, j + 1, k)] - Px[id] - Px[I(i - 1, j + 1, k)] + Px[I(i - 1, j, k)])
+                              + IY * IZ * (Pz[I(i, j + 1, k)] - Pz[id] - Pz[I(i, j + 1, k - 1)] + Pz[I(i, j, k - 1)]);
+                    o.Pd[1][id] = o.d1 * Py[id] + o.d2 * o.Pd_n_1[1][id] + o.d3 * o.ey[id] + o.d4 * o.e_n[1][id]
+                                + o.dNL * (gd + gd2 / o.N_EQ) / pow(ND1, 1.0 / 3.0);
+                }
+                // UpdateHydroPz general branch (main.cu:2188-2274)
+                if (hydro_metal(o, o.matZ[id])) {
+                    double ND1 = o.N_EQ + Nz[id];
+                    double gd2 = IZ * IZ * (Pz[I(i, j, k + 1)] * Nz[I(i, j, k + 1)] - 2.0 * Pz[id] * Nz[id] + Pz[I(i, j, k - 1)] * Nz[I(i, j, k - 1)])
+                               + IZ * IY * (Py[I(i, j, k + 1)] * Ny[I(i, j, k + 1)] - Py[id] * Ny[id] - Py[I(i, j - 1, k + 1)] * Ny[I(i, j - 1, k + 1)] + Py[I(i, j - 1, k)] * Ny[I(i, j - 1, k)])
+                               + IX * IZ * (Px[I(i, j, k + 1)] * Nx[I(i, j, k + 1)] - Px[id] * Nx[id] - Px[I(i - 1, j, k + 1)] * Nx[I(i - 1, j, k + 1)] + Px[I(i - 1, j, k)] * Nx[I(i - 1, j, k)]);
+                    double gd = IZ * IZ * (Pz[I(i, j, k + 1)] - 2.0 * Pz[id] + Pz[I(i, j, k - 1)])
+                              + IZ * IY * (Py[I(i, j, k + 1)] - Py[id] - Py[I(i, j - 1, k + 1)] + Py[I(i, j - 1, k)])
+                              + IX * IZ * (Px[I(i, j, k + 1)] - Px[id] - Px[I(i - 1, j, k + 1)] + Px[I(i - 1, j, k)]);
+                    o.Pd[2][id] = o.d1 * Pz[id] + o.d2 * o.Pd_n_1[2][id] + o.d3 * o.ez[id] + o.d4 * o.e_n[2][id]
+                                + o.dNL * (gd + gd2 / o.N_EQ) / pow(ND1, 1.0 / 3.0);
+                }
+            }
+}
+
+// ---------------------------------------------------------------- E update, main.cu:1048-1912
+
+// Drude + 2 CP ADE branch of one E component (main.cu:1178-1219; ey 1480-1536; ez 1771-1817).
+inline double ade_update(Oracle &o, int c, size_t id, double curl) {
+    double *e = (c == 0 ? o.ex.data() : c == 1 ? o.ey.data() : o.ez.data());
+    double CP1 = (o.d1 - 1) * o.Pd_n[c][id], CP3 = o.d2 * o.Pd_n_1[c][id];
+    double CP2 = 0, CP4 = 0;
+    for (int n = 0; n < 2; n++) {
+        CP2 += (o.c1[n] - 1) * o.Pcp_n[c][2 * id + n];
+        CP4 += o.c2[n] * o.Pcp_n_1[c][2 * id + n];
+    }
+    o.e_n_1[c][id] = o.e_n[c][id];
+    o.e_n[c][id] = e[id];
+    e[id] = (1 / o.C_E) * (o.dt * curl / Z0 + o.C_E_1 * o.e_n[c][id] - o.C_E_2 * o.e_n_1[c][id] - CP1 - CP2 - CP3 - CP4);
+    for (int n = 0; n < 2; n++)
+        o.Pcp[c][2 * id + n] = o.c1[n] * o.Pcp_n[c][2 * id + n] + o.c2[n] * o.Pcp_n_1[c][2 * id + n] + o.c3[n] * e[id] + o.c4[n] * o.e_n[c][id] + o.c5[n] * o.e_n_1[c][id];
+    o.Pd[c][id] = o.d1 * o.Pd_n[c][id] + o.d2 * o.Pd_n_1[c][id] + o.d3 * e[id] + o.d4 * o.e_n[c][id] + o.d5 * o.e_n_1[c][id];
+    o.Pd_n_1[c][id] = o.Pd_n[c][id];
+    o.Pd_n[c][id] = o.Pd[c][id];
+    for (int n = 0; n < 2; n++) {
+        o.Pcp_n_1[c][2 * id + n] = o.Pcp_n[c][2 * id + n];
+        o.Pcp_n[c][2 * id + n] = o.Pcp[c][2 * id + n];
+    }
+    return e[id];
+}
+
+// Hydro branch, second half (main.cu:1247-1271): E update with the current term, then shifts.
+inline void hydro_e_update(Oracle &o, int c, size_t id, double curl) {
+    double *e = (c == 0 ? o.ex.data() : c == 1 ? o.ey.data() : o.ez.data());
+    double CP2 = 0, CP4 = 0;
+    for (int n = 0; n < 2; n++) {
+        CP2 += (o.c1[n] - 1) * o.Pcp_n[c][2 * id + n];
+        CP4 += o.c2[n] * o.Pcp_n_1[c][2 * id + n];
+    }
+    o.e_n_1[c][id] = o.e_n[c][id];
+    o.e_n[c][id] = e[id];
+    e[id] = (1 / o.C_E) * (o.dt * curl / Z0 + o.dt * o.Pd[c][id] * E0 * (o.N_EQ + o.ND[c][id]) + o.C_E_1 * o.e_n[c][id] - o.C_E_2 * o.e_n_1[c][id] - CP2 - CP4);
+    for (int n = 0; n < 2; n++)
+        o.Pcp[c][2 * id + n] = o.c1[n] * o.Pcp_n[c][2 * id + n] + o.c2[n] * o.Pcp_n_1[c][2 * id + n] + o.c3[n] * e[id] + o.c4[n] * o.e_n[c][id] + o.c5[n] * o.e_n_1[c][id];
+    o.Pd_n_2[c][id] = o.Pd_n_1[c][id];
+    o.Pd_n_1[c][id] = o.Pd_n[c][id];
+    o.Pd_n[c][id] = o.Pd[c][id];
+    for (int n = 0; n < 2; n++) {
+        o.Pcp_n_1[c][2 * id + n] = o.Pcp_n[c][2 * id + n];
+        o.Pcp_n[c][2 * id + n] = o.Pcp[c][2 * id + n];
+    }
+    double hold = o.ND_prev[c][id];
+    o.ND_prev[c][id] = o.ND[c][id];
+    o.ND[c][id] = hold;
+}
+
+// Hydro continuity (main.cu:1224-1239, 1537-1554, 1818-1837), evaluated for every
+// metal cell of one component before any of that component's state moves.
+void hydro_continuity(Oracle &o, int c, const std::vector<int> &mask) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double dt = o.dt, IX = 1.0 / o.dx, NEQ = o.N_EQ;
+    const vec &Px = o.Pd_n[0], &Py = o.Pd_n[1], &Pz = o.Pd_n[2];
+    const vec &Nx = o.ND_prev[0], &Ny = o.ND_prev[1], &Nz = o.ND_prev[2];
+    vec upd(o.ND[c]);
+    auto I = [&](int i, int j, int k) { return o.I3(i, j, k); };
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < NX - 1; i++)
+        for (int j = 1; j < NY - 1; j++)
+            for (int k = 1; k < NZ - 1; k++) {
+                size_t id = I(i, j, k);
+                if (!(mask[id] > 1 && mask[id] < 6)) continue;
+                double Vx1, Vx2, Nx1, Nx2, Vy1, Vy2, Ny1, Ny2, Vz1, Vz2, Nz1, Nz2, r;
+                if (c == 0) {
+                    Vx1 = Px[I(i + 1, j, k)]; Vx2 = Px[I(i - 1, j, k)]; Nx1 = Nx[I(i + 1, j, k)]; Nx2 = Nx[I(i - 1, j, k)];
+                    Vy1 = 0.5 * (Py[I(i + 1, j, k)] + Py[id]); Vy2 = 0.5 * (Py[I(i + 1, j - 1, k)] + Py[I(i, j - 1, k)]);
+                    Ny1 = 0.5 * (Ny[I(i + 1, j, k)] + Ny[id]); Ny2 = 0.5 * (Ny[I(i + 1, j - 1, k)] + Ny[I(i, j - 1, k)]);
+                    Vz1 = 0.5 * (Pz[I(i + 1, j, k)] + Pz[id]); Vz2 = 0.5 * (Pz[I(i + 1, j, k - 1)] + Pz[I(i, j, k - 1)]);
+                    Nz1 = 0.5 * (Nz[I(i + 1, j, k)] + Nz[id]); Nz2 = 0.5 * (Nz[I(i + 1, j, k - 1)] + Nz[I(i, j, k - 1)]);
+                    r = 0.5 * (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + (Nz1 * Vz1 - Nz2 * Vz2)
+                      + (0.5 * (Vx1 - Vx2) + (Vy1 - Vy2) + (Vz1 - Vz2)) * NEQ;
+                } else if (c == 1) {
+                    Vy1 = Py[I(i, j + 1, k)]; Vy2 = Py[I(i, j - 1, k)]; Ny1 = Ny[I(i, j + 1, k)]; Ny2 = Ny[I(i, j - 1, k)];
+                    Vx1 = 0.5 * (Px[I(i, j + 1, k)] + Px[id]); Vx2 = 0.5 * (Px[I(i - 1, j + 1, k)] + Px[I(i - 1, j, k)]);
+                    Nx1 = 0.5 * (Nx[I(i, j + 1, k)] + Nx[id]); Nx2 = 0.5 * (Nx[I(i - 1, j + 1, k)] + Nx[I(i - 1, j, k)]);
+                    Vz1 = 0.5 * (Pz[I(i, j + 1, k)] + Pz[id]); Vz2 = 0.5 * (Pz[I(i, j + 1, k - 1)] + Pz[I(i, j, k - 1)]);
+                    Nz1 = 0.5 * (Nz[I(i, j + 1, k)] + Nz[id]); Nz2 = 0.5 * (Nz[I(i, j + 1, k - 1)] + Nz[I(i, j, k - 1)]);
+                    r = (Nx1 * Vx1 - Nx2 * Vx2) + 0.5 * (Ny1 * Vy1 - Ny2 * Vy2) + (Nz1 * Vz1 - Nz2 * Vz2)
+                      + ((Vx1 - Vx2) + 0.5 * (Vy1 - Vy2) + (Vz1 - Vz2)) * NEQ;
+                } else {
+                    Vz1 = Pz[I(i, j, k + 1)]; Vz2 = Pz[I(i, j, k - 1)]; Nz1 = Nz[I(i, j, k + 1)]; Nz2 = Nz[I(i, j, k - 1)];
+                    Vx1 = 0.5 * (Px[I(i, j, k + 1)] + Px[id]); Vx2 = 0.5 * (Px[I(i - 1, j, k + 1)] + Px[I(i - 1, j, k)]);
+                    Nx1 = 0.5 * (Nx[I(i, j, k + 1)] + Nx[id]); Nx2 = 0.5 * (Nx[I(i - 1, j, k + 1)] + Nx[I(i - 1, j, k)]);
+                    Vy1 = 0.5 * (Py[I(i, j, k + 1)] + Py[id]); Vy2 = 0.5 * (Py[I(i, j - 1, k + 1)] + Py[I(i, j - 1, k)]);
+                    Ny1 = 0.5 * (Ny[I(i, j, k + 1)] + Ny[id]); Ny2 = 0.5 * (Ny[I(i, j - 1, k + 1)] + Ny[I(i, j - 1, k)]);
+                    (void)Nz1; (void)Nz2;
+                    r = (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + 0.5 * (Vz1 - Vz2)
+                      + ((Vx1 - Vx2) + (Vy1 - Vy2) + 0.5 * (Vz1 - Vz2)) * NEQ;
+                }
+                upd[id] = o.ND[c][id] - 2.0 * dt * IX * r;
+            }
+    o.ND[c].swap(upd);
+}
+
+void update_e(Oracle &o) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double dx = o.dx, dy = o.dy, dz = o.dz, Cee = o.Cee, Ceh = o.Ceh;
+    const int NcY = o.NcY, NcZ = o.NcZ;
+    if (o.hydro) hydro_p(o);
+    // UPDATE_ex, main.cu:1172-1327
+    if (o.hydro) hydro_continuity(o, 0, o.matX);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < NX - 1; i++)
+        for (int j = 1; j < NY - 1; j++)
+            for (int k = 1; k < NZ - 1; k++) {
+                size_t id = o.I3(i, j, k);
+                double curl = (o.hz[id] - o.hz[o.I3(i, j - 1, k)]) / o.kedy[j] - (o.hy[id] - o.hy[o.I3(i, j, k - 1)]) / o.kedz[k];
+                int m = o.matX[id];
+                if (m > 1 && m < 6) {
+                    if (!o.hydro) ade_update(o, 0, id, curl);
+                    else hydro_e_update(o, 0, id, curl);
+                } else {
+                    o.ex[id] = Cee * o.ex[id] + Ceh * curl;
+                }
+                if (k < o.cpml_N_Z + 1 && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                    size_t p = (size_t)k + (NcZ + 1) * ((size_t)j + (size_t)NY * i);
+                    o.psi_Ex_z_N[p] = o.be_z_N[k] * o.psi_Ex_z_N[p] + o.ce_z_N[k] * (o.hy[id] - o.hy[o.I3(i, j, k - 1)]) / dz;
+                    o.ex[id] -= Ceh * o.psi_Ex_z_N[p];
+                }
+                if (k >= o.cpml_F_Z && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                    int k2 = k - o.cpml_F_Z;
+                    size_t p = (size_t)k2 + (NcZ + 1) * ((size_t)j + (size_t)NY * i);
+                    o.psi_Ex_z_F[p] = o.be_z_F[k2] * o.psi_Ex_z_F[p] + o.ce_z_F[k2] * (o.hy[id] - o.hy[o.I3(i, j, k - 1)]) / dz;
+                    o.ex[id] -= Ceh * o.psi_Ex_z_F[p];
+                }
+                if (j < o.cpml_N_Y + 1 && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                    size_t p = (size_t)k + NZ * ((size_t)j + (size_t)(NcY + 1) * i);
+                    o.psi_Ex_y_N[p] = o.be_y_N[j] * o.psi_Ex_y_N[p] + o.ce_y_N[j] * (o.hz[id] - o.hz[o.I3(i, j - 1, k)]) / dy;
+                    o.ex[id] += Ceh * o.psi_Ex_y_N[p];
+                }
+                if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                    int j2 = j - o.cpml_F_Y;
+                    size_t p = (size_t)k + NZ * ((size_t)j2 + (size_t)(NcY + 1) * i);
+                    o.psi_Ex_y_F[p] = o.be_y_F[j2] * o.psi_Ex_y_F[p] + o.ce_y_F[j2] * (o.hz[id] - o.hz[o.I3(i, j - 1, k)]) / dy;
+                    o.ex[id] += Ceh * o.psi_Ex_y_F[p];
+                }
+            }
+    // UPDATE_ey, main.cu:1476-1627
+    if (o.hydro) hydro_continuity(o, 1, o.matY);
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < NX - 1; i++)
+        for (int j = 0; j < NY - 1; j++)
+            for (int k = 1; k < NZ - 1; k++) {
+                size_t id = o.I3(i, j, k);
+                double curl = (o.hx[id] - o.hx[o.I3(i, j, k - 1)]) / o.kedz[k] - (o.hz[id] - o.hz[o.I3(i - 1, j, k)]) / o.kedx[i];
+                int m = o.matY[id];
+                if (m > 1 && m < 6) {
+                    if (!o.hydro) ade_update(o, 1, id, curl);
+                    else hydro_e_update(o, 1, id, curl);
+                } else {
+                    o.ey[id] = Cee * o.ey[id] + Ceh * curl;
+                }
+                if (k < o.cpml_N_Z + 1 && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                    size_t p = (size_t)k + (NcZ + 1) * ((size_t)j + (size_t)NY * i);
+                    o.psi_Ey_z_N[p] = o.be_z_N[k] * o.psi_Ey_z_N[p] + o.ce_z_N[k] * (o.hx[id] - o.hx[o.I3(i, j, k - 1)]) / dz;
+                    o.ey[id] += Ceh * o.psi_Ey_z_N[p];
+                }
+                if (k >= o.cpml_F_Z && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                    int k2 = k - o.cpml_F_Z;
+                    size_t p = (size_t)k2 + (NcZ + 1) * ((size_t)j + (size_t)NY * i);
+                    o.psi_Ey_z_F[p] = o.be_z_F[k2] * o.psi_Ey_z_F[p] + o.ce_z_F[k2] * (o.hx[id] - o.hx[o.I3(i, j, k - 1)]) / dz;
+                    o.ey[id] += Ceh * o.psi_Ey_z_F[p];
+                }
+                if (i < o.cpml_N_X + 1 && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                    size_t p = o.I3(i, j, k);
+                    o.psi_Ey_x_N[p] = o.be_x_N[i] * o.psi_Ey_x_N[p] + o.ce_x_N[i] * (o.hz[id] - o.hz[o.I3(i - 1, j, k)]) / dx;
+                    o.ey[id] -= Ceh * o.psi_Ey_x_N[p];
+                }
+                if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                    int i2 = i - o.cpml_F_X;
+                    size_t p = o.I3(i2, j, k);
+                    o.psi_Ey_x_F[p] = o.be_x_F[i2] * o.psi_Ey_x_F[p] + o.ce_x_F[i2] * (o.hz[id] - o.hz[o.I3(i - 1, j, k)]) / dx;
+                    o.ey[id] -= Ceh * o.psi_Ey_x_F[p];
+                }
+            }
+    // UPDATE_ez, main.cu:1767-1904
+    if (o.hydro) hydro_continuity(o, 2, o.matZ);
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < NX - 1; i++)
+        for (int j = 1; j < NY - 1; j++)
+            for (int k = 0; k < NZ - 1; k++) {
+                size_t id = o.I3(i, j, k);
+                double curl = (o.hy[id] - o.hy[o.I3(i - 1, j, k)]) / o.kedx[i] - (o.hx[id] - o.hx[o.I3(i, j - 1, k)]) / o.kedy[j];
+                int m = o.matZ[id];
+                if (m > 1 && m < 6) {
+                    if (!o.hydro) ade_update(o, 2, id, curl);
+                    else hydro_e_update(o, 2, id, curl);
+                } else {
+                    o.ez[id] = Cee * o.ez[id] + Ceh * curl;
+                }
+                if (j < o.cpml_N_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                    size_t p = (size_t)k + NZ * ((size_t)j + (size_t)(NcY + 1) * i);
+                    o.psi_Ez_y_N[p] = o.be_y_N[j] * o.psi_Ez_y_N[p] + o.ce_y_N[j] * (o.hx[id] - o.hx[o.I3(i, j - 1, k)]) / dy;
+                    o.ez[id] -= Ceh * o.psi_Ez_y_N[p];
+                }
+                if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                    int j2 = j - o.cpml_F_Y;
+                    size_t p = (size_t)k + NZ * ((size_t)j2 + (size_t)(NcY + 1) * i);
+                    o.psi_Ez_y_F[p] = o.be_y_F[j2] * o.psi_Ez_y_F[p] + o.ce_y_F[j2] * (o.hx[id] - o.hx[o.I3(i, j - 1, k)]) / dy;
+                    o.ez[id] -= Ceh * o.psi_Ez_y_F[p];
+                }
+                if (i < o.cpml_N_X + 1 && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                    size_t p = o.I3(i, j, k);
+                    o.psi_Ez_x_N[p] = o.be_x_N[i] * o.psi_Ez_x_N[p] + o.ce_x_N[i] * (o.hy[id] - o.hy[o.I3(i - 1, j, k)]) / dx;
+                    o.ez[id] += Ceh * o.psi_Ez_x_N[p];
+                }
+                if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                    int i2 = i - o.cpml_F_X;
+                    size_t p = o.I3(i2, j, k);
+                    o.psi_Ez_x_F[p] = o.be_x_F[i2] * o.psi_Ez_x_F[p] + o.ce_x_F[i2] * (o.hy[id] - o.hy[o.I3(i - 1, j, k)]) / dx;
+                    o.ez[id] += Ceh * o.psi_Ez_x_F[p];
+                }
+            }
+}
+
+// ---------------------------------------------------------------- DFT, main.cu:35-366
+
+void face_comps(int axis, int comps[4]) {
+    // E_a, E_b, H_a, H_b with (a, b) cyclic after the normal axis
+    int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    comps[0] = a; comps[1] = b; comps[2] = 3 + a; comps[3] = 3 + b;
+}
+
+const vec &field(const Oracle &o, int c) {
+    switch (c) {
+        case 0: return o.ex; case 1: return o.ey; case 2: return o.ez;
+        case 3: return o.hx; case 4: return o.hy; default: return o.hz;
+    }
+}
+
+void dft_sample(Oracle &o) {
+    const int F = o.P.NUM_freq;
+    const int t = o.t;
+    std::vector<double> cE(F), sE(F), cH(F), sH(F);
+    for (int f = 0; f < F; f++) {
+        // main.cu:72-75: phases (t+1)dt for E, (t+1/2)dt for H, evaluated in double
+        double pe = 2.0 * PI * (t + 1.0) * o.dt * o.freq[f], ph = 2.0 * PI * (t + 0.5) * o.dt * o.freq[f];
+        cE[f] = cos(pe); sE[f] = sin(pe); cH[f] = cos(ph); sH[f] = sin(ph);
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int q = 0; q < 12; q++) {
+        int box = q / 6, axis = (q / 2) % 3, side = q % 2;
+        const int *s = box ? o.abs_s : o.sca_s, *e = box ? o.abs_e : o.sca_e;
+        int plane = side ? e[axis] - 1 : s[axis];
+        int outer = axis == 0 ? 1 : 0, inner = axis == 2 ? 1 : 2;
+        int comps[4];
+        face_comps(axis, comps);
+        int na = o.face_na[q], nb = o.face_nb[q];
+        for (int a = 0; a < na; a++)
+            for (int b = 0; b < nb; b++) {
+                int idx3[3];
+                idx3[axis] = plane; idx3[outer] = s[outer] + a; idx3[inner] = s[inner] + b;
+                size_t id = o.I3(idx3[0], idx3[1], idx3[2]);
+                for (int c = 0; c < 4; c++) {
+                    double v = field(o, comps[c])[id];
+                    const double *cs = c < 2 ? cE.data() : cH.data();
+                    const double *sn = c < 2 ? sE.data() : sH.data();
+                    double *acc = o.dft[q * 4 + c].data();
+                    for (int f = 0; f < F; f++) {
+                        size_t off = (((size_t)f * na + a) * nb + b) * 2;
+                        acc[off] += v * cs[f];
+                        acc[off + 1] += v * sn[f];
+                    }
+                }
+            }
+    }
+    o.samples++;
+}
+
+void end_step(Oracle &o) {
+    // main.cu:2920: DFT on every dft_stride-th step when StaticField == 0
+    if (o.P.StaticField == 0 && o.t % o.P.dft_stride == 0) dft_sample(o);
+    o.t++;
+}
+
+// ---------------------------------------------------------------- flux, Spectral.cu:411-1177
+
+void flux(const Oracle &o, double *sca, double *abs_, double *ext, double *raw) {
+    const int F = o.P.NUM_freq;
+    std::vector<double> Q(12 * F, 0.0);
+    for (int m = 0; m < F; m++) {
+        double amp = std::abs(o.E_incident[m]);
+        double a2 = amp * amp;
+        for (int q = 0; q < 12; q++) {
+            const int na = o.face_na[q], nb = o.face_nb[q];
+            const double *E0r = o.dft[q * 4 + 0].data(), *E1r = o.dft[q * 4 + 1].data();
+            const double *H0r = o.dft[q * 4 + 2].data(), *H1r = o.dft[q * 4 + 3].data();
+            double acc = 0;
+            for (int a = 0; a < na; a++)
+                for (int b = 0; b < nb; b++) {
+                    size_t off = (((size_t)m * na + a) * nb + b) * 2;
+                    std::complex<double> Ea(E0r[off], E0r[off + 1]), Eb(E1r[off], E1r[off + 1]);
+                    std::complex<double> Ha(H0r[off], H0r[off + 1]), Hb(H1r[off], H1r[off + 1]);
+                    // Re(E_a conj(H_b) - E_b conj(H_a)) dA / |E_inc|^2 with dA = dx*dy on every face (Spectral.cu:740)
+                    acc += std::real(Ea * std::conj(Hb) - Eb * std::conj(Ha)) * o.dx * o.dy;
+                }
+            Q[q * F + m] = acc / a2;
+        }
+    }
+    double area = o.P.xs_area > 0 ? o.P.xs_area : PI * o.P.nano_sphere_radius * o.P.nano_sphere_radius;
+    for (int m = 0; m < F; m++) {
+        // Spectral.cu:1162-1168; faces q = box*6 + axis*2 + side
+        double s = 0, a = 0;
+        for (int ax = 0; ax < 3; ax++) {
+            s += -Q[(0 * 6 + ax * 2 + 0) * F + m] + Q[(0 * 6 + ax * 2 + 1) * F + m];
+            a += -Q[(1 * 6 + ax * 2 + 0) * F + m] + Q[(1 * 6 + ax * 2 + 1) * F + m];
+        }
+        s = s / area;
+        a = -a / area;
+        if (sca) sca[m] = s;
+        if (abs_) abs_[m] = a;
+        if (ext) ext[m] = a + s;
+    }
+    if (raw) for (int i = 0; i < 12 * F; i++) raw[i] = Q[i];
+}
+
+}  // namespace
+
+// ================================================================ C API (tests only)
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+void orc_default_params(fdtd_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->Tend = 1000; p->dx = p->dy = p->dz = 2e-9;
+    p->NCELLX = p->NCELLY = p->NCELLZ = 64;
+    p->NcpmlX = p->NcpmlY = p->NcpmlZ = 10;
+    p->nano_sphere = 1; p->nano_sphere_radius = 20e-9;
+    p->material = 3; p->first_medium = 0;
+    p->NUM_freq = 16; p->WL_or_freq = 1;
+    p->BandWidth = 6.79e14; p->NONLOCAL = 0; p->MAX_AMP = 1.0;
+    p->TimeFactor = 1.7320508075688772; p->lam_0 = 600e-9; p->StaticField = 0;
+    p->tfsf_inset = 10; p->abs_offset = 5; p->sca_offset = 5;
+    p->inc_theta = 0; p->inc_phi = 0; p->polar_psi = 0;
+    p->cpml_exp = 3; p->kappa_max = 5; p->alpha_max = 0.05; p->sigma_factor = 0.8;
+    p->lam_min = 200e-9; p->lam_max = 700e-9;
+    p->dft_stride = 4; p->dft_block = 64; p->source_offset = 50;
+    p->hydro_beta = 1.39e6 * std::sqrt(0.75); p->xs_area = 0;
+}
+
+int orc_create(const fdtd_params *p, void **out) {
+    Oracle *o = new (std::nothrow) Oracle();
+    if (!o) return FDTD_ERR_ALLOC;
+    int rc = setup(*o, *p);
+    if (rc) { delete o; return rc; }
+    *out = o;
+    return FDTD_OK;
+}
+
+void orc_destroy(void *h) { delete (Oracle *)h; }
+
+int orc_info(void *h, fdtd_grid_info *g) {
+    Oracle &o = *(Oracle *)h;
+    std::memset(g, 0, sizeof(*g));
+    g->NX = o.NX; g->NY = o.NY; g->NZ = o.NZ;
+    g->Ncpml[0] = o.NcX; g->Ncpml[1] = o.NcY; g->Ncpml[2] = o.NcZ;
+    g->dt = o.dt;
+    g->tfsf[0] = o.NtfsfX; g->tfsf[1] = o.NtfsfY; g->tfsf[2] = o.NtfsfZ;
+    for (int a = 0; a < 3; a++) {
+        g->abs_start[a] = o.abs_s[a]; g->abs_end[a] = o.abs_e[a];
+        g->sca_start[a] = o.sca_s[a]; g->sca_end[a] = o.sca_e[a];
+    }
+    g->inc_length = o.inc_Length; g->m0 = o.m0;
+    g->width = o.width; g->delay = o.delay; g->f_0 = o.f_0;
+    g->num_freq = o.P.NUM_freq; g->t = o.t; g->hydro = o.hydro;
+    g->sample_count = (int)o.samples;
+    const std::vector<int> *ms[3] = {&o.matX, &o.matY, &o.matZ};
+    for (int c = 0; c < 3; c++) {
+        int n = 0;
+        for (int v : *ms[c]) n += (v > 1 && v < 6);
+        g->n_dispersive[c] = n;
+    }
+    return FDTD_OK;
+}
+
+int orc_set_materials(void *h, const uint8_t *mx, const uint8_t *my, const uint8_t *mz) {
+    Oracle &o = *(Oracle *)h;
+    size_t N = o.ex.size();
+    for (size_t i = 0; i < N; i++) { o.matX[i] = mx[i]; o.matY[i] = my[i]; o.matZ[i] = mz[i]; }
+    return FDTD_OK;
+}
+
+int orc_get_materials(void *h, int comp, uint8_t *out) {
+    Oracle &o = *(Oracle *)h;
+    const std::vector<int> &m = comp == 0 ? o.mat : comp == 1 ? o.matX : comp == 2 ? o.matY : o.matZ;
+    for (size_t i = 0; i < m.size(); i++) out[i] = (uint8_t)m[i];
+    return FDTD_OK;
+}
+
+int orc_update_h(void *h) { update_h(*(Oracle *)h); return FDTD_OK; }
+int orc_tfsf(void *h) { tfsf(*(Oracle *)h); return FDTD_OK; }
+int orc_update_e(void *h) { update_e(*(Oracle *)h); return FDTD_OK; }
+int orc_end_step(void *h) { end_step(*(Oracle *)h); return FDTD_OK; }
+
+int orc_step(void *h, int n) {
+    Oracle &o = *(Oracle *)h;
+    for (int s = 0; s < n; s++) { update_h(o); tfsf(o); update_e(o); end_step(o); }
+    return FDTD_OK;
+}
+
+int orc_get_field(void *h, int comp, double *out) {
+    Oracle &o = *(Oracle *)h;
+    if (comp < 0 || comp > 5) return FDTD_ERR_INVALID;
+    const vec &f = field(o, comp);
+    std::memcpy(out, f.data(), f.size() * sizeof(double));
+    return FDTD_OK;
+}
+
+int orc_set_field(void *h, int comp, const double *in) {
+    Oracle &o = *(Oracle *)h;
+    if (comp < 0 || comp > 5) return FDTD_ERR_INVALID;
+    vec &f = const_cast<vec &>(field(o, comp));
+    std::memcpy(f.data(), in, f.size() * sizeof(double));
+    return FDTD_OK;
+}
+
+int orc_get_line(void *h, double *e, double *hh) {
+    Oracle &o = *(Oracle *)h;
+    std::memcpy(e, o.e_inc.data(), o.e_inc.size() * sizeof(double));
+    std::memcpy(hh, o.h_inc.data(), o.h_inc.size() * sizeof(double));
+    return FDTD_OK;
+}
+
+int orc_get_spectrum_setup(void *h, double *freq, double *einc) {
+    Oracle &o = *(Oracle *)h;
+    for (int m = 0; m < o.P.NUM_freq; m++) {
+        freq[m] = o.freq[m];
+        einc[2 * m] = o.E_incident[m].real();
+        einc[2 * m + 1] = o.E_incident[m].imag();
+    }
+    return FDTD_OK;
+}
+
+int orc_get_dft(void *h, int face, int comp, double *out) {
+    Oracle &o = *(Oracle *)h;
+    if (face < 0 || face > 11 || comp < 0 || comp > 3) return FDTD_ERR_INVALID;
+    const vec &d = o.dft[face * 4 + comp];
+    std::memcpy(out, d.data(), d.size() * sizeof(double));
+    return FDTD_OK;
+}
+
+int orc_flux(void *h, double *sca, double *abs_, double *ext, double *raw) {
+    flux(*(Oracle *)h, sca, abs_, ext, raw);
+    return FDTD_OK;
+}
+
+// Coefficient dump for closed-form checks: order documented in oracle/oracle.py.
+int orc_coeffs(void *h, double *out) {
+    Oracle &o = *(Oracle *)h;
+    int n = 0;
+    out[n++] = o.dt; out[n++] = o.C_E; out[n++] = o.C_E_1; out[n++] = o.C_E_2;
+    out[n++] = o.d1; out[n++] = o.d2; out[n++] = o.d3; out[n++] = o.d4; out[n++] = o.d5; out[n++] = o.dNL;
+    for (int i = 0; i < 2; i++) { out[n++] = o.c1[i]; out[n++] = o.c2[i]; out[n++] = o.c3[i]; out[n++] = o.c4[i]; out[n++] = o.c5[i]; }
+    out[n++] = o.eps_inf; out[n++] = o.N_EQ; out[n++] = o.Che; out[n++] = o.Ceh;
+    return n;
+}
+
+// CPML profile dump: axis 0..2, which 0..3 = be_N, ce_N, be_F, ce_F (Nc+1), 4..7 = bh_N, ch_N, bh_F, ch_F (Nc),
+// 8 = ked (N), 9 = khd (N).
+int orc_cpml(void *h, int axis, int which, double *out) {
+    Oracle &o = *(Oracle *)h;
+    const vec *tab[3][10] = {
+        {&o.be_x_N, &o.ce_x_N, &o.be_x_F, &o.ce_x_F, &o.bh_x_N, &o.ch_x_N, &o.bh_x_F, &o.ch_x_F, &o.kedx, &o.khdx},
+        {&o.be_y_N, &o.ce_y_N, &o.be_y_F, &o.ce_y_F, &o.bh_y_N, &o.ch_y_N, &o.bh_y_F, &o.ch_y_F, &o.kedy, &o.khdy},
+        {&o.be_z_N, &o.ce_z_N, &o.be_z_F, &o.ce_z_F, &o.bh_z_N, &o.ch_z_N, &o.bh_z_F, &o.ch_z_F, &o.kedz, &o.khdz}};
+    if (axis < 0 || axis > 2 || which < 0 || which > 9) return -1;
+    const vec &v = *tab[axis][which];
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return (int)v.size();
+}
+
+int orc_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"
