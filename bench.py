@@ -1,0 +1,194 @@
+"""Throughput benchmark of the FDTD hot path on one MI355X.
+
+Workload (the 256^3 precision-study configuration): 256^3 interior cells at
+dx = 1 nm with a 12-cell CPML on every side (281^3 cells in total), a gold
+dimer of two 30 nm spheres with a 2 nm gap along x (Drude + 2 critical-point
+model, staircased), an x-polarised plane wave travelling along z through a
+TFSF box, and DFT flux monitors at 64 wavelengths from 400 to 1000 nm buffered
+K = 64 samples per contraction.  One benchmark step is one full FDTD time
+step: 1D line, fused H/CPML/TFSF/E update, dispersive ADE, DFT sampling every
+4th step and the buffered DFT contraction every 64 samples.
+
+Prints one JSON line.  value = Mcell-updates/s = 281^3 * steps / wall time of
+the timed steps (inputs resident in HBM).  roofline: the fused step kernel's
+algorithmic bytes per launch / its average launch time from HIP events.
+cpu_baseline: the FP64 OpenMP oracle on a bounded sample of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("fdtd-cuda_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(mod, steps_total=20000):
+    p = mod.default_params()
+    p.NCELLX = p.NCELLY = p.NCELLZ = 256
+    p.NcpmlX = p.NcpmlY = p.NcpmlZ = 12
+    p.dx = p.dy = p.dz = 1e-9
+    p.nano_sphere = 0
+    p.nano_sphere_radius = 30e-9
+    p.material = 3
+    p.Tend = steps_total
+    p.NUM_freq = 64
+    p.lam_min, p.lam_max = 400e-9, 1000e-9
+    p.dft_block = 64
+    p.inc_phi = 90.0            # E along +x, travelling along +z
+    p.TimeFactor = 3 ** 0.5     # dt = 0.5 * Courant limit
+    p.tfsf_inset = 10
+    p.xs_area = 2 * np.pi * (30e-9) ** 2
+    return p
+
+
+def dimer_centres(T):
+    c = T // 2
+    return [(c - 31, c, c), (c + 31, c, c)]
+
+
+# (int)(r/dx) truncates 30e-9/1e-9 = 29.999999999999996 to 29, so the radius
+# passed to the staircase rule is 30.5 nm: 30-cell spheres, 2-cell gap.
+DIMER_RADIUS = 30.5e-9
+
+
+def algorithmic_bytes(info, p):
+    """FP32 bytes the fused step kernel must move per launch.
+
+    Each of the 6 field components is read once and written once (48 B per
+    cell).  Cells inside a CPML slab add 4 psi values (2 H + 2 E components
+    with a derivative along that axis) read and written per axis: 32 B.
+    """
+    N = np.array([info.NX, info.NY, info.NZ], dtype=np.int64)
+    nc = np.array(info.Ncpml[:], dtype=np.int64)
+    cells = int(N.prod())
+    total = 48 * cells
+    for a in range(3):
+        others = int(np.prod([N[b] for b in range(3) if b != a]))
+        total += 32 * 2 * int(nc[a]) * others
+    return total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2048)
+    ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs each simulation on a single MI355X "
+              "(multi-GPU slab decomposition is not implemented); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import fdtd_amd
+
+    p = workload_params(fdtd_amd)
+    sim = fdtd_amd.Simulation(p)
+    info = sim.info
+    T = info.NX
+    for c in dimer_centres(T):
+        sim.add_sphere(c, DIMER_RADIUS)
+    info = sim.get_info()
+    cells = info.NX * info.NY * info.NZ
+
+    sim.step(args.warmup)
+    sim.sync()
+    try:
+        import torch
+        torch.cuda.synchronize()
+    except Exception:
+        torch = None
+    sim.kernel_timing(1)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    if torch is not None:
+        torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    k_ms, k_n = sim.kernel_timing(0)
+
+    value = cells * args.steps / wall / 1e6
+    ms_step = wall * 1e3 / args.steps
+    abytes = algorithmic_bytes(info, p)
+    k_avg_s = (k_ms / 1e3) / max(k_n, 1)
+    achieved = abytes / k_avg_s / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p, info, args.cpu_steps)
+
+    out = {
+        "metric": "Mcell-updates/s",
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "gold dimer 2x30 nm, 2 nm gap, 256^3 interior + 12-cell CPML (281^3), dx=1 nm, "
+                        "TFSF x-pol plane wave, Drude+2CP, 64-wavelength DFT flux monitors (K=64)",
+            "grid": [info.NX, info.NY, info.NZ],
+            "dispersive_cells": list(info.n_dispersive[:]),
+            "tile": {"tj": int(os.environ.get("FDTD_TJ", 8)), "ichunk": int(os.environ.get("FDTD_ICHUNK", 32))},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_fused_step",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": abytes,
+            "kernel_avg_ms": round(k_avg_s * 1e3, 5),
+            "launches": k_n,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(p, info, nsteps):
+    """FP64 OpenMP oracle on the same 281^3 dimer grid: 2 warm-up steps, then nsteps timed."""
+    import oracle as O
+    o = O.Oracle(p)
+    T = info.NX
+    mx, my, mz = (o.get_materials(c) for c in (1, 2, 3))
+    # the dimer masks exactly as the GPU solver built them
+    import fdtd_amd
+    sim = fdtd_amd.Simulation(p)
+    for c in dimer_centres(T):
+        sim.add_sphere(c, DIMER_RADIUS)
+    o.set_materials(sim.get_materials(1), sim.get_materials(2), sim.get_materials(3))
+    sim.close()
+    del mx, my, mz
+    o.step(2)
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    wall = time.perf_counter() - t0
+    cells = info.NX * info.NY * info.NZ
+    return {
+        "value": round(cells * nsteps / wall / 1e6, 2),
+        "unit": "Mcell-updates/s",
+        "cores": o.threads(),
+        "kind": "port",
+        "sample": f"{nsteps} FP64 steps of the same 281^3 dimer grid after 2 warm-up steps ({wall:.1f} s)",
+    }
+
+
+if __name__ == "__main__":
+    main()

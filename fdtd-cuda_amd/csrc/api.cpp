@@ -42,13 +42,15 @@ struct fdtd_sim {
     long nbox = 0;
     uint8_t *disp_id[3] = {};
     float *disp_st[3][8] = {};
+    float *h_vnew[3] = {}, *h_nd[3] = {}, *h_ndp[3] = {};
     int n_disp[3] = {0, 0, 0};
     // DFT
     FaceDev fd{};
     long mtot = 0;
     float *X = nullptr;             // [K][mtot]
     double *acc = nullptr;          // [2F][mtot]
-    double *W = nullptr;            // [2][K][2F]
+    double *W = nullptr;            // [2][K][2F] FP64 twiddles (exact mode)
+    void *Wf = nullptr;             // split-BF16 twiddle fragments (MFMA mode)
     double *freq_d = nullptr;
     double *flux_d = nullptr;       // [12][F]
     int K = 64, slot = 0;
@@ -93,11 +95,15 @@ int dalloc(fdtd_sim *s, T **ptr, size_t count) {
     return FDTD_OK;
 }
 
+// Allocation zero-fills run on the solver stream, so uploads are queued on the
+// same stream (and waited for, the host vector being temporary) to keep them
+// ordered after the fill.
 template <typename T>
 int upload(fdtd_sim *s, T **ptr, const std::vector<T> &v) {
     int rc = dalloc(s, ptr, v.size());
     if (rc) return rc;
-    HIPCHK(s, hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpyAsync(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     return FDTD_OK;
 }
 
@@ -179,13 +185,28 @@ DispDev make_disp(fdtd_sim *s, int parity) {
     d.a2 = (float)(A.C_E2 / CE);
     const double *k[3] = {A.d, A.c[0], A.c[1]};
     for (int q = 0; q < 3; q++) {
-        d.b1[q] = (float)((k[q][0] - 1) * e0 / CE);
-        d.b2[q] = (float)(k[q][1] * e0 / CE);
-        d.p1[q] = (float)k[q][0];
-        d.p2[q] = (float)k[q][1];
+        // (k1 + k2 - 1) vanishes identically for the Drude pole (setup.cu:1300-1301)
+        const double sum = q == 0 ? 0.0 : (k[q][0] - 1.0) + k[q][1];
+        d.e0[q] = (float)(sum * e0 / CE);
+        d.e1[q] = (float)(-k[q][1] * e0 / CE);
+        d.g0[q] = (float)sum;
+        d.g1[q] = (float)(-k[q][1]);
         d.p3[q] = (float)(k[q][2] / e0);
         d.p4[q] = (float)(k[q][3] / e0);
         d.p5[q] = (float)(k[q][4] / e0);
+    }
+    d.hydro = S.hydro ? 1 : 0;
+    if (S.hydro) {
+        const double dt = S.dt, dx = S.P.dx;
+        for (int c = 0; c < 3; c++) { d.vnew[c] = s->h_vnew[c]; d.nd[c] = s->h_nd[c]; d.nd_prev[c] = s->h_ndp[c]; }
+        d.h_d1m1 = (float)(A.d[0] - 1.0);
+        d.h_d3 = (float)A.d[2];
+        d.h_d4 = (float)A.d[3];
+        // dNL / N_EQ^{1/3} = beta^2 / dd with n~ = ND / N_EQ (setup.cu:1362)
+        d.h_nl = (float)(A.dNL / std::cbrt(A.N_EQ));
+        d.h_inv_d = (float)(1.0 / dx);
+        d.h_cont = (float)(2.0 * dt / dx);
+        d.h_cur = (float)(dt * fdtd::kE0 * A.N_EQ / CE);
     }
     return d;
 }
@@ -221,8 +242,8 @@ int build_dispersive(fdtd_sim *s) {
         if (lo[a] <= S.abs_s[a] || hi[a] >= S.abs_e[a] - 1)
             return set_err(s, FDTD_ERR_INVALID, "dispersive cells must lie strictly inside the absorption box");
     }
-    if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "hydrodynamic model not available in this build");
-    for (int a = 0; a < 3; a++) { s->box_lo[a] = lo[a]; s->box_n[a] = hi[a] - lo[a] + 1; }
+    // one halo cell on every side: hydro stencils read neighbours of metal cells
+    for (int a = 0; a < 3; a++) { s->box_lo[a] = lo[a] - 1; s->box_n[a] = hi[a] - lo[a] + 3; }
     s->nbox = (long)s->box_n[0] * s->box_n[1] * s->box_n[2];
     for (int c = 0; c < 3; c++) {
         std::vector<uint8_t> ids(s->nbox, 0);
@@ -230,7 +251,7 @@ int build_dispersive(fdtd_sim *s) {
         for (int bi = 0; bi < s->box_n[0]; bi++)
             for (int bj = 0; bj < s->box_n[1]; bj++)
                 for (int bk = 0; bk < s->box_n[2]; bk++) {
-                    const uint8_t v = m[((size_t)(lo[0] + bi) * NY + lo[1] + bj) * NZ + lo[2] + bk];
+                    const uint8_t v = m[((size_t)(s->box_lo[0] + bi) * NY + s->box_lo[1] + bj) * NZ + s->box_lo[2] + bk];
                     ids[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk] = (v > 1 && v < 6) ? 1 : 0;
                 }
         int rc = upload(s, &s->disp_id[c], ids);
@@ -238,6 +259,11 @@ int build_dispersive(fdtd_sim *s) {
         for (int q = 0; q < 8; q++) {
             rc = dalloc(s, &s->disp_st[c][q], s->nbox);
             if (rc) return rc;
+        }
+        if (S.hydro) {
+            if ((rc = dalloc(s, &s->h_vnew[c], s->nbox)) || (rc = dalloc(s, &s->h_nd[c], s->nbox)) ||
+                (rc = dalloc(s, &s->h_ndp[c], s->nbox)))
+                return rc;
         }
     }
     return FDTD_OK;
@@ -267,8 +293,13 @@ void build_faces(fdtd_sim *s) {
 int contract(fdtd_sim *s) {
     if (s->slot == 0) return FDTD_OK;
     const int F = s->S.P.NUM_freq;
-    HIPCHK(s, launch_twiddle(s->W, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
-    HIPCHK(s, launch_dft_contract(s->X, s->W, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
+    if (s->S.P.dft_exact) {
+        HIPCHK(s, launch_twiddle(s->W, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
+        HIPCHK(s, launch_dft_contract(s->X, s->W, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
+    } else {
+        HIPCHK(s, launch_twiddle_frag(s->Wf, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
+        HIPCHK(s, launch_dft_mfma(s->X, s->Wf, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
+    }
     s->slot = 0;
     return FDTD_OK;
 }
@@ -376,7 +407,13 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
     s->K = S.P.dft_block;
     s->slot_t.assign(s->K, 0);
     const int F = S.P.NUM_freq;
-    if ((rc = dalloc(s, &s->X, (size_t)s->K * s->mtot)) || (rc = dalloc(s, &s->acc, (size_t)2 * F * s->mtot)) ||
+    // the sample ring holds K rows padded to the MFMA k-step (16); padding rows stay zero
+    const int kpad = (s->K + 15) / 16 * 16;
+    const size_t wf_bytes = (size_t)2 * (kpad / 16) * ((2 * F + 31) / 32) * 128 * 16;
+    uint8_t *wf = nullptr;
+    if ((rc = dalloc(s, &wf, wf_bytes))) return fail(rc);
+    s->Wf = wf;
+    if ((rc = dalloc(s, &s->X, (size_t)kpad * s->mtot)) || (rc = dalloc(s, &s->acc, (size_t)2 * F * s->mtot)) ||
         (rc = dalloc(s, &s->W, (size_t)2 * s->K * 2 * F)) || (rc = upload(s, &s->freq_d, S.freq)) ||
         (rc = dalloc(s, &s->flux_d, (size_t)12 * F)))
         return fail(rc);
@@ -445,6 +482,17 @@ int fdtd_set_materials(fdtd_sim *s, const uint8_t *mx, const uint8_t *my, const 
     return FDTD_OK;
 }
 
+int fdtd_add_sphere(fdtd_sim *s, const double centre[3], double radius) {
+    if (!s || !centre || !(radius > 0)) return FDTD_ERR_INVALID;
+    Setup &S = s->S;
+    std::vector<uint8_t> m[4] = {S.mat[0], S.mat[1], S.mat[2], S.mat[3]};
+    fdtd::sphere_masks_at(S, centre, radius, m);
+    const int rc = fdtd_set_materials(s, m[1].data(), m[2].data(), m[3].data());
+    if (rc) return rc;
+    S.mat[0] = m[0];
+    return FDTD_OK;
+}
+
 int fdtd_get_materials(const fdtd_sim *s, int comp, uint8_t *out) {
     if (!s || !out || comp < 0 || comp > 3) return FDTD_ERR_INVALID;
     std::memcpy(out, s->S.mat[comp].data(), s->S.mat[comp].size());
@@ -485,7 +533,7 @@ int fdtd_update_e(fdtd_sim *s) {
     HIPCHK(s, launch_split_e(d, s->stream));
     if (s->nbox) {
         DispDev dd = make_disp(s, s->p);
-        HIPCHK(s, launch_dispersive(dd, s->stream));
+        HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
     }
     return FDTD_OK;
 }
@@ -517,7 +565,6 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         }
         for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
         const bool samp = sampling_step(s);
-        if (samp) d.sample_row = s->X + (size_t)s->slot * s->mtot;
         if (s->timing) {
             hipEvent_t a = next_event(s, 0), b = next_event(s, 1);
             if (!a || !b) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
@@ -530,12 +577,17 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         }
         if (s->nbox) {
             DispDev dd = make_disp(s, 1 - q);
-            HIPCHK(s, launch_dispersive(dd, s->stream));
+            HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
         }
         s->p ^= 1;
         s->pp ^= 1;
         s->le ^= 1;
         if (samp) {
+            // tangential E^{n+1} and H^{n+1/2} on the monitor faces (ScattAbs, main.cu:2920)
+            StepDev ds = make_step(s, false);
+            for (int c = 0; c < 3; c++) { ds.Ein[c] = s->E[s->p][c]; ds.Hin[c] = s->H[s->p][c]; }
+            ds.sample_row = s->X + (size_t)s->slot * s->mtot;
+            HIPCHK(s, launch_dft_sample_split(ds, s->stream));
             int rc = after_sample(s);
             if (rc) return rc;
         }
@@ -563,7 +615,7 @@ int fdtd_set_field(fdtd_sim *s, int comp, const float *in) {
     if (!s || !in || comp < 0 || comp > 5) return FDTD_ERR_INVALID;
     float *f = comp < 3 ? s->E[s->p][comp] : s->H[s->p][comp - 3];
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(s->dense, in, (size_t)s->g.NX * s->g.NY * s->g.NZ * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpyAsync(s->dense, in, (size_t)s->g.NX * s->g.NY * s->g.NZ * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIPCHK(s, launch_pack_field(f, s->g, s->dense, 0, s->stream));
     if (comp < 3 && s->nbox) {
         DispDev d = make_disp(s, s->p);

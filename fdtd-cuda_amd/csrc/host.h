@@ -62,6 +62,8 @@ void default_params(fdtd_params *p);
 bool build_setup(const fdtd_params &p, Setup &s, std::string &err);
 // Staircased sphere masks (setup.cu:219-248).
 void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]);
+// Same rule around an arbitrary centre (cells).
+void sphere_masks_at(const Setup &s, const double centre[3], double radius, std::vector<uint8_t> m[4]);
 double source_pulse(const Setup &s, int t);
 
 }  // namespace fdtd

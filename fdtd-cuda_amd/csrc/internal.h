@@ -96,17 +96,24 @@ struct DispDev {
     int lo[3], n[3];
     const uint8_t *id[3];           // per E component: 1 = dispersive, 0 = not
     const float *rho[3];            // design density per E component, or null (rho = 1)
-    // state per component, box-dense SoA: E^n, E^{n-1}, Drude P^n, P^{n-1}, CP0 P^n, P^{n-1}, CP1 P^n, P^{n-1}
+    // state per component, box-dense SoA: 0 E^n, 1 E^{n-1}, then per pole q
+    // (0 Drude, 1-2 CP) P~^n at 2+2q and J~^n = P~^n - P~^{n-1} at 3+2q, P~ = P/eps0
     float *st[3][8];
     const float *H[3];              // H^{n+1/2}
     float *E[3];                    // output field (E^{n+1} written at dispersive cells)
     const float *ik_e[3];
-    // ADE coefficients with P stored as P/eps0 and 1/C_E folded in
-    float a0, a1, a2;               // E^{n+1} = a0*curl + a1*E^n - a2*E^{n-1} - sum(b1 P^n + b2 P^{n-1})
-    float b1[3], b2[3];             // poles: 0 Drude, 1-2 CP
-    float p1[3], p2[3], p3[3], p4[3], p5[3];   // P^{n+1} = p1 P^n + p2 P^{n-1} + p3 E^{n+1} + p4 E^n + p5 E^{n-1}
-    // design-density variant: vacuum/metal blend (rho in [0,1])
-    float ce_vac, ce_met_minus_vac; // eps0*eps_inf(rho) + rho*sum(c3)  split into parts (scaled)
+    // E^{n+1} = a0*curl + a1*E^n - a2*E^{n-1} - sum_q (e0[q] P^n + e1[q] J^n)
+    // J^{n+1} = g1[q] J^n + g0[q] P^n + p3[q] E^{n+1} + p4[q] E^n + p5[q] E^{n-1};  P^{n+1} = P^n + J^{n+1}
+    float a0, a1, a2;
+    float e0[3], e1[3], g0[3], g1[3], p3[3], p4[3], p5[3];
+    // hydrodynamic model (pole 0 becomes the electron velocity v, st[2] = v^n,
+    // st[3] = v^n - v^{n-1}; charge stored as n~ = ND / N_EQ)
+    int hydro;
+    float *vnew[3];                 // v^{n+1} from the momentum update
+    float *nd[3], *nd_prev[3];      // n~^{n-1} (updated to n~^{n+1}), n~^n
+    float h_d1m1, h_d3, h_d4;       // v^{n+1} = v + (d1-1) J + d3 E^n + d4 E^{n-1} + h_nl (gd + gd2) / cbrt(1 + n~)
+    float h_nl, h_inv_d, h_cont;    // h_nl = beta^2/dd, 1/dx, 2 dt/dx
+    float h_cur;                    // dt e0 N_EQ / C_E: current term of the E update
 };
 
 // 1D incident line (main.cu:2280-2337, 2831-2859), FP64.
@@ -136,9 +143,16 @@ hipError_t launch_twiddle(double *W, const int *t_slot_host, int K, int nvalid, 
                           double dt, hipStream_t st);
 hipError_t launch_dft_contract(const float *X, const double *W, double *acc, long mtot, long mE_pad, int K,
                                int F, hipStream_t st);
+// Split-BF16 MFMA path: twiddle fragments (2 * ceil(K/16) * ceil(2F/32) * 128 * 16 bytes) and contraction.
+hipError_t launch_twiddle_frag(void *Wf, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
+                               double dt, hipStream_t st);
+hipError_t launch_dft_mfma(const float *X, const void *Wf, double *acc, long mtot, long mE_pad, int K, int F,
+                           hipStream_t st);
 hipError_t launch_flux_reduce(const double *acc, const FaceDev &fd, long mtot, long mE_pad, int F, double dA,
                               double *out, hipStream_t st);
 hipError_t launch_pack_field(const float *f, const Geom &g, float *dense, int to_dense, hipStream_t st);
 hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t st);
+// Hydrodynamic model: velocity update, then per component (x, y, z) continuity and E update.
+hipError_t launch_hydro(const DispDev &d, hipStream_t st);
 
 #endif

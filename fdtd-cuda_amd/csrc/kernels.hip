@@ -417,6 +417,13 @@ __global__ void k_dft_sample_split(StepDev s) {
 
 // ----------------------------------------------------------------- dispersive box (main.cu:1178-1219)
 
+// The reference advances each pole as P^{n+1} = k1 P^n + k2 P^{n-1} + drive and
+// subtracts (k1-1) P^n + k2 P^{n-1} in the E update.  The same recurrence is
+// carried here in (P, J = P^n - P^{n-1}) form: (k1-1) P^n + k2 P^{n-1} =
+// (k1+k2-1) P^n - k2 J^n, and J^{n+1} = -k2 J^n + (k1+k2-1) P^n + drive.  For
+// the Drude pole k1+k2-1 = 0 exactly, so the secularly growing P drops out and
+// the FP32 update avoids the near-total cancellation of the two-level form.
+
 __global__ void k_dispersive(DispDev d) {
     const long nb = (long)d.n[0] * d.n[1] * d.n[2];
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,21 +443,160 @@ __global__ void k_dispersive(DispDev d) {
         else curl = (d.H[1][p] - d.H[1][p - g.sxy]) * d.ik_e[0][i] - (d.H[0][p] - d.H[0][p - g.sz]) * d.ik_e[1][j];
         float *const *st = d.st[c];
         const float en = st[0][idx], en1 = st[1][idx];
-        const float pd = st[2][idx], pd1 = st[3][idx];
-        const float pa = st[4][idx], pa1 = st[5][idx];
-        const float pb = st[6][idx], pb1 = st[7][idx];
-        const float e = d.a0 * curl + d.a1 * en - d.a2 * en1
-                      - (d.b1[0] * pd + d.b2[0] * pd1) - (d.b1[1] * pa + d.b2[1] * pa1) - (d.b1[2] * pb + d.b2[2] * pb1);
-        st[2][idx] = d.p1[0] * pd + d.p2[0] * pd1 + d.p3[0] * e + d.p4[0] * en + d.p5[0] * en1;
-        st[3][idx] = pd;
-        st[4][idx] = d.p1[1] * pa + d.p2[1] * pa1 + d.p3[1] * e + d.p4[1] * en + d.p5[1] * en1;
-        st[5][idx] = pa;
-        st[6][idx] = d.p1[2] * pb + d.p2[2] * pb1 + d.p3[2] * e + d.p4[2] * en + d.p5[2] * en1;
-        st[7][idx] = pb;
+        float P[3], J[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) { P[q] = st[2 + 2 * q][idx]; J[q] = st[3 + 2 * q][idx]; }
+        float e = d.a0 * curl + d.a1 * en - d.a2 * en1;
+#pragma unroll
+        for (int q = 0; q < 3; q++) e -= d.e0[q] * P[q] + d.e1[q] * J[q];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float jn = d.g1[q] * J[q] + d.g0[q] * P[q] + d.p3[q] * e + d.p4[q] * en + d.p5[q] * en1;
+            st[2 + 2 * q][idx] = P[q] + jn;
+            st[3 + 2 * q][idx] = jn;
+        }
         st[1][idx] = en;
         st[0][idx] = e;
         d.E[c][p] = e;
     }
+}
+
+// ----------------------------------------------------------------- hydrodynamic model (main.cu:1916-2276)
+
+__device__ inline bool box_interior(const DispDev &d, long idx, int &bi, int &bj, int &bk) {
+    bk = (int)(idx % d.n[2]);
+    bj = (int)((idx / d.n[2]) % d.n[1]);
+    bi = (int)(idx / ((long)d.n[2] * d.n[1]));
+    return bi > 0 && bj > 0 && bk > 0 && bi < d.n[0] - 1 && bj < d.n[1] - 1 && bk < d.n[2] - 1;
+}
+
+// Momentum update of the electron velocity (UpdateHydroPx/Py/Pz, general
+// branch): v^{n+1} = d1 v^n + d2 v^{n-1} + d3 E^n + d4 E^{n-1}
+//                  + beta^2/dd (GradDiv v + GradDiv (v n~)) / cbrt(1 + n~),
+// every neighbour at time level n.  With d1 + d2 = 1 the first two terms are
+// v^n + (d1 - 1)(v^n - v^{n-1}).  Convection and magnetic terms are off, as
+// in every configuration of the reference (setup.cu:992-999).
+__global__ void k_hydro_v(DispDev d) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    int bi, bj, bk;
+    if (!box_interior(d, idx, bi, bj, bk)) return;
+    const long oi = (long)d.n[1] * d.n[2], oj = d.n[2], ok = 1;
+    const float I2 = d.h_inv_d * d.h_inv_d;
+    const float *V0 = d.st[0][2], *V1 = d.st[1][2], *V2 = d.st[2][2];
+    const float *N0 = d.nd_prev[0], *N1 = d.nd_prev[1], *N2 = d.nd_prev[2];
+#define VN(V, N, o) ((V)[idx + (o)] * (N)[idx + (o)])
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (!d.id[c][idx]) continue;
+        float gd, gd2;
+        if (c == 0) {
+            gd = (V0[idx + oi] - 2.f * V0[idx] + V0[idx - oi]) + (V1[idx + oi] - V1[idx] - V1[idx + oi - oj] + V1[idx - oj])
+               + (V2[idx + oi] - V2[idx] - V2[idx + oi - ok] + V2[idx - ok]);
+            gd2 = (VN(V0, N0, oi) - 2.f * VN(V0, N0, 0) + VN(V0, N0, -oi)) + (VN(V1, N1, oi) - VN(V1, N1, 0) - VN(V1, N1, oi - oj) + VN(V1, N1, -oj))
+                + (VN(V2, N2, oi) - VN(V2, N2, 0) - VN(V2, N2, oi - ok) + VN(V2, N2, -ok));
+        } else if (c == 1) {
+            gd = (V1[idx + oj] - 2.f * V1[idx] + V1[idx - oj]) + (V0[idx + oj] - V0[idx] - V0[idx - oi + oj] + V0[idx - oi])
+               + (V2[idx + oj] - V2[idx] - V2[idx + oj - ok] + V2[idx - ok]);
+            gd2 = (VN(V1, N1, oj) - 2.f * VN(V1, N1, 0) + VN(V1, N1, -oj)) + (VN(V0, N0, oj) - VN(V0, N0, 0) - VN(V0, N0, oj - oi) + VN(V0, N0, -oi))
+                + (VN(V2, N2, oj) - VN(V2, N2, 0) - VN(V2, N2, oj - ok) + VN(V2, N2, -ok));
+        } else {
+            gd = (V2[idx + ok] - 2.f * V2[idx] + V2[idx - ok]) + (V1[idx + ok] - V1[idx] - V1[idx - oj + ok] + V1[idx - oj])
+               + (V0[idx + ok] - V0[idx] - V0[idx - oi + ok] + V0[idx - oi]);
+            gd2 = (VN(V2, N2, ok) - 2.f * VN(V2, N2, 0) + VN(V2, N2, -ok)) + (VN(V1, N1, ok) - VN(V1, N1, 0) - VN(V1, N1, ok - oj) + VN(V1, N1, -oj))
+                + (VN(V0, N0, ok) - VN(V0, N0, 0) - VN(V0, N0, ok - oi) + VN(V0, N0, -oi));
+        }
+        const float *const *st = d.st[c];
+        const float np = d.nd_prev[c][idx];
+        d.vnew[c][idx] = st[2][idx] + d.h_d1m1 * st[3][idx] + d.h_d3 * st[0][idx] + d.h_d4 * st[1][idx]
+                       + d.h_nl * I2 * (gd + gd2) / cbrtf(1.f + np);
+    }
+#undef VN
+}
+
+// Continuity for component c (main.cu:1224-1239, 1537-1554, 1818-1837):
+// n~^{n+1} = n~^{n-1} - 2 dt/dx * (div(n~ v) + div v), the normal-axis terms
+// halved; earlier components are already at level n+1 (x -> y -> z order).
+// The z update keeps the reference's 0.5 (Vz1 - Vz2) in place of the
+// n~-weighted normal term (main.cu:1837).
+__global__ void k_hydro_cont(DispDev d, int c) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    int bi, bj, bk;
+    if (!box_interior(d, idx, bi, bj, bk) || !d.id[c][idx]) return;
+    const long oi = (long)d.n[1] * d.n[2], oj = d.n[2], ok = 1;
+    const float *Px = d.st[0][2], *Py = d.st[1][2], *Pz = d.st[2][2];
+    const float *Nx = d.nd_prev[0], *Ny = d.nd_prev[1], *Nz = d.nd_prev[2];
+    float r;
+    if (c == 0) {
+        const float Vx1 = Px[idx + oi], Vx2 = Px[idx - oi], Nx1 = Nx[idx + oi], Nx2 = Nx[idx - oi];
+        const float Vy1 = 0.5f * (Py[idx + oi] + Py[idx]), Vy2 = 0.5f * (Py[idx + oi - oj] + Py[idx - oj]);
+        const float Ny1 = 0.5f * (Ny[idx + oi] + Ny[idx]), Ny2 = 0.5f * (Ny[idx + oi - oj] + Ny[idx - oj]);
+        const float Vz1 = 0.5f * (Pz[idx + oi] + Pz[idx]), Vz2 = 0.5f * (Pz[idx + oi - ok] + Pz[idx - ok]);
+        const float Nz1 = 0.5f * (Nz[idx + oi] + Nz[idx]), Nz2 = 0.5f * (Nz[idx + oi - ok] + Nz[idx - ok]);
+        r = 0.5f * (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + (Nz1 * Vz1 - Nz2 * Vz2)
+          + (0.5f * (Vx1 - Vx2) + (Vy1 - Vy2) + (Vz1 - Vz2));
+    } else if (c == 1) {
+        const float Vy1 = Py[idx + oj], Vy2 = Py[idx - oj], Ny1 = Ny[idx + oj], Ny2 = Ny[idx - oj];
+        const float Vx1 = 0.5f * (Px[idx + oj] + Px[idx]), Vx2 = 0.5f * (Px[idx - oi + oj] + Px[idx - oi]);
+        const float Nx1 = 0.5f * (Nx[idx + oj] + Nx[idx]), Nx2 = 0.5f * (Nx[idx - oi + oj] + Nx[idx - oi]);
+        const float Vz1 = 0.5f * (Pz[idx + oj] + Pz[idx]), Vz2 = 0.5f * (Pz[idx + oj - ok] + Pz[idx - ok]);
+        const float Nz1 = 0.5f * (Nz[idx + oj] + Nz[idx]), Nz2 = 0.5f * (Nz[idx + oj - ok] + Nz[idx - ok]);
+        r = (Nx1 * Vx1 - Nx2 * Vx2) + 0.5f * (Ny1 * Vy1 - Ny2 * Vy2) + (Nz1 * Vz1 - Nz2 * Vz2)
+          + ((Vx1 - Vx2) + 0.5f * (Vy1 - Vy2) + (Vz1 - Vz2));
+    } else {
+        const float Vz1 = Pz[idx + ok], Vz2 = Pz[idx - ok];
+        const float Vx1 = 0.5f * (Px[idx + ok] + Px[idx]), Vx2 = 0.5f * (Px[idx - oi + ok] + Px[idx - oi]);
+        const float Nx1 = 0.5f * (Nx[idx + ok] + Nx[idx]), Nx2 = 0.5f * (Nx[idx - oi + ok] + Nx[idx - oi]);
+        const float Vy1 = 0.5f * (Py[idx + ok] + Py[idx]), Vy2 = 0.5f * (Py[idx - oj + ok] + Py[idx - oj]);
+        const float Ny1 = 0.5f * (Ny[idx + ok] + Ny[idx]), Ny2 = 0.5f * (Ny[idx - oj + ok] + Ny[idx - oj]);
+        r = (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + 0.5f * (Vz1 - Vz2)
+          + ((Vx1 - Vx2) + (Vy1 - Vy2) + 0.5f * (Vz1 - Vz2));
+    }
+    d.nd[c][idx] = d.nd[c][idx] - d.h_cont * r;
+}
+
+// E update of component c with the hydrodynamic current (main.cu:1247-1271):
+// E^{n+1} = a0 curl + h_cur v^{n+1} (1 + n~^{n+1}) + a1 E^n - a2 E^{n-1} - CP terms;
+// then the CP poles advance, v and n~ shift one level.
+__global__ void k_hydro_e(DispDev d, int c) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    int bi, bj, bk;
+    if (!box_interior(d, idx, bi, bj, bk) || !d.id[c][idx]) return;
+    const int i = d.lo[0] + bi, j = d.lo[1] + bj, k = d.lo[2] + bk;
+    const Geom &g = d.g;
+    const long p = gidx(g, i, j, k);
+    float curl;
+    if (c == 0) curl = (d.H[2][p] - d.H[2][p - g.sz]) * d.ik_e[1][j] - (d.H[1][p] - d.H[1][p - 1]) * d.ik_e[2][k];
+    else if (c == 1) curl = (d.H[0][p] - d.H[0][p - 1]) * d.ik_e[2][k] - (d.H[2][p] - d.H[2][p - g.sxy]) * d.ik_e[0][i];
+    else curl = (d.H[1][p] - d.H[1][p - g.sxy]) * d.ik_e[0][i] - (d.H[0][p] - d.H[0][p - g.sz]) * d.ik_e[1][j];
+    float *const *st = d.st[c];
+    const float en = st[0][idx], en1 = st[1][idx];
+    const float vn = d.vnew[c][idx], ndn = d.nd[c][idx];
+    float P[2], J[2];
+#pragma unroll
+    for (int q = 1; q < 3; q++) { P[q - 1] = st[2 + 2 * q][idx]; J[q - 1] = st[3 + 2 * q][idx]; }
+    float e = d.a0 * curl + d.h_cur * vn * (1.f + ndn) + d.a1 * en - d.a2 * en1;
+#pragma unroll
+    for (int q = 1; q < 3; q++) e -= d.e0[q] * P[q - 1] + d.e1[q] * J[q - 1];
+#pragma unroll
+    for (int q = 1; q < 3; q++) {
+        const float jn = d.g1[q] * J[q - 1] + d.g0[q] * P[q - 1] + d.p3[q] * e + d.p4[q] * en + d.p5[q] * en1;
+        st[2 + 2 * q][idx] = P[q - 1] + jn;
+        st[3 + 2 * q][idx] = jn;
+    }
+    st[3][idx] = vn - st[2][idx];
+    st[2][idx] = vn;
+    const float hold = d.nd_prev[c][idx];
+    d.nd_prev[c][idx] = ndn;
+    d.nd[c][idx] = hold;
+    st[1][idx] = en;
+    st[0][idx] = e;
+    d.E[c][p] = e;
 }
 
 // Copies E^n from the field into the box state (after a field overwrite) or
@@ -553,6 +699,107 @@ __global__ __launch_bounds__(256) void k_dft_contract(const float *X, const doub
         if (r0 + rr < 2 * F) acc[(long)(r0 + rr) * mtot + col] += a[rr];
 }
 
+// ----------------------------------------------------------------- MFMA split-BF16 contraction
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Round-to-nearest-even FP32 -> BF16 (finite inputs).
+__device__ inline __bf16 to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
+}
+__device__ inline float from_bf16(__bf16 b) {
+    return __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
+// Twiddles in the A-operand fragment order of v_mfma_f32_32x32x16_bf16:
+// lane l of m-tile mt and k-step ks holds W[s][r] for r = 32 mt + (l % 32) and
+// s = 16 ks + 8 (l / 32) + j, j = 0..7, split into hi and lo BF16 halves.
+// Wf index: (((half * nks + ks) * nmt + mt) * 2 + hi_lo) * 64 + lane.
+__global__ void k_twiddle_frag(bf16x8 *Wf, SlotTimes ts, int nks, int nmt, int nvalid, int F, const double *freq,
+                               double dt) {
+    const int n = 2 * nks * nmt * 64;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int l = x % 64, mt = (x / 64) % nmt, ks = (x / (64 * nmt)) % nks, half = x / (64 * nmt * nks);
+    const int r = 32 * mt + (l % 32);
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int sidx = 16 * ks + 8 * (l / 32) + j;
+        double v = 0.0;
+        if (sidx < nvalid && r < 2 * F) {
+            const int f = r % F;
+            const double tt = ts.t[sidx] + (half == 0 ? 1.0 : 0.5);
+            const double ph = 2.0 * kPI * tt * dt * freq[f];
+            v = r < F ? cos(ph) : sin(ph);
+        }
+        const float vf = (float)v;
+        hi[j] = to_bf16(vf);
+        lo[j] = to_bf16((float)(v - (double)from_bf16(hi[j])));
+    }
+    const long base = ((((long)half * nks + ks) * nmt + mt) * 2) * 64 + l;
+    Wf[base] = hi;
+    Wf[base + 64] = lo;
+}
+
+// acc[r][col] += sum_s W[half][s][r] X[s][col] with split-BF16 products
+// (hi*hi + hi*lo + lo*hi) accumulated in FP32 per block, then added to the
+// FP64 accumulator.  One wave = 32 columns x up to 4 m-tiles (128 rows);
+// grid.y walks groups of 4 m-tiles.  B operand: lane l holds column l % 32,
+// slots 16 ks + 8 (l / 32) + j.  C/D: lane l holds column l % 32, rows
+// 8 i + 4 (l / 32) + j in element 4 i + j.
+__global__ __launch_bounds__(256) void k_dft_mfma(const float *X, const bf16x8 *Wf, double *acc, long mtot,
+                                                 long mE_pad, int nks, int nmt, int rows) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const long col0 = ((long)blockIdx.x * 4 + w) * 32;
+    if (col0 >= mtot) return;
+    const int half = col0 >= mE_pad ? 1 : 0;
+    const int mg = blockIdx.y;
+    const int nm = min(4, nmt - 4 * mg);
+    const long col = col0 + (l & 31);
+    const bool cok = col < mtot;
+    f32x16 c[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) c[q][e] = 0.f;
+    for (int ks = 0; ks < nks; ks++) {
+        bf16x8 bh, bl;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int sidx = 16 * ks + 8 * (l >> 5) + j;
+            const float x = cok ? X[(long)sidx * mtot + col] : 0.f;
+            bh[j] = to_bf16(x);
+            bl[j] = to_bf16(x - from_bf16(bh[j]));
+        }
+        const long wb = (((long)half * nks + ks) * nmt + 4 * mg) * 2 * 64 + l;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (q < nm) {
+                const bf16x8 ah = Wf[wb + (long)q * 128], al = Wf[wb + (long)q * 128 + 64];
+                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c[q], 0, 0, 0);
+                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c[q], 0, 0, 0);
+                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c[q], 0, 0, 0);
+            }
+        }
+    }
+    if (!cok) return;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q >= nm) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = 32 * (4 * mg + q) + 8 * i + 4 * (l >> 5) + j;
+                if (r < rows) acc[(long)r * mtot + col] += (double)c[q][4 * i + j];
+            }
+    }
+}
+
 // Flux through one face at one frequency: sum Re(Ea conj(Hb) - Eb conj(Ha)) * dA (Spectral.cu:737-1132).
 __global__ __launch_bounds__(256) void k_flux(const double *acc, FaceDev fd, long mtot, long mE_pad, int F,
                                               double dA, double *out) {
@@ -650,6 +897,18 @@ hipError_t launch_dispersive(const DispDev &d, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_hydro(const DispDev &d, hipStream_t st) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    if (nb <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nb + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_hydro_v, grid, block, 0, st, d);
+    for (int c = 0; c < 3; c++) {
+        hipLaunchKernelGGL(k_hydro_cont, grid, block, 0, st, d, c);
+        hipLaunchKernelGGL(k_hydro_e, grid, block, 0, st, d, c);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t st) {
     const long nb = (long)d.n[0] * d.n[1] * d.n[2];
     if (nb <= 0) return hipSuccess;
@@ -677,6 +936,26 @@ hipError_t launch_dft_contract(const float *X, const double *W, double *acc, lon
                                int F, hipStream_t st) {
     dim3 grid((unsigned)((mtot + 255) / 256), (2 * F + kRC - 1) / kRC), block(256);
     hipLaunchKernelGGL(k_dft_contract, grid, block, 0, st, X, W, acc, mtot, mE_pad, K, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddle_frag(void *Wf, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
+                               double dt, hipStream_t st) {
+    if (K > 256) return hipErrorInvalidValue;
+    SlotTimes ts;
+    for (int i = 0; i < 256; i++) ts.t[i] = i < K ? t_slot_host[i] : 0;
+    const int nks = (K + 15) / 16, nmt = (2 * F + 31) / 32;
+    const int n = 2 * nks * nmt * 64;
+    hipLaunchKernelGGL(k_twiddle_frag, dim3((n + 255) / 256), dim3(256), 0, st, (bf16x8 *)Wf, ts, nks, nmt, nvalid, F,
+                       freq, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_mfma(const float *X, const void *Wf, double *acc, long mtot, long mE_pad, int K, int F,
+                           hipStream_t st) {
+    const int nks = (K + 15) / 16, nmt = (2 * F + 31) / 32;
+    dim3 grid((unsigned)((mtot + 127) / 128), (nmt + 3) / 4), block(256);
+    hipLaunchKernelGGL(k_dft_mfma, grid, block, 0, st, X, (const bf16x8 *)Wf, acc, mtot, mE_pad, nks, nmt, 2 * F);
     return hipGetLastError();
 }
 

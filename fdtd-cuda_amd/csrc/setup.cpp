@@ -5,6 +5,7 @@
 // (main.cu:2446-2463) and the E_incident sum (main.cu:2629-2644).
 #include "host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -42,6 +43,7 @@ void default_params(fdtd_params *p) {
     p->source_offset = 50;
     p->hydro_beta = 1.39e6 * std::sqrt(0.75);
     p->xs_area = 0;
+    p->dft_exact = 0;
 }
 
 namespace {
@@ -90,23 +92,31 @@ double source_pulse(const Setup &s, int t) {
     return s.P.MAX_AMP * std::exp(-(double)b2 / 2.0) * std::sin(2 * kPI * s.f0 * t * s.dt);
 }
 
-void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]) {
+void sphere_masks_at(const Setup &s, const double centre[3], double radius, std::vector<uint8_t> m[4]) {
     const int NX = s.N[0], NY = s.N[1], NZ = s.N[2];
     const int r = (int)(radius / s.P.dx);
-    const double cx = NX / 2, cy = NY / 2, cz = NZ / 2;
+    const double cx = centre[0], cy = centre[1], cz = centre[2];
     const uint8_t mat = (uint8_t)s.P.material;
-    for (int i = 0; i < NX; i++)
-        for (int j = 0; j < NY; j++)
-            for (int k = 0; k < NZ; k++) {
+    auto in = [&](double x, double y, double z) {
+        return std::pow((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz), 0.5) < r;
+    };
+    const int lo[3] = {std::max(0, (int)(cx - r - 2)), std::max(0, (int)(cy - r - 2)), std::max(0, (int)(cz - r - 2))};
+    const int hi[3] = {std::min(NX, (int)(cx + r + 3)), std::min(NY, (int)(cy + r + 3)), std::min(NZ, (int)(cz + r + 3))};
+    for (int i = lo[0]; i < hi[0]; i++)
+        for (int j = lo[1]; j < hi[1]; j++)
+            for (int k = lo[2]; k < hi[2]; k++) {
                 const size_t id = ((size_t)i * NY + j) * NZ + k;
-                auto in = [&](double x, double y, double z) {
-                    return std::pow((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz), 0.5) < r;
-                };
                 if (in(i, j, k)) m[0][id] = mat;
                 if (in(i + 0.5, j, k)) m[1][id] = mat;
                 if (in(i, j + 0.5, k)) m[2][id] = mat;
                 if (in(i, j, k + 0.5)) m[3][id] = mat;
             }
+}
+
+// The reference centres its sphere on the cell NCELL/2 of the augmented grid.
+void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]) {
+    const double c[3] = {(double)(s.N[0] / 2), (double)(s.N[1] / 2), (double)(s.N[2] / 2)};
+    sphere_masks_at(s, c, radius, m);
 }
 
 bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfdtd_hip.so")
 
 EXPORTS = [
     "fdtd_default_params", "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_get_info",
-    "fdtd_set_materials", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
+    "fdtd_set_materials", "fdtd_add_sphere", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
     "fdtd_dft_accumulate", "fdtd_step", "fdtd_sync", "fdtd_get_field", "fdtd_set_field",
     "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
 ]
@@ -40,6 +40,7 @@ def load_library(path=LIB_PATH):
         "fdtd_last_error": (C.c_char_p, [vp]),
         "fdtd_get_info": (ip, [vp, C.POINTER(GridInfo)]),
         "fdtd_set_materials": (ip, [vp, u8p, u8p, u8p]),
+        "fdtd_add_sphere": (ip, [vp, C.POINTER(C.c_double * 3), C.c_double]),
         "fdtd_get_materials": (ip, [vp, ip, u8p]),
         "fdtd_update_h": (ip, [vp]),
         "fdtd_tfsf": (ip, [vp]),
@@ -115,6 +116,11 @@ class Simulation:
     def set_materials(self, mx, my, mz):
         arrs = [np.ascontiguousarray(m, dtype=np.uint8).reshape(-1) for m in (mx, my, mz)]
         self._check(self.lib.fdtd_set_materials(self.h, *[_ptr(a, C.c_uint8) for a in arrs]))
+        self.info = self.get_info()
+
+    def add_sphere(self, centre, radius):
+        c = (C.c_double * 3)(*[float(x) for x in centre])
+        self._check(self.lib.fdtd_add_sphere(self.h, C.byref(c), float(radius)))
         self.info = self.get_info()
 
     def get_materials(self, comp):

@@ -47,6 +47,7 @@ class Params(C.Structure):
         ("source_offset", C.c_int),
         ("hydro_beta", C.c_double),
         ("xs_area", C.c_double),
+        ("dft_exact", C.c_int),
     ]
 
     def as_dict(self):

@@ -84,6 +84,7 @@ typedef struct fdtd_params {
     int    source_offset;      /* source cell m0 - source_offset, 50 (main.cu:2851)             */
     double hydro_beta;         /* nonlocal velocity beta (m/s), 1.39e6*sqrt(3/4) (setup.cu:980-983) */
     double xs_area;            /* cross-section normaliser (m^2); 0 -> pi*r^2 (Spectral.cu:1166) */
+    int    dft_exact;          /* 0: split-BF16 MFMA contraction (default), 1: FP64 contraction */
 } fdtd_params;
 
 /* Derived grid facts (read-only). */
@@ -129,6 +130,12 @@ int  fdtd_get_info(const fdtd_sim *sim, fdtd_grid_info *info);
  * lie strictly inside the absorption box. */
 int  fdtd_set_materials(fdtd_sim *sim, const uint8_t *id_x, const uint8_t *id_y,
                         const uint8_t *id_z);
+/* Paints a staircased sphere of the configured material into the masks with
+ * the reference's rule (setup.cu:219-248): an E component at Yee position x
+ * is inside when |x - centre| < (int)(radius/dx), centre in cells.  Used for
+ * the dimer and other multi-particle geometries; same validity rules as
+ * fdtd_set_materials. */
+int  fdtd_add_sphere(fdtd_sim *sim, const double centre[3], double radius);
 /* Copies a mask out (comp 0: mat_matrix, 1..3: X, Y, Z). */
 int  fdtd_get_materials(const fdtd_sim *sim, int comp, uint8_t *out);
 

@@ -932,7 +932,7 @@ void orc_default_params(fdtd_params *p) {
     p->cpml_exp = 3; p->kappa_max = 5; p->alpha_max = 0.05; p->sigma_factor = 0.8;
     p->lam_min = 200e-9; p->lam_max = 700e-9;
     p->dft_stride = 4; p->dft_block = 64; p->source_offset = 50;
-    p->hydro_beta = 1.39e6 * std::sqrt(0.75); p->xs_area = 0;
+    p->hydro_beta = 1.39e6 * std::sqrt(0.75); p->xs_area = 0; p->dft_exact = 0;
 }
 
 int orc_create(const fdtd_params *p, void **out) {

@@ -14,7 +14,7 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
-def small_params(mod, n=24, ncpml=6, radius=8e-9, steps=160, nfreq=8, block=8, sphere=1, phi=0.0):
+def small_params(mod, n=24, ncpml=6, radius=8e-9, steps=160, nfreq=8, block=8, sphere=1, phi=0.0, exact=0):
     """Small grid for parity runs (T = n + 2*ncpml + 1), short pulse so fields are large."""
     p = mod.default_params()
     p.NCELLX = p.NCELLY = p.NCELLZ = n
@@ -30,4 +30,5 @@ def small_params(mod, n=24, ncpml=6, radius=8e-9, steps=160, nfreq=8, block=8, s
     p.lam_min, p.lam_max = 400e-9, 900e-9
     p.dft_block = block
     p.inc_phi = phi
+    p.dft_exact = exact
     return p

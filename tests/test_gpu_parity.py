@@ -1,8 +1,10 @@
 """HIP solver against the FP64 oracle on small grids, through the C ABI.
 
-Tolerances: fields and DFT spectra are FP32 on the device against FP64 in the
-oracle; after ~160 steps the relative error (L-inf normalised by the field
-maximum) stays near 1e-6..1e-5, so 2e-4 is the bar here.
+Tolerances: fields are FP32 on the device against FP64 in the oracle.  The
+error of a component is its L-inf difference normalised by the largest value
+of any E (or H) component, so cross-polarised components that are nearly zero
+are judged against the field that drives them.  After 400 steps (the pulse has
+crossed the TFSF box) that error sits near 1e-6..1e-5; the bar is 2e-4.
 """
 import numpy as np
 import pytest
@@ -20,6 +22,14 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+def field_errors(s, o):
+    fo = [o.get_field(c) for c in range(6)]
+    fs = [s.get_field(c) for c in range(6)]
+    me = max(np.abs(f).max() for f in fo[:3])
+    mh = max(np.abs(f).max() for f in fo[3:])
+    return [float(np.abs(fs[c] - fo[c]).max() / (me if c < 3 else mh)) for c in range(6)]
+
+
 def run_pair(p, steps, fused):
     o = O.Oracle(p)
     s = fdtd_amd.Simulation(p)
@@ -33,24 +43,21 @@ def run_pair(p, steps, fused):
 
 @pytest.mark.parametrize("fused", [False, True])
 def test_fields_gold_sphere(fused):
-    p = small_params(O, steps=160)
-    o, s = run_pair(p, 160, fused)
+    p = small_params(O, steps=400)
+    o, s = run_pair(p, 400, fused)
     assert s.get_info().n_dispersive[0] == o.info.n_dispersive[0] > 0
-    for c in range(6):
-        err = rel(s.get_field(c), o.get_field(c))
-        print(f"fused={fused} comp={c} rel={err:.3e}", flush=True)
-        assert err < TOL, (c, err)
+    errs = field_errors(s, o)
+    print(f"gold sphere fused={fused} errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < TOL, errs
 
 
 @pytest.mark.parametrize("fused", [False, True])
 def test_fields_vacuum_x_polarised(fused):
-    p = small_params(O, sphere=0, phi=90.0, steps=140)
-    o, s = run_pair(p, 140, fused)
-    for c in range(6):
-        err = rel(s.get_field(c), o.get_field(c))
-        print(f"vacuum fused={fused} comp={c} rel={err:.3e}", flush=True)
-        if np.max(np.abs(o.get_field(c))) > 1e-6:
-            assert err < TOL, (c, err)
+    p = small_params(O, sphere=0, phi=90.0, steps=400)
+    o, s = run_pair(p, 400, fused)
+    errs = field_errors(s, o)
+    print(f"vacuum fused={fused} errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < TOL, errs
 
 
 def test_line_matches_oracle():
@@ -64,8 +71,8 @@ def test_line_matches_oracle():
 
 @pytest.mark.parametrize("fused", [False, True])
 def test_dft_and_flux(fused):
-    p = small_params(O, steps=160, block=8)
-    o, s = run_pair(p, 161, fused)   # ends mid-block: exercises the partial contraction
+    p = small_params(O, steps=450, block=8)
+    o, s = run_pair(p, 449, fused)   # 113 samples: ends mid-block, exercises the partial contraction
     worst = 0.0
     for face in range(12):
         for comp in range(4):
@@ -81,26 +88,59 @@ def test_dft_and_flux(fused):
 
 
 def test_fused_equals_split():
-    p = small_params(O, steps=120)
+    p = small_params(O, steps=400)
     a = fdtd_amd.Simulation(p)
     b = fdtd_amd.Simulation(p)
-    a.step(120)
-    b.split_step(120)
+    a.step(400)
+    b.split_step(400)
     for c in range(6):
         fa, fb = a.get_field(c), b.get_field(c)
         err = rel(fa, fb)
         print(f"fused vs split comp={c} rel={err:.3e} bitwise={np.array_equal(fa, fb)}", flush=True)
-        assert err < 1e-6
+        assert err < 1e-5
 
 
 @pytest.mark.parametrize("tj", [4, 8, 16])
 def test_tile_shapes_agree(tj, monkeypatch):
-    p = small_params(O, steps=60)
+    p = small_params(O, steps=350)
     ref = fdtd_amd.Simulation(p)
-    ref.split_step(60)
+    ref.split_step(350)
     monkeypatch.setenv("FDTD_TJ", str(tj))
     monkeypatch.setenv("FDTD_ICHUNK", "7")
     s = fdtd_amd.Simulation(p)
-    s.step(60)
+    s.step(350)
     for c in range(6):
-        assert rel(s.get_field(c), ref.get_field(c)) < 1e-6
+        assert rel(s.get_field(c), ref.get_field(c)) < 1e-5
+
+
+def test_mfma_split_bf16_matches_fp64_contraction():
+    """The split-BF16 MFMA contraction against the exact FP64 contraction on the
+    same samples: checks the MFMA operand layout end to end and the split
+    precision (hi*hi + hi*lo + lo*hi keeps ~2^-16 relative error per product)."""
+    p = small_params(O, steps=450, block=16, nfreq=24)
+    a = fdtd_amd.Simulation(p)
+    p2 = small_params(O, steps=450, block=16, nfreq=24)
+    p2.dft_exact = 1
+    b = fdtd_amd.Simulation(p2)
+    a.step(450)
+    b.step(450)
+    worst = 0.0
+    for face in range(12):
+        for comp in range(4):
+            worst = max(worst, rel(a.get_dft(face, comp), b.get_dft(face, comp)))
+    print(f"mfma vs fp64 contraction worst rel={worst:.3e}", flush=True)
+    assert worst < 1e-4
+    fa, fb = a.flux(), b.flux()
+    assert rel(fa["ext"], fb["ext"]) < 1e-4
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_hydrodynamic_sphere(fused):
+    """NONLOCAL = 3: velocity/charge state on the metal box against the oracle."""
+    p = small_params(O, steps=400)
+    p.NONLOCAL = 3
+    o, s = run_pair(p, 400, fused)
+    assert s.get_info().hydro == 1
+    errs = field_errors(s, o)
+    print(f"hydro fused={fused} errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < TOL, errs

@@ -224,3 +224,34 @@ def test_source_spectrum():
     np.testing.assert_allclose(einc, expect, rtol=1e-9)
     lam = np.linspace(p.lam_min, p.lam_max, p.NUM_freq)
     np.testing.assert_allclose(freq, C0 / lam, rtol=1e-14)
+
+
+def test_hydro_local_limit_is_drude():
+    """NONLOCAL = 3 with the pressure term off (beta = 0) and linear charge:
+    the discrete velocity + current update must reproduce the Drude pole with
+    wp^2 = e^2 N_EQ / (eps0 m_e), to first order in w dt (the velocity is
+    driven by E^n and E^{n-1})."""
+    E0, ME = 1.60217662e-19, 9.10938356e-31
+    o, p = make()
+    p.NONLOCAL = 3
+    p.hydro_beta = 0.0
+    oh = O.Oracle(p)
+    c = oh.coeffs()
+    assert c["dNL"] == 0.0
+    dt, NEQ = c["dt"], c["N_EQ"]
+    lam = np.linspace(400e-9, 1000e-9, 13)
+    w = 2 * np.pi * C0 / lam
+    z = np.exp(1j * w * dt)
+    v_over_e = (c["d3"] + c["d4"] / z) / (z - c["d1"] - c["d2"] / z)
+    chi_h = -dt * E0 * NEQ * z * v_over_e / (EP0 * (z - 1))
+    chi_cp = 0
+    for i in range(2):
+        k1, k2, k3, k4, k5 = (c[f"{n}_{i}"] for n in ("c1", "c2", "c3", "c4", "c5"))
+        chi_cp = chi_cp + (k3 * z + k4 + k5 / z) / (z - k1 - k2 / z) / EP0
+    eps_h = c["eps_inf"] + chi_cp + chi_h
+    wp2 = E0 ** 2 * NEQ / (EP0 * ME)
+    assert np.sqrt(wp2) == pytest.approx(1.3202e16, rel=2e-4)
+    # continuum Drude with this wp and the gold damping, plus the same CP poles
+    ref = drude_cp_eps(w) + (1.3202e16 ** 2 - wp2) / (w ** 2 - 1j * 1.0805e14 * w)
+    err = np.abs(eps_h - ref) / np.abs(ref)
+    assert np.all(err < 5 * w * dt), (err, w * dt)
