@@ -41,6 +41,7 @@
 
 #include <cmath>
 #include <complex>
+#include <map>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -108,6 +109,14 @@ struct Oracle {
     std::vector<vec> dft;  // 12*4 arrays of size F*na*nb*2
     int face_na[12], face_nb[12];
     long long samples;
+    // design density (1 = the reference material) on ADE cells
+    vec rho;
+    int dlo[3] = {0, 0, 0}, dn[3] = {0, 0, 0};
+    // point objective J = sum_f sum_c |sum_t E_c^{t+1}(r0) e^{i w_f (t+1) dt}|^2
+    int obj_on = 0;
+    int r0[3] = {0, 0, 0};
+    vec obj_freq;
+    std::vector<std::complex<double>> g;   // [f][c]
 
     inline size_t I3(int i, int j, int k) const { return (size_t)k + (size_t)NZ * ((size_t)j + (size_t)NY * i); }
 };
@@ -591,9 +600,24 @@ void hydro_p(Oracle &o) {
 
 // ---------------------------------------------------------------- E update, main.cu:1048-1912
 
+// Coefficients of the E update at density r: eps_inf(r) = 1 + r (eps_inf - 1),
+// pole drives scaled by r (a linear blend of vacuum and the metal); r = 1 gives
+// the reference's C_E, C_E_1, C_E_2 (setup.cu:1306-1332) exactly.
+inline void ade_coefs(const Oracle &o, double r, double &CE, double &CE1, double &CE2) {
+    if (r == 1.0) { CE = o.C_E; CE1 = o.C_E_1; CE2 = o.C_E_2; return; }
+    const double S3 = o.d3 + o.c3[0] + o.c3[1], S4 = o.d4 + o.c4[0] + o.c4[1], S5 = o.d5 + o.c5[0] + o.c5[1];
+    const double e = EP0 * (1.0 + r * (o.eps_inf - 1.0));
+    CE = e + r * S3;
+    CE1 = e - r * S4;
+    CE2 = r * S5;
+}
+
 // Drude + 2 CP ADE branch of one E component (main.cu:1178-1219; ey 1480-1536; ez 1771-1817).
 inline double ade_update(Oracle &o, int c, size_t id, double curl) {
     double *e = (c == 0 ? o.ex.data() : c == 1 ? o.ey.data() : o.ez.data());
+    const double r = o.rho.empty() ? 1.0 : o.rho[id];
+    double CE, CE1, CE2;
+    ade_coefs(o, r, CE, CE1, CE2);
     double CP1 = (o.d1 - 1) * o.Pd_n[c][id], CP3 = o.d2 * o.Pd_n_1[c][id];
     double CP2 = 0, CP4 = 0;
     for (int n = 0; n < 2; n++) {
@@ -602,10 +626,10 @@ inline double ade_update(Oracle &o, int c, size_t id, double curl) {
     }
     o.e_n_1[c][id] = o.e_n[c][id];
     o.e_n[c][id] = e[id];
-    e[id] = (1 / o.C_E) * (o.dt * curl / Z0 + o.C_E_1 * o.e_n[c][id] - o.C_E_2 * o.e_n_1[c][id] - CP1 - CP2 - CP3 - CP4);
+    e[id] = (1 / CE) * (o.dt * curl / Z0 + CE1 * o.e_n[c][id] - CE2 * o.e_n_1[c][id] - CP1 - CP2 - CP3 - CP4);
     for (int n = 0; n < 2; n++)
-        o.Pcp[c][2 * id + n] = o.c1[n] * o.Pcp_n[c][2 * id + n] + o.c2[n] * o.Pcp_n_1[c][2 * id + n] + o.c3[n] * e[id] + o.c4[n] * o.e_n[c][id] + o.c5[n] * o.e_n_1[c][id];
-    o.Pd[c][id] = o.d1 * o.Pd_n[c][id] + o.d2 * o.Pd_n_1[c][id] + o.d3 * e[id] + o.d4 * o.e_n[c][id] + o.d5 * o.e_n_1[c][id];
+        o.Pcp[c][2 * id + n] = o.c1[n] * o.Pcp_n[c][2 * id + n] + o.c2[n] * o.Pcp_n_1[c][2 * id + n] + r * (o.c3[n] * e[id] + o.c4[n] * o.e_n[c][id] + o.c5[n] * o.e_n_1[c][id]);
+    o.Pd[c][id] = o.d1 * o.Pd_n[c][id] + o.d2 * o.Pd_n_1[c][id] + r * (o.d3 * e[id] + o.d4 * o.e_n[c][id] + o.d5 * o.e_n_1[c][id]);
     o.Pd_n_1[c][id] = o.Pd_n[c][id];
     o.Pd_n[c][id] = o.Pd[c][id];
     for (int n = 0; n < 2; n++) {
@@ -865,7 +889,294 @@ void dft_sample(Oracle &o) {
 void end_step(Oracle &o) {
     // main.cu:2920: DFT on every dft_stride-th step when StaticField == 0
     if (o.P.StaticField == 0 && o.t % o.P.dft_stride == 0) dft_sample(o);
+    if (o.obj_on) {
+        const size_t id = o.I3(o.r0[0], o.r0[1], o.r0[2]);
+        const double ev[3] = {o.ex[id], o.ey[id], o.ez[id]};
+        for (size_t f = 0; f < o.obj_freq.size(); f++) {
+            const std::complex<double> w = std::exp(std::complex<double>(0, 2 * PI * o.obj_freq[f] * (o.t + 1.0) * o.dt));
+            for (int c = 0; c < 3; c++) o.g[f * 3 + c] += ev[c] * w;
+        }
+    }
     o.t++;
+}
+
+double objective(const Oracle &o) {
+    double J = 0;
+    for (auto &v : o.g) J += std::norm(v);
+    return J;
+}
+
+// ---------------------------------------------------------------- discrete adjoint (no reference counterpart)
+//
+// Gradient of the point objective with respect to the design densities by the
+// transposed step run in reverse time.  The forward step is linear in the
+// state for fixed densities (the TFSF line is an external source), so the
+// adjoint applies, per step and in reverse order, the transposes of the E
+// update (vacuum or ADE, then the psi_E corrections) and of the H update
+// (main update, then psi_H).  Serial loops with scatter; small grids only.
+// Non-hydro models only.
+
+struct Adjoint {
+    std::map<const vec *, vec> a;   // adjoint of each state array
+    vec &of(const vec &v) {
+        vec &r = a[&v];
+        if (r.size() != v.size()) r.assign(v.size(), 0.0);
+        return r;
+    }
+};
+
+// Transpose of one psi recursion psi' = b psi + c (F(+) - F(-)) / d feeding
+// field += sgn * C * psi'.  ahat: adjoint of the updated field (already final).
+inline void psi_T(Adjoint &A, const vec &psi, size_t p, double b, double c, double d, double sgn, double C,
+                  double ahat, vec &aF, size_t fp, size_t fm) {
+    vec &ap = A.of(psi);
+    const double tot = ap[p] + sgn * C * ahat;
+    aF[fp] += c / d * tot;
+    aF[fm] -= c / d * tot;
+    ap[p] = b * tot;
+}
+
+void adjoint_e(Oracle &o, Adjoint &A, int t, const std::vector<double> &traj, const std::vector<size_t> &dcells,
+               const std::vector<int> &dmap, std::vector<double> &grad) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double Cee = o.Cee, Ceh = o.Ceh, dx = o.dx, dy = o.dy, dz = o.dz;
+    const int NcY = o.NcY, NcZ = o.NcZ;
+    vec *aE[3] = {&A.of(o.ex), &A.of(o.ey), &A.of(o.ez)};
+    vec *aH[3] = {&A.of(o.hx), &A.of(o.hy), &A.of(o.hz)};
+    const size_t nd = dcells.size();
+    const std::vector<int> *mats[3] = {&o.matX, &o.matY, &o.matZ};
+    for (int c = 0; c < 3; c++) {
+        for (int i = 0; i < NX; i++)
+            for (int j = 0; j < NY; j++)
+                for (int k = 0; k < NZ; k++) {
+                    bool dom;
+                    if (c == 0) dom = i < NX - 1 && j > 0 && j < NY - 1 && k > 0 && k < NZ - 1;
+                    else if (c == 1) dom = i > 0 && i < NX - 1 && j < NY - 1 && k > 0 && k < NZ - 1;
+                    else dom = i > 0 && i < NX - 1 && j > 0 && j < NY - 1 && k < NZ - 1;
+                    if (!dom) continue;
+                    const size_t id = o.I3(i, j, k);
+                    vec &aEc = *aE[c];
+                    double ah = aEc[id];
+                    // (b) transposed psi_E corrections, reverse of the forward order
+                    if (c == 0) {
+                        if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                            int j2 = j - o.cpml_F_Y;
+                            psi_T(A, o.psi_Ex_y_F, (size_t)k + NZ * ((size_t)j2 + (size_t)(NcY + 1) * i), o.be_y_F[j2], o.ce_y_F[j2], dy, +1, Ceh, ah, *aH[2], id, o.I3(i, j - 1, k));
+                        }
+                        if (j < o.cpml_N_Y + 1 && i < o.cpml_x_lim && k < o.cpml_z_lim)
+                            psi_T(A, o.psi_Ex_y_N, (size_t)k + NZ * ((size_t)j + (size_t)(NcY + 1) * i), o.be_y_N[j], o.ce_y_N[j], dy, +1, Ceh, ah, *aH[2], id, o.I3(i, j - 1, k));
+                        if (k >= o.cpml_F_Z && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                            int k2 = k - o.cpml_F_Z;
+                            psi_T(A, o.psi_Ex_z_F, (size_t)k2 + (NcZ + 1) * ((size_t)j + (size_t)NY * i), o.be_z_F[k2], o.ce_z_F[k2], dz, -1, Ceh, ah, *aH[1], id, o.I3(i, j, k - 1));
+                        }
+                        if (k < o.cpml_N_Z + 1 && i < o.cpml_x_lim && j < o.cpml_y_lim)
+                            psi_T(A, o.psi_Ex_z_N, (size_t)k + (NcZ + 1) * ((size_t)j + (size_t)NY * i), o.be_z_N[k], o.ce_z_N[k], dz, -1, Ceh, ah, *aH[1], id, o.I3(i, j, k - 1));
+                    } else if (c == 1) {
+                        if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                            int i2 = i - o.cpml_F_X;
+                            psi_T(A, o.psi_Ey_x_F, o.I3(i2, j, k), o.be_x_F[i2], o.ce_x_F[i2], dx, -1, Ceh, ah, *aH[2], id, o.I3(i - 1, j, k));
+                        }
+                        if (i < o.cpml_N_X + 1 && j < o.cpml_y_lim && k < o.cpml_z_lim)
+                            psi_T(A, o.psi_Ey_x_N, o.I3(i, j, k), o.be_x_N[i], o.ce_x_N[i], dx, -1, Ceh, ah, *aH[2], id, o.I3(i - 1, j, k));
+                        if (k >= o.cpml_F_Z && i < o.cpml_x_lim && j < o.cpml_y_lim) {
+                            int k2 = k - o.cpml_F_Z;
+                            psi_T(A, o.psi_Ey_z_F, (size_t)k2 + (NcZ + 1) * ((size_t)j + (size_t)NY * i), o.be_z_F[k2], o.ce_z_F[k2], dz, +1, Ceh, ah, *aH[0], id, o.I3(i, j, k - 1));
+                        }
+                        if (k < o.cpml_N_Z + 1 && i < o.cpml_x_lim && j < o.cpml_y_lim)
+                            psi_T(A, o.psi_Ey_z_N, (size_t)k + (NcZ + 1) * ((size_t)j + (size_t)NY * i), o.be_z_N[k], o.ce_z_N[k], dz, +1, Ceh, ah, *aH[0], id, o.I3(i, j, k - 1));
+                    } else {
+                        if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                            int i2 = i - o.cpml_F_X;
+                            psi_T(A, o.psi_Ez_x_F, o.I3(i2, j, k), o.be_x_F[i2], o.ce_x_F[i2], dx, +1, Ceh, ah, *aH[1], id, o.I3(i - 1, j, k));
+                        }
+                        if (i < o.cpml_N_X + 1 && j < o.cpml_y_lim && k < o.cpml_z_lim)
+                            psi_T(A, o.psi_Ez_x_N, o.I3(i, j, k), o.be_x_N[i], o.ce_x_N[i], dx, +1, Ceh, ah, *aH[1], id, o.I3(i - 1, j, k));
+                        if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                            int j2 = j - o.cpml_F_Y;
+                            psi_T(A, o.psi_Ez_y_F, (size_t)k + NZ * ((size_t)j2 + (size_t)(NcY + 1) * i), o.be_y_F[j2], o.ce_y_F[j2], dy, -1, Ceh, ah, *aH[0], id, o.I3(i, j - 1, k));
+                        }
+                        if (j < o.cpml_N_Y && i < o.cpml_x_lim && k < o.cpml_z_lim)
+                            psi_T(A, o.psi_Ez_y_N, (size_t)k + NZ * ((size_t)j + (size_t)(NcY + 1) * i), o.be_y_N[j], o.ce_y_N[j], dy, -1, Ceh, ah, *aH[0], id, o.I3(i, j - 1, k));
+                    }
+                    // (a) transposed main update: vacuum or ADE
+                    double Q;
+                    const int m = (*mats[c])[id];
+                    if (m > 1 && m < 6) {
+                        const double r = o.rho.empty() ? 1.0 : o.rho[id];
+                        double CE, CE1, CE2;
+                        ade_coefs(o, r, CE, CE1, CE2);
+                        const double k1[3] = {o.d1, o.c1[0], o.c1[1]}, k2[3] = {o.d2, o.c2[0], o.c2[1]};
+                        const double k3[3] = {o.d3, o.c3[0], o.c3[1]}, k4[3] = {o.d4, o.c4[0], o.c4[1]}, k5[3] = {o.d5, o.c5[0], o.c5[1]};
+                        vec &ae1 = A.of(o.e_n[c]);
+                        double *aP[3] = {&A.of(o.Pd_n[c])[id], &A.of(o.Pcp_n[c])[2 * id], &A.of(o.Pcp_n[c])[2 * id + 1]};
+                        double *aP1[3] = {&A.of(o.Pd_n_1[c])[id], &A.of(o.Pcp_n_1[c])[2 * id], &A.of(o.Pcp_n_1[c])[2 * id + 1]};
+                        double Pp[3], P1p[3];
+                        for (int q = 0; q < 3; q++) { Pp[q] = *aP[q]; P1p[q] = *aP1[q]; }
+                        double tot = ah;
+                        for (int q = 0; q < 3; q++) tot += r * k3[q] * Pp[q];
+                        Q = o.dt / (Z0 * CE) * tot;
+                        double newE = CE1 / CE * tot + ae1[id], newE1 = -CE2 / CE * tot;
+                        for (int q = 0; q < 3; q++) { newE += r * k4[q] * Pp[q]; newE1 += r * k5[q] * Pp[q]; }
+                        for (int q = 0; q < 3; q++) {
+                            *aP[q] = -(k1[q] - 1) / CE * tot + k1[q] * Pp[q] + P1p[q];
+                            *aP1[q] = -k2[q] / CE * tot + k2[q] * Pp[q];
+                        }
+                        if (!dmap.empty() && dmap[id] >= 0) {
+                            // d(step)/d(rho) at fixed inputs, contracted with the adjoint of the outputs
+                            const size_t v = (size_t)dmap[id];
+                            const double En1 = traj[((size_t)(t + 2) * 3 + c) * nd + v];   // E^{t+1}
+                            const double En = traj[((size_t)(t + 1) * 3 + c) * nd + v];    // E^t
+                            const double Em = traj[((size_t)t * 3 + c) * nd + v];          // E^{t-1}
+                            const double S3 = o.d3 + o.c3[0] + o.c3[1], S4 = o.d4 + o.c4[0] + o.c4[1], S5 = o.d5 + o.c5[0] + o.c5[1];
+                            const double dCE = EP0 * (o.eps_inf - 1.0) + S3, dCE1 = EP0 * (o.eps_inf - 1.0) - S4, dCE2 = S5;
+                            const double dE = (dCE1 * En - dCE2 * Em - En1 * dCE) / CE;
+                            double gsum = tot * dE;
+                            for (int q = 0; q < 3; q++) gsum += Pp[q] * (k3[q] * En1 + k4[q] * En + k5[q] * Em);
+                            grad[v] += gsum;
+                        }
+                        aEc[id] = newE;
+                        ae1[id] = newE1;
+                    } else {
+                        Q = Ceh * ah;
+                        aEc[id] = Cee * ah;
+                    }
+                    // transposed curl of H
+                    if (c == 0) {
+                        (*aH[2])[id] += Q / o.kedy[j]; (*aH[2])[o.I3(i, j - 1, k)] -= Q / o.kedy[j];
+                        (*aH[1])[id] -= Q / o.kedz[k]; (*aH[1])[o.I3(i, j, k - 1)] += Q / o.kedz[k];
+                    } else if (c == 1) {
+                        (*aH[0])[id] += Q / o.kedz[k]; (*aH[0])[o.I3(i, j, k - 1)] -= Q / o.kedz[k];
+                        (*aH[2])[id] -= Q / o.kedx[i]; (*aH[2])[o.I3(i - 1, j, k)] += Q / o.kedx[i];
+                    } else {
+                        (*aH[1])[id] += Q / o.kedx[i]; (*aH[1])[o.I3(i - 1, j, k)] -= Q / o.kedx[i];
+                        (*aH[0])[id] -= Q / o.kedy[j]; (*aH[0])[o.I3(i, j - 1, k)] += Q / o.kedy[j];
+                    }
+                }
+    }
+}
+
+void adjoint_h(Oracle &o, Adjoint &A) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double dx = o.dx, dy = o.dy, dz = o.dz, Chh = o.Chh, Che = o.Che;
+    const int NcY = o.NcY, NcZ = o.NcZ;
+    vec &aex = A.of(o.ex), &aey = A.of(o.ey), &aez = A.of(o.ez);
+    vec &ahx = A.of(o.hx), &ahy = A.of(o.hy), &ahz = A.of(o.hz);
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+            for (int k = 0; k < NZ; k++) {
+                const size_t id = o.I3(i, j, k);
+                if (i < NX && j < NY - 1 && k < NZ - 1) {  // hx
+                    const double ah = ahx[id];
+                    if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                        int j2 = j - o.cpml_F_Y;
+                        psi_T(A, o.psi_Hx_y_F, (size_t)k + NZ * ((size_t)j2 + (size_t)NcY * i), o.bh_y_F[j2], o.ch_y_F[j2], dy, -1, Che, ah, aez, o.I3(i, j + 1, k), id);
+                    }
+                    if (j < o.cpml_N_Y && i < o.cpml_x_lim && j < o.cpml_y_lim)
+                        psi_T(A, o.psi_Hx_y_N, (size_t)k + NZ * ((size_t)j + (size_t)NcY * i), o.bh_y_N[j], o.ch_y_N[j], dy, -1, Che, ah, aez, o.I3(i, j + 1, k), id);
+                    if (k >= o.cpml_F_Z && j < o.cpml_y_lim && i < o.cpml_x_lim) {
+                        int k2 = k - o.cpml_F_Z;
+                        psi_T(A, o.psi_Hx_z_F, (size_t)k2 + NcZ * ((size_t)j + (size_t)NY * i), o.bh_z_F[k2], o.ch_z_F[k2], dz, +1, Che, ah, aey, o.I3(i, j, k + 1), id);
+                    }
+                    if (k < o.cpml_N_Z && i < o.cpml_x_lim && j < o.cpml_y_lim)
+                        psi_T(A, o.psi_Hx_z_N, (size_t)k + NcZ * ((size_t)j + (size_t)NY * i), o.bh_z_N[k], o.ch_z_N[k], dz, +1, Che, ah, aey, o.I3(i, j, k + 1), id);
+                    aey[o.I3(i, j, k + 1)] += Che * ah / o.khdz[k]; aey[id] -= Che * ah / o.khdz[k];
+                    aez[o.I3(i, j + 1, k)] -= Che * ah / o.khdy[j]; aez[id] += Che * ah / o.khdy[j];
+                    ahx[id] = Chh * ah;
+                }
+                if (i < NX - 1 && j < NY && k < NZ - 1) {  // hy
+                    const double ah = ahy[id];
+                    if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        int i2 = i - o.cpml_F_X;
+                        psi_T(A, o.psi_Hy_x_F, o.I3(i2, j, k), o.bh_x_F[i2], o.ch_x_F[i2], dx, +1, Che, ah, aez, o.I3(i + 1, j, k), id);
+                    }
+                    if (i < o.cpml_N_X && j < o.cpml_y_lim && k < o.cpml_z_lim)
+                        psi_T(A, o.psi_Hy_x_N, o.I3(i, j, k), o.bh_x_N[i], o.ch_x_N[i], dx, +1, Che, ah, aez, o.I3(i + 1, j, k), id);
+                    if (k >= o.cpml_F_Z && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        int k2 = k - o.cpml_F_Z;
+                        psi_T(A, o.psi_Hy_z_F, (size_t)k2 + NcZ * ((size_t)j + (size_t)NY * i), o.bh_z_F[k2], o.ch_z_F[k2], dz, -1, Che, ah, aex, o.I3(i, j, k + 1), id);
+                    }
+                    if (k < o.cpml_N_Z && j < o.cpml_y_lim && k < o.cpml_z_lim)
+                        psi_T(A, o.psi_Hy_z_N, (size_t)k + NcZ * ((size_t)j + (size_t)NY * i), o.bh_z_N[k], o.ch_z_N[k], dz, -1, Che, ah, aex, o.I3(i, j, k + 1), id);
+                    aez[o.I3(i + 1, j, k)] += Che * ah / o.khdx[i]; aez[id] -= Che * ah / o.khdx[i];
+                    aex[o.I3(i, j, k + 1)] -= Che * ah / o.khdz[k]; aex[id] += Che * ah / o.khdz[k];
+                    ahy[id] = Chh * ah;
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ) {  // hz
+                    const double ah = ahz[id];
+                    if (j >= o.cpml_F_Y && i < o.cpml_x_lim && k < o.cpml_z_lim) {
+                        int j2 = j - o.cpml_F_Y;
+                        psi_T(A, o.psi_Hz_y_F, (size_t)k + NZ * ((size_t)j2 + (size_t)NcY * i), o.bh_y_F[j2], o.ch_y_F[j2], dy, +1, Che, ah, aex, o.I3(i, j + 1, k), id);
+                    }
+                    if (j < o.cpml_N_Y && i < o.cpml_x_lim && k < o.cpml_z_lim)
+                        psi_T(A, o.psi_Hz_y_N, (size_t)k + NZ * ((size_t)j + (size_t)NcY * i), o.bh_y_N[j], o.ch_y_N[j], dy, +1, Che, ah, aex, o.I3(i, j + 1, k), id);
+                    if (i >= o.cpml_F_X && j < o.cpml_y_lim && k < o.cpml_z_lim) {
+                        int i2 = i - o.cpml_F_X;
+                        psi_T(A, o.psi_Hz_x_F, o.I3(i2, j, k), o.bh_x_F[i2], o.ch_x_F[i2], dx, -1, Che, ah, aey, o.I3(i + 1, j, k), id);
+                    }
+                    if (i < o.cpml_N_X && j < o.cpml_y_lim && k < o.cpml_z_lim)
+                        psi_T(A, o.psi_Hz_x_N, o.I3(i, j, k), o.bh_x_N[i], o.ch_x_N[i], dx, -1, Che, ah, aey, o.I3(i + 1, j, k), id);
+                    aex[o.I3(i, j + 1, k)] += Che * ah / o.khdy[j]; aex[id] -= Che * ah / o.khdy[j];
+                    aey[o.I3(i + 1, j, k)] -= Che * ah / o.khdx[i]; aey[id] += Che * ah / o.khdx[i];
+                    ahz[id] = Chh * ah;
+                }
+            }
+}
+
+// Forward nsteps from the current (initial) state recording E at the design
+// cells, then the reverse sweep.  grad[v] = dJ/drho_v over the design block
+// (x fastest is NOT used: the block is i-major, k fastest).
+int forward_adjoint(Oracle &o, int nsteps, double *J, double *grad_out) {
+    if (o.hydro) return fail("adjoint: hydrodynamic model not supported");
+    if (!o.obj_on) return fail("adjoint: no objective set");
+    std::vector<size_t> dcells;
+    std::vector<int> dmap;
+    const size_t N = o.ex.size();
+    if (o.dn[0] > 0) {
+        dmap.assign(N, -1);
+        for (int a = 0; a < o.dn[0]; a++)
+            for (int b = 0; b < o.dn[1]; b++)
+                for (int c = 0; c < o.dn[2]; c++) {
+                    const size_t id = o.I3(o.dlo[0] + a, o.dlo[1] + b, o.dlo[2] + c);
+                    dmap[id] = (int)dcells.size();
+                    dcells.push_back(id);
+                }
+    }
+    const size_t nd = dcells.size();
+    std::vector<double> traj((size_t)(nsteps + 2) * 3 * nd, 0.0);   // E^{-1}, E^0, ..., E^nsteps
+    auto record = [&](int slot) {
+        for (size_t v = 0; v < nd; v++) {
+            traj[((size_t)slot * 3 + 0) * nd + v] = o.ex[dcells[v]];
+            traj[((size_t)slot * 3 + 1) * nd + v] = o.ey[dcells[v]];
+            traj[((size_t)slot * 3 + 2) * nd + v] = o.ez[dcells[v]];
+        }
+    };
+    const int t0 = o.t;
+    record(1);
+    for (int s = 0; s < nsteps; s++) {
+        update_h(o); tfsf(o); update_e(o); end_step(o);
+        record(s + 2);
+    }
+    *J = objective(o);
+    std::vector<double> grad(nd, 0.0);
+    Adjoint A;
+    const size_t r0 = o.I3(o.r0[0], o.r0[1], o.r0[2]);
+    vec *aE[3] = {&A.of(o.ex), &A.of(o.ey), &A.of(o.ez)};
+    for (int s = nsteps - 1; s >= 0; s--) {
+        const int t = t0 + s;
+        // dJ/dE_c^{t+1}(r0) = sum_f 2 Re(conj(g_fc) e^{i w_f (t+1) dt})
+        for (int c = 0; c < 3; c++) {
+            double a = 0;
+            for (size_t f = 0; f < o.obj_freq.size(); f++) {
+                const std::complex<double> w = std::exp(std::complex<double>(0, 2 * PI * o.obj_freq[f] * (t + 1.0) * o.dt));
+                a += 2.0 * std::real(std::conj(o.g[f * 3 + c]) * w);
+            }
+            (*aE[c])[r0] += a;
+        }
+        adjoint_e(o, A, s, traj, dcells, dmap, grad);
+        adjoint_h(o, A);
+    }
+    for (size_t v = 0; v < nd; v++) grad_out[v] = grad[v];
+    return FDTD_OK;
 }
 
 // ---------------------------------------------------------------- flux, Spectral.cu:411-1177
@@ -1067,5 +1378,35 @@ int orc_cpml(void *h, int axis, int which, double *out) {
 }
 
 int orc_threads(void) { return omp_get_max_threads(); }
+
+// Design block [lo, lo+n) with densities rho (i-major, k fastest): every E
+// component there takes the ADE branch with coefficients blended by rho.
+int orc_set_design(void *h, const int *lo, const int *n, const double *rho) {
+    Oracle &o = *(Oracle *)h;
+    if (o.rho.empty()) o.rho.assign(o.ex.size(), 1.0);
+    for (int a = 0; a < 3; a++) { o.dlo[a] = lo[a]; o.dn[a] = n[a]; }
+    size_t v = 0;
+    for (int a = 0; a < n[0]; a++)
+        for (int b = 0; b < n[1]; b++)
+            for (int c = 0; c < n[2]; c++) {
+                const size_t id = o.I3(lo[0] + a, lo[1] + b, lo[2] + c);
+                o.rho[id] = rho[v++];
+                o.matX[id] = o.matY[id] = o.matZ[id] = o.P.material;
+            }
+    return FDTD_OK;
+}
+
+int orc_set_objective(void *h, const int *r0, int nf, const double *freq) {
+    Oracle &o = *(Oracle *)h;
+    for (int a = 0; a < 3; a++) o.r0[a] = r0[a];
+    o.obj_freq.assign(freq, freq + nf);
+    o.g.assign((size_t)nf * 3, 0.0);
+    o.obj_on = 1;
+    return FDTD_OK;
+}
+
+double orc_objective(void *h) { return objective(*(Oracle *)h); }
+
+int orc_adjoint(void *h, int nsteps, double *J, double *grad) { return forward_adjoint(*(Oracle *)h, nsteps, J, grad); }
 
 }  // extern "C"

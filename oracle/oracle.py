@@ -46,6 +46,10 @@ def load():
         "orc_coeffs": (ip, [vp, dp]),
         "orc_cpml": (ip, [vp, ip, ip, dp]),
         "orc_threads": (ip, []),
+        "orc_set_design": (ip, [vp, C.POINTER(C.c_int * 3), C.POINTER(C.c_int * 3), dp]),
+        "orc_set_objective": (ip, [vp, C.POINTER(C.c_int * 3), ip, dp]),
+        "orc_objective": (C.c_double, [vp]),
+        "orc_adjoint": (ip, [vp, ip, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -169,3 +173,28 @@ class Oracle:
 
     def threads(self):
         return self.lib.orc_threads()
+
+    def set_design(self, lo, rho):
+        """Density block at lo with shape rho.shape (i-major, k fastest)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        self.design_shape = rho.shape
+        a, b = (C.c_int * 3)(*lo), (C.c_int * 3)(*rho.shape)
+        self.lib.orc_set_design(self.h, C.byref(a), C.byref(b), _ptr(rho.reshape(-1)))
+
+    def set_objective(self, r0, freqs):
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        a = (C.c_int * 3)(*r0)
+        self.lib.orc_set_objective(self.h, C.byref(a), len(f), _ptr(f))
+
+    def objective(self):
+        return self.lib.orc_objective(self.h)
+
+    def adjoint(self, nsteps):
+        """Runs nsteps forward then the reverse sweep; returns (J, dJ/drho)."""
+        n = int(np.prod(self.design_shape))
+        g = np.empty(n)
+        J = C.c_double()
+        rc = self.lib.orc_adjoint(self.h, int(nsteps), C.byref(J), _ptr(g))
+        if rc != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return J.value, g.reshape(self.design_shape)
