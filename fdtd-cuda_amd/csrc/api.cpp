@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -56,8 +57,29 @@ struct fdtd_sim {
     int K = 64, slot = 0;
     std::vector<int> slot_t;
     int samples = 0;
+    // design region and adjoint
+    int des_lo[3] = {0, 0, 0}, des_n[3] = {0, 0, 0};
+    std::vector<float> des_rho;     // design densities (i-major, k fastest)
+    float *rho_box = nullptr;       // density per dispersive-box cell (1 outside the design block)
+    bool sampling = true;           // DFT sampling on (off while the adjoint recomputes a segment)
+    float *probe = nullptr;         // E(r0) per step while the adjoint's forward runs
+    long probe_p = 0;
+    int probe_t0 = 0;               // step whose E^{n+1} lands in probe[0..2]
+    float *traj_step[3] = {};       // dispersive kernel writes E^{n+1} here when set
+    float *traj_first[2][3] = {};   // and E^{n-1}, E^n here for the first recorded step
+    struct Adj {
+        bool ready = false;
+        float *E[2][3] = {}, *H[2][3] = {};
+        float *psiE[2][6] = {}, *psiH[2][6] = {};
+        float *st[3][7] = {};
+        float *Qb[3] = {}, *Eb[3] = {};
+        double *grad = nullptr;
+    } adj;
     // misc
     float *dense = nullptr;         // NX*NY*NZ staging for field copies
+    size_t fsz = 0;                 // floats per padded field allocation
+    long fbase = 0;                 // offset of element (0,0,0) in it
+    size_t hsz[6] = {}, esz[6] = {};
     int t = 0;
     int tj = 8, ichunk = 32;
     bool timing = false;
@@ -172,7 +194,6 @@ DispDev make_disp(fdtd_sim *s, int parity) {
     for (int a = 0; a < 3; a++) { d.lo[a] = s->box_lo[a]; d.n[a] = s->box_n[a]; d.ik_e[a] = s->ik_e[a]; }
     for (int c = 0; c < 3; c++) {
         d.id[c] = s->disp_id[c];
-        d.rho[c] = nullptr;
         for (int q = 0; q < 8; q++) d.st[c][q] = s->disp_st[c][q];
         d.H[c] = s->H[parity][c];
         d.E[c] = s->E[parity][c];
@@ -194,6 +215,23 @@ DispDev make_disp(fdtd_sim *s, int parity) {
         d.p3[q] = (float)(k[q][2] / e0);
         d.p4[q] = (float)(k[q][3] / e0);
         d.p5[q] = (float)(k[q][4] / e0);
+    }
+    d.rho = s->rho_box;
+    {
+        const double S3 = A.d[2] + A.c[0][2] + A.c[1][2], S4 = A.d[3] + A.c[0][3] + A.c[1][3], S5 = A.d[4] + A.c[0][4] + A.c[1][4];
+        d.u1 = (float)((A.eps_inf - 1.0) + S3 / e0);
+        d.u2 = (float)((A.eps_inf - 1.0) - S4 / e0);
+        d.u3 = (float)(S5 / e0);
+        d.A0 = (float)(S.dt / (fdtd::kZ0 * e0));
+        for (int q = 0; q < 3; q++) {
+            d.G0[q] = q == 0 ? 0.f : (float)((k[q][0] - 1.0) + k[q][1]);
+            d.K2[q] = (float)k[q][1];
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        d.traj[c] = s->traj_step[c];
+        d.traj_prev[0][c] = s->traj_first[0][c];
+        d.traj_prev[1][c] = s->traj_first[1][c];
     }
     d.hydro = S.hydro ? 1 : 0;
     if (S.hydro) {
@@ -266,6 +304,19 @@ int build_dispersive(fdtd_sim *s) {
                 return rc;
         }
     }
+    s->rho_box = nullptr;
+    if (!s->des_rho.empty()) {
+        std::vector<float> rb(s->nbox, 1.0f);
+        for (int a = 0; a < s->des_n[0]; a++)
+            for (int b = 0; b < s->des_n[1]; b++)
+                for (int c = 0; c < s->des_n[2]; c++) {
+                    const int bi = s->des_lo[0] + a - s->box_lo[0], bj = s->des_lo[1] + b - s->box_lo[1], bk = s->des_lo[2] + c - s->box_lo[2];
+                    rb[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk] = s->des_rho[((size_t)a * s->des_n[1] + b) * s->des_n[2] + c];
+                }
+        int rc2 = upload(s, &s->rho_box, rb);
+        if (rc2) return rc2;
+    }
+    s->adj.ready = false;
     return FDTD_OK;
 }
 
@@ -362,6 +413,8 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
     g.sxy = (long)g.sz * (g.NY + 2);
     const size_t fsz = (size_t)g.sxy * (g.NX + 2);
     const long base = g.sxy + g.sz + 1;
+    s->fsz = fsz;
+    s->fbase = base;
     for (int q = 0; q < 2; q++)
         for (int c = 0; c < 3; c++) {
             float *a, *b;
@@ -376,6 +429,7 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
     const size_t ez_ = (size_t)NX * NY * 2 * (ncz + 1), ey_ = (size_t)NX * 2 * (ncy + 1) * NZ, ex_ = (size_t)2 * (ncx + 1) * NY * NZ;
     const size_t hsz[6] = {hz_, hy_, hz_, hx_, hx_, hy_};
     const size_t esz[6] = {ez_, ey_, ez_, ex_, ey_, ex_};
+    for (int a = 0; a < 6; a++) { s->hsz[a] = hsz[a]; s->esz[a] = esz[a]; }
     for (int q = 0; q < 2; q++)
         for (int a = 0; a < 6; a++)
             if ((rc = dalloc(s, &s->psiH[q][a], hsz[a]))) return fail(rc);
@@ -493,6 +547,272 @@ int fdtd_add_sphere(fdtd_sim *s, const double centre[3], double radius) {
     return FDTD_OK;
 }
 
+int fdtd_set_design(fdtd_sim *s, const int lo[3], const int n[3], const float *rho) {
+    if (!s || !lo || !n || !rho) return FDTD_ERR_INVALID;
+    Setup &S = s->S;
+    for (int a = 0; a < 3; a++) {
+        if (n[a] < 1 || lo[a] <= S.abs_s[a] || lo[a] + n[a] - 1 >= S.abs_e[a] - 1)
+            return set_err(s, FDTD_ERR_INVALID, "design block must lie strictly inside the absorption box");
+    }
+    if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "design densities need the local Drude-CP model");
+    const size_t nv = (size_t)n[0] * n[1] * n[2];
+    for (size_t v = 0; v < nv; v++)
+        if (!(rho[v] >= 0.f && rho[v] <= 1.f)) return set_err(s, FDTD_ERR_INVALID, "densities must lie in [0, 1]");
+    const int NY = S.N[1], NZ = S.N[2];
+    std::vector<uint8_t> m[3] = {S.mat[1], S.mat[2], S.mat[3]};
+    for (int a = 0; a < n[0]; a++)
+        for (int b = 0; b < n[1]; b++)
+            for (int c = 0; c < n[2]; c++) {
+                const size_t id = ((size_t)(lo[0] + a) * NY + lo[1] + b) * NZ + lo[2] + c;
+                for (int q = 0; q < 3; q++) m[q][id] = (uint8_t)S.P.material;
+            }
+    for (int a = 0; a < 3; a++) { s->des_lo[a] = lo[a]; s->des_n[a] = n[a]; }
+    s->des_rho.assign(rho, rho + nv);
+    return fdtd_set_materials(s, m[0].data(), m[1].data(), m[2].data());
+}
+
+namespace {
+
+// Full forward state at a step boundary (fields at the current parity, psi,
+// dispersive box, 1D line) in device memory.
+struct Checkpoint {
+    std::vector<void *> bufs;
+    int t = 0, p = 0, pp = 0, le = 0;
+};
+
+int ckpt_copy(fdtd_sim *s, Checkpoint &ck, bool save) {
+    std::vector<std::pair<void *, size_t>> items;   // (live pointer, bytes)
+    for (int c = 0; c < 3; c++) {
+        items.push_back({s->E[s->p][c] - s->fbase, s->fsz * sizeof(float)});
+        items.push_back({s->H[s->p][c] - s->fbase, s->fsz * sizeof(float)});
+    }
+    for (int a = 0; a < 6; a++) items.push_back({s->psiH[s->pp][a], s->hsz[a] * sizeof(float)});
+    for (int a = 0; a < 6; a++) items.push_back({s->psiE[a], s->esz[a] * sizeof(float)});
+    if (s->nbox)
+        for (int c = 0; c < 3; c++)
+            for (int q = 0; q < 8; q++) items.push_back({s->disp_st[c][q], s->nbox * sizeof(float)});
+    items.push_back({s->line_e[s->le], s->S.L * sizeof(double)});
+    items.push_back({s->line_h, s->S.L * sizeof(double)});
+    items.push_back({s->mur, 2 * sizeof(double)});
+    if (save) {
+        for (auto &it : items) {
+            void *b = nullptr;
+            if (hipMalloc(&b, it.second) != hipSuccess) return set_err(s, FDTD_ERR_ALLOC, "checkpoint allocation failed");
+            ck.bufs.push_back(b);
+            HIPCHK(s, hipMemcpyAsync(b, it.first, it.second, hipMemcpyDeviceToDevice, s->stream));
+        }
+        ck.t = s->t; ck.p = s->p; ck.pp = s->pp; ck.le = s->le;
+    } else {
+        for (size_t n = 0; n < items.size(); n++)
+            HIPCHK(s, hipMemcpyAsync(items[n].first, ck.bufs[n], items[n].second, hipMemcpyDeviceToDevice, s->stream));
+    }
+    return FDTD_OK;
+}
+
+void ckpt_free(Checkpoint &ck) {
+    for (void *b : ck.bufs) (void)hipFree(b);
+    ck.bufs.clear();
+}
+
+int adj_alloc(fdtd_sim *s) {
+    if (s->adj.ready) return FDTD_OK;
+    int rc;
+    for (int q = 0; q < 2; q++)
+        for (int c = 0; c < 3; c++) {
+            float *a, *b;
+            if ((rc = dalloc(s, &a, s->fsz)) || (rc = dalloc(s, &b, s->fsz))) return rc;
+            s->adj.E[q][c] = a + s->fbase;
+            s->adj.H[q][c] = b + s->fbase;
+        }
+    for (int q = 0; q < 2; q++)
+        for (int a = 0; a < 6; a++)
+            if ((rc = dalloc(s, &s->adj.psiE[q][a], s->esz[a])) || (rc = dalloc(s, &s->adj.psiH[q][a], s->hsz[a]))) return rc;
+    for (int c = 0; c < 3; c++) {
+        for (int q = 0; q < 7; q++)
+            if ((rc = dalloc(s, &s->adj.st[c][q], s->nbox))) return rc;
+        if ((rc = dalloc(s, &s->adj.Qb[c], s->nbox)) || (rc = dalloc(s, &s->adj.Eb[c], s->nbox))) return rc;
+    }
+    if ((rc = dalloc(s, &s->adj.grad, s->nbox))) return rc;
+    s->adj.ready = true;
+    return FDTD_OK;
+}
+
+int adj_zero(fdtd_sim *s) {
+    for (int q = 0; q < 2; q++)
+        for (int c = 0; c < 3; c++) {
+            HIPCHK(s, hipMemsetAsync(s->adj.E[q][c] - s->fbase, 0, s->fsz * sizeof(float), s->stream));
+            HIPCHK(s, hipMemsetAsync(s->adj.H[q][c] - s->fbase, 0, s->fsz * sizeof(float), s->stream));
+        }
+    for (int q = 0; q < 2; q++)
+        for (int a = 0; a < 6; a++) {
+            HIPCHK(s, hipMemsetAsync(s->adj.psiE[q][a], 0, s->esz[a] * sizeof(float), s->stream));
+            HIPCHK(s, hipMemsetAsync(s->adj.psiH[q][a], 0, s->hsz[a] * sizeof(float), s->stream));
+        }
+    for (int c = 0; c < 3; c++)
+        for (int q = 0; q < 7; q++) HIPCHK(s, hipMemsetAsync(s->adj.st[c][q], 0, s->nbox * sizeof(float), s->stream));
+    HIPCHK(s, hipMemsetAsync(s->adj.grad, 0, s->nbox * sizeof(double), s->stream));
+    return FDTD_OK;
+}
+
+}  // namespace
+
+namespace {
+
+struct AdjScratch {
+    float *probe = nullptr;             // E(r0) per forward step
+    float *traj = nullptr;              // E on the box for one segment: slots E^{t-1}, E^t, ..., E^{t+len}
+    std::vector<Checkpoint> cks;
+};
+
+int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3], int nf, const double *freq,
+                double *J, double *grad);
+
+}  // namespace
+
+int fdtd_adjoint(fdtd_sim *s, int nsteps, int ckpt, const int r0[3], int nf, const double *freq, double *J,
+                 double *grad) {
+    if (!s || !r0 || !freq || !J || !grad || nsteps < 1 || ckpt < 1 || nf < 1) return FDTD_ERR_INVALID;
+    const Setup &S = s->S;
+    if (s->des_rho.empty() || !s->rho_box) return set_err(s, FDTD_ERR_INVALID, "adjoint: no design region set");
+    if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "adjoint: hydrodynamic model not supported");
+    for (int a = 0; a < 3; a++)
+        if (r0[a] < 0 || r0[a] >= S.N[a]) return set_err(s, FDTD_ERR_INVALID, "adjoint: objective cell outside the grid");
+    for (int f = 0; f < nf; f++)
+        if (!std::isfinite(freq[f])) return set_err(s, FDTD_ERR_INVALID, "adjoint: non-finite frequency");
+    // the transposed dispersive update assumes no dispersive cell carries CPML terms
+    for (int a = 0; a < 3; a++)
+        if (s->box_lo[a] < S.cp[a].Nc || s->box_lo[a] + s->box_n[a] > S.cp[a].F)
+            return set_err(s, FDTD_ERR_UNSUPPORTED, "adjoint: dispersive cells inside the CPML");
+    AdjScratch W;
+    const int rc = adjoint_run(s, W, nsteps, ckpt, r0, nf, freq, J, grad);
+    (void)hipStreamSynchronize(s->stream);
+    for (auto &c : W.cks) ckpt_free(c);
+    if (W.probe) (void)hipFree(W.probe);
+    if (W.traj) (void)hipFree(W.traj);
+    s->probe = nullptr;
+    s->sampling = true;
+    for (int c = 0; c < 3; c++) s->traj_step[c] = s->traj_first[0][c] = s->traj_first[1][c] = nullptr;
+    return rc;
+}
+
+namespace {
+
+int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3], int nf, const double *freq,
+                double *J, double *grad) {
+    const Setup &S = s->S;
+    int rc;
+    if ((rc = adj_alloc(s))) return rc;
+    const int t0 = s->t;
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    const long nbox = s->nbox;
+    if (hipMalloc((void **)&W.probe, (size_t)3 * nsteps * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&W.traj, (size_t)(ckpt + 2) * 3 * nbox * sizeof(float)) != hipSuccess)
+        return set_err(s, FDTD_ERR_ALLOC, "adjoint: scratch allocation failed");
+    float *const probe = W.probe, *const traj = W.traj;
+    std::vector<Checkpoint> &cks = W.cks;
+    cks.resize(nseg + 1);
+    // forward with checkpoints and the objective probe
+    s->probe = probe;
+    s->probe_p = gidx(s->g, r0[0], r0[1], r0[2]);
+    s->probe_t0 = t0;
+    for (int seg = 0; seg < nseg; seg++) {
+        if ((rc = ckpt_copy(s, cks[seg], true))) return rc;
+        const int len = std::min(ckpt, nsteps - seg * ckpt);
+        if ((rc = fdtd_step(s, len))) return rc;
+    }
+    s->probe = nullptr;
+    if ((rc = ckpt_copy(s, cks[nseg], true))) return rc;
+    std::vector<float> pr((size_t)3 * nsteps);
+    if (hipStreamSynchronize(s->stream) != hipSuccess ||
+        hipMemcpy(pr.data(), probe, pr.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_err(s, FDTD_ERR_DEVICE, "adjoint: probe copy failed");
+    // objective in FP64
+    std::vector<std::complex<double>> gsum((size_t)nf * 3, 0.0);
+    for (int t = t0; t < t0 + nsteps; t++)
+        for (int f = 0; f < nf; f++) {
+            const std::complex<double> w = std::exp(std::complex<double>(0, 2 * fdtd::kPI * freq[f] * (t + 1.0) * S.dt));
+            for (int c = 0; c < 3; c++) gsum[(size_t)f * 3 + c] += (double)pr[(size_t)3 * (t - t0) + c] * w;
+        }
+    double Jv = 0;
+    for (auto &v : gsum) Jv += std::norm(v);
+    *J = Jv;
+    // reverse sweep
+    if ((rc = adj_zero(s))) return rc;
+    s->sampling = false;
+    int pa = 0, pe = 0, ph = 0;
+    const long p0 = gidx(s->g, r0[0], r0[1], r0[2]);
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        Checkpoint &ck = cks[seg];
+        s->t = ck.t; s->p = ck.p; s->pp = ck.pp; s->le = ck.le;
+        if ((rc = ckpt_copy(s, ck, false))) return rc;
+        const int len = std::min(ckpt, nsteps - seg * ckpt);
+        for (int st = 0; st < len; st++) {
+            for (int c = 0; c < 3; c++) {
+                s->traj_step[c] = traj + ((size_t)(st + 2) * 3 + c) * nbox;
+                s->traj_first[0][c] = st == 0 ? traj + ((size_t)0 * 3 + c) * nbox : nullptr;
+                s->traj_first[1][c] = st == 0 ? traj + ((size_t)1 * 3 + c) * nbox : nullptr;
+            }
+            if ((rc = fdtd_step(s, 1))) return rc;
+        }
+        for (int c = 0; c < 3; c++) s->traj_step[c] = s->traj_first[0][c] = s->traj_first[1][c] = nullptr;
+        for (int st = len - 1; st >= 0; st--) {
+            const int t = ck.t + st;
+            double a[3] = {0, 0, 0};
+            for (int f = 0; f < nf; f++) {
+                const std::complex<double> w = std::exp(std::complex<double>(0, 2 * fdtd::kPI * freq[f] * (t + 1.0) * S.dt));
+                for (int c = 0; c < 3; c++) a[c] += 2.0 * std::real(std::conj(gsum[(size_t)f * 3 + c]) * w);
+            }
+            HIPCHK(s, launch_adj_inject(s->adj.E[pa], p0, (float)a[0], (float)a[1], (float)a[2], s->stream));
+            AdjDispDev ad{};
+            ad.d = make_disp(s, s->p);
+            for (int c = 0; c < 3; c++) {
+                ad.Ehat[c] = s->adj.E[pa][c];
+                for (int q = 0; q < 7; q++) ad.a[c][q] = s->adj.st[c][q];
+                ad.Qb[c] = s->adj.Qb[c];
+                ad.Eb[c] = s->adj.Eb[c];
+                ad.tm1[c] = traj + ((size_t)st * 3 + c) * nbox;
+                ad.t0[c] = traj + ((size_t)(st + 1) * 3 + c) * nbox;
+                ad.tp1[c] = traj + ((size_t)(st + 2) * 3 + c) * nbox;
+            }
+            ad.grad = s->adj.grad;
+            HIPCHK(s, launch_adj_disp(ad, s->stream));
+            AdjDev av{};
+            const StepDev sd = make_step(s, false);
+            av.g = sd.g; av.c = sd.c;
+            for (int x = 0; x < 3; x++) av.cp[x] = sd.cp[x];
+            for (int c = 0; c < 3; c++) {
+                av.Ein[c] = s->adj.E[pa][c]; av.Hin[c] = s->adj.H[pa][c];
+                av.Eout[c] = s->adj.E[1 - pa][c]; av.Hout[c] = s->adj.H[1 - pa][c];
+                av.bid[c] = s->disp_id[c]; av.Qb[c] = s->adj.Qb[c]; av.Eb[c] = s->adj.Eb[c];
+            }
+            for (int x = 0; x < 6; x++) {
+                av.psiE_in.p[x] = s->adj.psiE[pe][x]; av.psiE_out.p[x] = s->adj.psiE[1 - pe][x];
+                av.psiH_in.p[x] = s->adj.psiH[ph][x]; av.psiH_out.p[x] = s->adj.psiH[1 - ph][x];
+            }
+            for (int x = 0; x < 3; x++) { av.blo[x] = s->box_lo[x]; av.bn[x] = s->box_n[x]; }
+            HIPCHK(s, launch_adj_step(av, s->tj, s->ichunk, s->stream));
+            pa ^= 1; pe ^= 1; ph ^= 1;
+        }
+    }
+    // gradient on the design block
+    std::vector<double> gb(nbox);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(gb.data(), s->adj.grad, nbox * sizeof(double), hipMemcpyDeviceToHost));
+    size_t v = 0;
+    for (int a = 0; a < s->des_n[0]; a++)
+        for (int b = 0; b < s->des_n[1]; b++)
+            for (int c = 0; c < s->des_n[2]; c++) {
+                const int bi = s->des_lo[0] + a - s->box_lo[0], bj = s->des_lo[1] + b - s->box_lo[1], bk = s->des_lo[2] + c - s->box_lo[2];
+                grad[v++] = gb[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk];
+            }
+    // leave the forward state after nsteps
+    Checkpoint &fin = cks[nseg];
+    s->t = fin.t; s->p = fin.p; s->pp = fin.pp; s->le = fin.le;
+    return ckpt_copy(s, fin, false);
+}
+
+}  // namespace
+
 int fdtd_get_materials(const fdtd_sim *s, int comp, uint8_t *out) {
     if (!s || !out || comp < 0 || comp > 3) return FDTD_ERR_INVALID;
     std::memcpy(out, s->S.mat[comp].data(), s->S.mat[comp].size());
@@ -564,7 +884,7 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
             d.Eout[c] = s->E[1 - q][c]; d.Hout[c] = s->H[1 - q][c];
         }
         for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
-        const bool samp = sampling_step(s);
+        const bool samp = s->sampling && sampling_step(s);
         if (s->timing) {
             hipEvent_t a = next_event(s, 0), b = next_event(s, 1);
             if (!a || !b) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
@@ -578,6 +898,10 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         if (s->nbox) {
             DispDev dd = make_disp(s, 1 - q);
             HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
+        }
+        if (s->probe) {
+            const float *E1[3] = {s->E[1 - q][0], s->E[1 - q][1], s->E[1 - q][2]};
+            HIPCHK(s, launch_probe(E1, s->probe_p, s->probe + 3 * (size_t)(s->t - s->probe_t0), s->stream));
         }
         s->p ^= 1;
         s->pp ^= 1;

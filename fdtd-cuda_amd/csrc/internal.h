@@ -95,7 +95,13 @@ struct DispDev {
     Geom g;
     int lo[3], n[3];
     const uint8_t *id[3];           // per E component: 1 = dispersive, 0 = not
-    const float *rho[3];            // design density per E component, or null (rho = 1)
+    const float *rho;               // design density per box cell, or null (rho = 1 everywhere)
+    // density blend (eps_inf(rho) = 1 + rho (eps_inf - 1), pole drives x rho), all divided by eps0:
+    // C_E/eps0 = 1 + rho u1, C_E1/eps0 = 1 + rho u2, C_E2/eps0 = rho u3
+    float u1, u2, u3, A0;           // A0 = dt / (z0 eps0)
+    float G0[3], K2[3];             // k1 + k2 - 1 and k2 per pole
+    float *traj[3];                 // if set: E^{n+1} of every box cell is written here (adjoint recompute)
+    float *traj_prev[2][3];         // if set (first recorded step): E^{n-1}, E^n written here
     // state per component, box-dense SoA: 0 E^n, 1 E^{n-1}, then per pole q
     // (0 Drude, 1-2 CP) P~^n at 2+2q and J~^n = P~^n - P~^{n-1} at 3+2q, P~ = P/eps0
     float *st[3][8];
@@ -114,6 +120,33 @@ struct DispDev {
     float h_d1m1, h_d3, h_d4;       // v^{n+1} = v + (d1-1) J + d3 E^n + d4 E^{n-1} + h_nl (gd + gd2) / cbrt(1 + n~)
     float h_nl, h_inv_d, h_cont;    // h_nl = beta^2/dd, 1/dx, 2 dt/dx
     float h_cur;                    // dt e0 N_EQ / C_E: current term of the E update
+};
+
+// Adjoint step arguments.  The transposed step has the stencil shape of the
+// forward step: the adjoint of H^{n+1/2} gathers forward differences of the
+// E-side weights V, the adjoint of E^n gathers backward differences of the
+// H-side weights W.
+struct AdjDev {
+    Geom g;
+    CoefDev c;
+    CpmlDev cp[3];
+    const float *Ein[3], *Hin[3];   // adjoint of E^{n+1}, H^{n+1/2}
+    float *Eout[3], *Hout[3];       // adjoint of E^n, H^{n-1/2}
+    PsiSet psiE_in, psiE_out, psiH_in, psiH_out;
+    // dispersive box: curl weight Q and the E^n adjoint base from k_adj_disp
+    int blo[3], bn[3];
+    const uint8_t *bid[3];
+    const float *Qb[3], *Eb[3];
+};
+
+// Adjoint of the dispersive update on the box (run before k_adj_step).
+struct AdjDispDev {
+    DispDev d;                      // forward coefficients, geometry, rho
+    const float *Ehat[3];           // adjoint of E^{n+1} (field layout)
+    float *a[3][7];                 // adjoint state: 0 E^{n-1}, 1+2q P_q, 2+2q J_q
+    float *Qb[3], *Eb[3];
+    const float *tm1[3], *t0[3], *tp1[3];   // E^{n-1}, E^n, E^{n+1} on the box (gradient)
+    double *grad;                   // dJ/drho per box cell, or null
 };
 
 // 1D incident line (main.cu:2280-2337, 2831-2859), FP64.
@@ -154,5 +187,12 @@ hipError_t launch_pack_field(const float *f, const Geom &g, float *dense, int to
 hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t st);
 // Hydrodynamic model: velocity update, then per component (x, y, z) continuity and E update.
 hipError_t launch_hydro(const DispDev &d, hipStream_t st);
+
+// Adjoint: inject dJ/dE^{n+1}(r0), transposed dispersive update, fused transposed step.
+hipError_t launch_adj_inject(float *const E[3], long p0, float ax, float ay, float az, hipStream_t st);
+hipError_t launch_adj_disp(const AdjDispDev &a, hipStream_t st);
+hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, hipStream_t st);
+// E at one cell for every step (objective probe).
+hipError_t launch_probe(const float *const E[3], long p0, float *out, hipStream_t st);
 
 #endif

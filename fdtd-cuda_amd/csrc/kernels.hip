@@ -446,15 +446,29 @@ __global__ void k_dispersive(DispDev d) {
         float P[3], J[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) { P[q] = st[2 + 2 * q][idx]; J[q] = st[3 + 2 * q][idx]; }
-        float e = d.a0 * curl + d.a1 * en - d.a2 * en1;
+        float a0 = d.a0, a1 = d.a1, a2 = d.a2, r = 1.f;
+        float e0[3] = {d.e0[0], d.e0[1], d.e0[2]}, e1[3] = {d.e1[0], d.e1[1], d.e1[2]};
+        if (d.rho) {
+            // vacuum-to-metal blend of the ADE coefficients at density r
+            r = d.rho[idx];
+            const float inv = 1.f / (1.f + r * d.u1);
+            a0 = d.A0 * inv;
+            a1 = (1.f + r * d.u2) * inv;
+            a2 = r * d.u3 * inv;
 #pragma unroll
-        for (int q = 0; q < 3; q++) e -= d.e0[q] * P[q] + d.e1[q] * J[q];
+            for (int q = 0; q < 3; q++) { e0[q] = d.G0[q] * inv; e1[q] = -d.K2[q] * inv; }
+        }
+        float e = a0 * curl + a1 * en - a2 * en1;
+#pragma unroll
+        for (int q = 0; q < 3; q++) e -= e0[q] * P[q] + e1[q] * J[q];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            const float jn = d.g1[q] * J[q] + d.g0[q] * P[q] + d.p3[q] * e + d.p4[q] * en + d.p5[q] * en1;
+            const float jn = d.g1[q] * J[q] + d.g0[q] * P[q] + r * (d.p3[q] * e + d.p4[q] * en + d.p5[q] * en1);
             st[2 + 2 * q][idx] = P[q] + jn;
             st[3 + 2 * q][idx] = jn;
         }
+        if (d.traj[c]) d.traj[c][idx] = e;
+        if (d.traj_prev[0][c]) { d.traj_prev[0][c][idx] = en1; d.traj_prev[1][c][idx] = en; }
         st[1][idx] = en;
         st[0][idx] = e;
         d.E[c][p] = e;
@@ -836,6 +850,266 @@ __global__ void k_pack(const float *f, Geom g, float *dense, int to_dense) {
     else const_cast<float *>(f)[p] = dense[d];
 }
 
+// ----------------------------------------------------------------- discrete adjoint
+//
+// No reference counterpart.  The forward step is linear in the state for
+// fixed densities (TFSF is an external source), so the adjoint applies the
+// transposed updates in reverse time.  Weights (derived from the forward
+// update equations and their CPML recursions):
+//   E side, per E cell with Q = Ceh * Ehat (vacuum) or a0 * Ehat_tot (ADE):
+//     Vxy = Q ik_y + c_y (psi^ex_y + Q)   Vxz = Q ik_z - c_z (psi^ex_z - Q)
+//     Vyz = Q ik_z + c_z (psi^ey_z + Q)   Vyx = Q ik_x - c_x (psi^ey_x - Q)
+//     Vzx = Q ik_x + c_x (psi^ez_x + Q)   Vzy = Q ik_y - c_y (psi^ez_y - Q)
+//   H side, per H cell with G = Che * Hhat_tot:
+//     Wxz = G ik_z + c_z (psi^hx_z + G)   Wxy = G ik_y - c_y (psi^hx_y - G)
+//     Wyx = G ik_x + c_x (psi^hy_x + G)   Wyz = G ik_z - c_z (psi^hy_z - G)
+//     Wzy = G ik_y + c_y (psi^hz_y + G)   Wzx = G ik_x - c_x (psi^hz_x - G)
+//   Hhat_tot = Hhat' + forward differences of V; Ehat = base + backward
+//   differences of W; each psi adjoint becomes b * (its bracket).
+
+__device__ inline bool e_dom(const Geom &g, int c, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0) return false;
+    if (c == 0) return i < g.NX - 1 && j > 0 && j < g.NY - 1 && k > 0 && k < g.NZ - 1;
+    if (c == 1) return i > 0 && i < g.NX - 1 && j < g.NY - 1 && k > 0 && k < g.NZ - 1;
+    return i > 0 && i < g.NX - 1 && j > 0 && j < g.NY - 1 && k < g.NZ - 1;
+}
+__device__ inline bool h_dom(const Geom &g, int c, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0) return false;
+    if (c == 0) return i < g.NX && j < g.NY - 1 && k < g.NZ - 1;
+    if (c == 1) return i < g.NX - 1 && j < g.NY && k < g.NZ - 1;
+    return i < g.NX - 1 && j < g.NY - 1 && k < g.NZ;
+}
+
+__device__ inline long adj_box(const AdjDev &a, int c, int i, int j, int k) {
+    const int bi = i - a.blo[0], bj = j - a.blo[1], bk = k - a.blo[2];
+    if (bi < 0 || bj < 0 || bk < 0 || bi >= a.bn[0] || bj >= a.bn[1] || bk >= a.bn[2]) return -1;
+    const long b = ((long)bi * a.bn[1] + bj) * a.bn[2] + bk;
+    return a.bid[c][b] ? b : -1;
+}
+
+// E-side weights of component c at (i,j,k) (va, vb in the order of the table
+// above) and the base of its E^n adjoint; wr: owner updates psi_E.
+__device__ inline void adj_v(const AdjDev &a, int c, int i, int j, int k, float eh, bool wr, float &va, float &vb,
+                             float &base) {
+    va = vb = base = 0.f;
+    const Geom &g = a.g;
+    if (!e_dom(g, c, i, j, k)) return;
+    const CpmlDev &cx = a.cp[0], &cy = a.cp[1], &cz = a.cp[2];
+    if (a.bn[0] > 0) {
+        const long b = adj_box(a, c, i, j, k);
+        if (b >= 0) {
+            const float Q = a.Qb[c][b];
+            base = a.Eb[c][b];
+            if (c == 0) { va = Q * cy.ik_e[j]; vb = Q * cz.ik_e[k]; }
+            else if (c == 1) { va = Q * cz.ik_e[k]; vb = Q * cx.ik_e[i]; }
+            else { va = Q * cx.ik_e[i]; vb = Q * cy.ik_e[j]; }
+            return;
+        }
+    }
+    const float Q = a.c.Ceh * eh;
+    base = a.c.Cee * eh;
+    const bool pin = i < g.NX - 1 && j < g.NY - 1 && k < g.NZ - 1;
+    const int lx = pin ? e_layer(cx, i) : -1, ly = pin ? e_layer(cy, j) : -1, lz = pin ? e_layer(cz, k) : -1;
+    const int Lz = 2 * (cz.Nc + 1), Ly = 2 * (cy.Nc + 1);
+    if (c == 0) {
+        va = Q * cy.ik_e[j];
+        vb = Q * cz.ik_e[k];
+        if (ly >= 0) { const long q = yslab(g, Ly, i, ly, k); const float t = a.psiE_in.p[1][q] + Q; va += cy.ce[ly] * t; if (wr) a.psiE_out.p[1][q] = cy.be[ly] * t; }
+        if (lz >= 0) { const long q = zslab(g, Lz, i, j, lz); const float t = a.psiE_in.p[0][q] - Q; vb -= cz.ce[lz] * t; if (wr) a.psiE_out.p[0][q] = cz.be[lz] * t; }
+    } else if (c == 1) {
+        va = Q * cz.ik_e[k];
+        vb = Q * cx.ik_e[i];
+        if (lz >= 0) { const long q = zslab(g, Lz, i, j, lz); const float t = a.psiE_in.p[2][q] + Q; va += cz.ce[lz] * t; if (wr) a.psiE_out.p[2][q] = cz.be[lz] * t; }
+        if (lx >= 0) { const long q = xslab(g, lx, j, k); const float t = a.psiE_in.p[3][q] - Q; vb -= cx.ce[lx] * t; if (wr) a.psiE_out.p[3][q] = cx.be[lx] * t; }
+    } else {
+        va = Q * cx.ik_e[i];
+        vb = Q * cy.ik_e[j];
+        if (lx >= 0) { const long q = xslab(g, lx, j, k); const float t = a.psiE_in.p[5][q] + Q; va += cx.ce[lx] * t; if (wr) a.psiE_out.p[5][q] = cx.be[lx] * t; }
+        if (ly >= 0) { const long q = yslab(g, Ly, i, ly, k); const float t = a.psiE_in.p[4][q] - Q; vb -= cy.ce[ly] * t; if (wr) a.psiE_out.p[4][q] = cy.be[ly] * t; }
+    }
+}
+
+// H-side weights of component c at (i,j,k) from Hhat_tot; returns the H^{n-1/2} adjoint.
+__device__ inline float adj_w(const AdjDev &a, int c, int i, int j, int k, float ht, bool wr, float &wa, float &wb) {
+    wa = wb = 0.f;
+    const Geom &g = a.g;
+    if (!h_dom(g, c, i, j, k)) return 0.f;
+    const CpmlDev &cx = a.cp[0], &cy = a.cp[1], &cz = a.cp[2];
+    const float G = a.c.Che * ht;
+    const bool pin = i < g.NX - 1 && j < g.NY - 1 && k < g.NZ - 1;
+    const int lx = pin ? h_layer(cx, i) : -1, ly = pin ? h_layer(cy, j) : -1, lz = pin ? h_layer(cz, k) : -1;
+    const int Lz = 2 * cz.Nc, Ly = 2 * cy.Nc;
+    if (c == 0) {
+        wa = G * cz.ik_h[k];
+        wb = G * cy.ik_h[j];
+        if (lz >= 0) { const long q = zslab(g, Lz, i, j, lz); const float t = a.psiH_in.p[0][q] + G; wa += cz.ch[lz] * t; if (wr) a.psiH_out.p[0][q] = cz.bh[lz] * t; }
+        if (ly >= 0) { const long q = yslab(g, Ly, i, ly, k); const float t = a.psiH_in.p[1][q] - G; wb -= cy.ch[ly] * t; if (wr) a.psiH_out.p[1][q] = cy.bh[ly] * t; }
+    } else if (c == 1) {
+        wa = G * cx.ik_h[i];
+        wb = G * cz.ik_h[k];
+        if (lx >= 0) { const long q = xslab(g, lx, j, k); const float t = a.psiH_in.p[3][q] + G; wa += cx.ch[lx] * t; if (wr) a.psiH_out.p[3][q] = cx.bh[lx] * t; }
+        if (lz >= 0) { const long q = zslab(g, Lz, i, j, lz); const float t = a.psiH_in.p[2][q] - G; wb -= cz.ch[lz] * t; if (wr) a.psiH_out.p[2][q] = cz.bh[lz] * t; }
+    } else {
+        wa = G * cy.ik_h[j];
+        wb = G * cx.ik_h[i];
+        if (ly >= 0) { const long q = yslab(g, Ly, i, ly, k); const float t = a.psiH_in.p[5][q] + G; wa += cy.ch[ly] * t; if (wr) a.psiH_out.p[5][q] = cy.bh[ly] * t; }
+        if (lx >= 0) { const long q = xslab(g, lx, j, k); const float t = a.psiH_in.p[4][q] - G; wb -= cx.ch[lx] * t; if (wr) a.psiH_out.p[4][q] = cx.bh[lx] * t; }
+    }
+    return a.c.Chh * ht;
+}
+
+// Fused transposed step, the twin of k_fused_step: same tiling, same halo
+// rows/lanes, LDS exchange of the W weights needed at j-1 and k-1, W at i-1
+// carried in registers, V at i+1 computed one plane ahead.
+template <int TJ>
+__global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk) {
+    __shared__ float sW[2][4][TJ][65];   // 0 Wyz, 1 Wxz (k-1);  2 Wzy, 3 Wxy (j-1)
+    const Geom &g = a.g;
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int k = blockIdx.x * 63 - 1 + lane;
+    const int j = blockIdx.y * (TJ - 1) - 1 + row;
+    const int i0 = blockIdx.z * ichunk;
+    const int i1 = min(i0 + ichunk, g.NX);
+    const bool own = lane > 0 && row > 0 && k < g.NZ && j < g.NY;
+    const int kc = min(k, g.NZ - 1), jc = min(j, g.NY - 1);
+    const int kk = kc == k ? k : g.NZ, jj = jc == j ? j : g.NY;   // beyond the grid: no domain
+    const long sz = g.sz, sxy = g.sxy;
+    const long cidx = (long)jc * sz + kc;
+    const float *Ex = a.Ein[0], *Ey = a.Ein[1], *Ez = a.Ein[2];
+
+    // V weights (and bases) of the own cell at plane i, carried one plane ahead
+    float vxy, vxz, vyz, vyx, vzx, vzy, bx, by, bz;
+    float wzxP = 0.f, wyxP = 0.f;   // W at plane i-1
+    auto vplane = [&](int ip, bool wr, float &Vxy, float &Vxz, float &Vyz, float &Vyx, float &Vzx, float &Vzy, float &Bx,
+                      float &By, float &Bz) {
+        const long p = (long)ip * sxy + cidx;
+        adj_v(a, 0, ip, jj, kk, Ex[p], wr, Vxy, Vxz, Bx);
+        adj_v(a, 1, ip, jj, kk, Ey[p], wr, Vyz, Vyx, By);
+        adj_v(a, 2, ip, jj, kk, Ez[p], wr, Vzx, Vzy, Bz);
+    };
+    // adjoint of H^{n+1/2} at plane ip (all lanes), given V at ip and V at ip+1
+    auto htot = [&](int ip, float Vxy, float Vxz, float Vyz, float Vyx, float Vzx, float Vzy, float nVzx, float nVyx,
+                    float &hx, float &hy, float &hz) {
+        const long p = (long)ip * sxy + cidx;
+        float d0, d1, d2, t;
+        // neighbours at k+1 (ex, ey) and j+1 (ex, ez), recomputed from the read-only adjoint level
+        float vyz1, vxz1, vzy1, vxy1;
+        adj_v(a, 0, ip, jj, kk + 1, Ex[p + 1], false, d0, vxz1, t);
+        adj_v(a, 1, ip, jj, kk + 1, Ey[p + 1], false, vyz1, d1, t);
+        adj_v(a, 0, ip, jj + 1, kk, Ex[p + sz], false, vxy1, d2, t);
+        adj_v(a, 2, ip, jj + 1, kk, Ez[p + sz], false, d0, vzy1, t);
+        hx = a.Hin[0][p] + Vyz - vyz1 - Vzy + vzy1;
+        hy = a.Hin[1][p] - Vxz + vxz1 + Vzx - nVzx;
+        hz = a.Hin[2][p] + Vxy - vxy1 - Vyx + nVyx;
+    };
+
+    vplane(i0, own, vxy, vxz, vyz, vyx, vzx, vzy, bx, by, bz);
+    if (i0 > 0) {
+        float pxy, pxz, pyz, pyx, pzx, pzy, t0, t1, t2;
+        vplane(i0 - 1, false, pxy, pxz, pyz, pyx, pzx, pzy, t0, t1, t2);
+        float hx, hy, hz, wa, wb;
+        htot(i0 - 1, pxy, pxz, pyz, pyx, pzx, pzy, vzx, vyx, hx, hy, hz);
+        adj_w(a, 1, i0 - 1, jj, kk, hy, false, wyxP, wb);
+        adj_w(a, 2, i0 - 1, jj, kk, hz, false, wa, wzxP);
+    }
+    int buf = 0;
+    for (int i = i0; i < i1; i++) {
+        const long p = (long)i * sxy + cidx;
+        float nxy, nxz, nyz, nyx, nzx, nzy, nbx, nby, nbz;
+        vplane(i + 1, own && i + 1 < i1, nxy, nxz, nyz, nyx, nzx, nzy, nbx, nby, nbz);
+        float hx, hy, hz;
+        htot(i, vxy, vxz, vyz, vyx, vzx, vzy, nzx, nyx, hx, hy, hz);
+        float wxz, wxy, wyx, wyz, wzy, wzx;
+        const float hxi = adj_w(a, 0, i, jj, kk, hx, own, wxz, wxy);
+        const float hyi = adj_w(a, 1, i, jj, kk, hy, own, wyx, wyz);
+        const float hzi = adj_w(a, 2, i, jj, kk, hz, own, wzy, wzx);
+        if (own) { a.Hout[0][p] = hxi; a.Hout[1][p] = hyi; a.Hout[2][p] = hzi; }
+        sW[buf][0][row][lane] = wyz;
+        sW[buf][1][row][lane] = wxz;
+        sW[buf][2][row][lane] = wzy;
+        sW[buf][3][row][lane] = wxy;
+        __syncthreads();
+        if (own) {
+            const float wyz_k = sW[buf][0][row][lane - 1], wxz_k = sW[buf][1][row][lane - 1];
+            const float wzy_j = sW[buf][2][row - 1][lane], wxy_j = sW[buf][3][row - 1][lane];
+            const bool dx = e_dom(g, 0, i, j, k), dy = e_dom(g, 1, i, j, k), dz = e_dom(g, 2, i, j, k);
+            a.Eout[0][p] = dx ? bx - wyz_k + wyz + wzy_j - wzy : 0.f;
+            a.Eout[1][p] = dy ? by + wxz_k - wxz - wzxP + wzx : 0.f;
+            a.Eout[2][p] = dz ? bz - wxy_j + wxy + wyxP - wyx : 0.f;
+        }
+        wzxP = wzx; wyxP = wyx;
+        vxy = nxy; vxz = nxz; vyz = nyz; vyx = nyx; vzx = nzx; vzy = nzy;
+        bx = nbx; by = nby; bz = nbz;
+        buf ^= 1;
+    }
+}
+
+// Adjoint of the (P, J) dispersive update at every dispersive box cell:
+// given the adjoint of (E^{n+1}, E^n, P^{n+1}, J^{n+1}) returns that of
+// (E^n, E^{n-1}, P^n, J^n), the curl weight Q = a0 Ehat_tot for k_adj_step,
+// and accumulates dJ/drho = Ehat_tot dE'/drho + sum_q Jhat_tot (p3 E' + p4 E + p5 E1).
+__global__ void k_adj_disp(AdjDispDev A) {
+    const DispDev &d = A.d;
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    const int bk = (int)(idx % d.n[2]);
+    const int bj = (int)((idx / d.n[2]) % d.n[1]);
+    const int bi = (int)(idx / ((long)d.n[2] * d.n[1]));
+    const long p = gidx(d.g, d.lo[0] + bi, d.lo[1] + bj, d.lo[2] + bk);
+    double gsum = 0.0;
+    for (int c = 0; c < 3; c++) {
+        if (!d.id[c][idx]) continue;
+        float a0 = d.a0, a1 = d.a1, a2 = d.a2, r = 1.f, inv = 0.f;
+        float e0[3] = {d.e0[0], d.e0[1], d.e0[2]}, e1[3] = {d.e1[0], d.e1[1], d.e1[2]};
+        if (d.rho) {
+            r = d.rho[idx];
+            inv = 1.f / (1.f + r * d.u1);
+            a0 = d.A0 * inv;
+            a1 = (1.f + r * d.u2) * inv;
+            a2 = r * d.u3 * inv;
+            for (int q = 0; q < 3; q++) { e0[q] = d.G0[q] * inv; e1[q] = -d.K2[q] * inv; }
+        }
+        float *const *s = A.a[c];
+        float Jt[3];
+        float et = A.Ehat[c][p];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            Jt[q] = s[2 + 2 * q][idx] + s[1 + 2 * q][idx];   // J' feeds P' = P + J'
+            et += r * d.p3[q] * Jt[q];
+        }
+        float eb = a1 * et + s[0][idx], e1n = -a2 * et;
+#pragma unroll
+        for (int q = 0; q < 3; q++) { eb += r * d.p4[q] * Jt[q]; e1n += r * d.p5[q] * Jt[q]; }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float ph = s[1 + 2 * q][idx];
+            s[1 + 2 * q][idx] = ph + d.g0[q] * Jt[q] - e0[q] * et;
+            s[2 + 2 * q][idx] = d.g1[q] * Jt[q] - e1[q] * et;
+        }
+        s[0][idx] = e1n;
+        A.Qb[c][idx] = a0 * et;
+        A.Eb[c][idx] = eb;
+        if (A.grad && d.rho) {
+            const float Ep = A.tp1[c][idx], En = A.t0[c][idx], Em = A.tm1[c][idx];
+            const float dE = (d.u2 * En - d.u3 * Em - d.u1 * Ep) * inv;
+            double gc = (double)et * dE;
+#pragma unroll
+            for (int q = 0; q < 3; q++) gc += (double)Jt[q] * (d.p3[q] * Ep + d.p4[q] * En + d.p5[q] * Em);
+            gsum += gc;
+        }
+    }
+    if (A.grad && d.rho) A.grad[idx] += gsum;
+}
+
+__global__ void k_adj_inject(float *ex, float *ey, float *ez, long p0, float ax, float ay, float az) {
+    if (threadIdx.x == 0) { ex[p0] += ax; ey[p0] += ay; ez[p0] += az; }
+}
+
+__global__ void k_probe(const float *ex, const float *ey, const float *ez, long p0, float *out) {
+    if (threadIdx.x == 0) { out[0] = ex[p0]; out[1] = ey[p0]; out[2] = ez[p0]; }
+}
+
 }  // namespace
 
 // ================================================================ launch wrappers
@@ -862,6 +1136,35 @@ hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, hipStream_t s
         default:
             return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, hipStream_t st) {
+    const int nkt = (a.g.NZ + 62) / 63;
+    const int nchunk = (a.g.NX + ichunk - 1) / ichunk;
+    switch (tj) {
+        case 4: hipLaunchKernelGGL(k_adj_step<4>, dim3(nkt, (a.g.NY + 2) / 3, nchunk), dim3(64, 4), 0, st, a, ichunk); break;
+        case 8: hipLaunchKernelGGL(k_adj_step<8>, dim3(nkt, (a.g.NY + 6) / 7, nchunk), dim3(64, 8), 0, st, a, ichunk); break;
+        case 16: hipLaunchKernelGGL(k_adj_step<16>, dim3(nkt, (a.g.NY + 14) / 15, nchunk), dim3(64, 16), 0, st, a, ichunk); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_disp(const AdjDispDev &a, hipStream_t st) {
+    const long nb = (long)a.d.n[0] * a.d.n[1] * a.d.n[2];
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_disp, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_inject(float *const E[3], long p0, float ax, float ay, float az, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, st, E[0], E[1], E[2], p0, ax, ay, az);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(const float *const E[3], long p0, float *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, st, E[0], E[1], E[2], p0, out);
     return hipGetLastError();
 }
 

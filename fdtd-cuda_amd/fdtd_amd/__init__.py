@@ -19,6 +19,7 @@ EXPORTS = [
     "fdtd_set_materials", "fdtd_add_sphere", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
     "fdtd_dft_accumulate", "fdtd_step", "fdtd_sync", "fdtd_get_field", "fdtd_set_field",
     "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
+    "fdtd_set_design", "fdtd_adjoint",
 ]
 
 _lib = None
@@ -55,6 +56,8 @@ def load_library(path=LIB_PATH):
         "fdtd_get_dft": (ip, [vp, ip, ip, dp]),
         "fdtd_flux": (ip, [vp, dp, dp, dp, dp]),
         "fdtd_kernel_timing": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
+        "fdtd_set_design": (ip, [vp, C.POINTER(C.c_int * 3), C.POINTER(C.c_int * 3), fp]),
+        "fdtd_adjoint": (ip, [vp, ip, ip, C.POINTER(C.c_int * 3), ip, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -122,6 +125,26 @@ class Simulation:
         c = (C.c_double * 3)(*[float(x) for x in centre])
         self._check(self.lib.fdtd_add_sphere(self.h, C.byref(c), float(radius)))
         self.info = self.get_info()
+
+    def set_design(self, lo, rho):
+        """Design block of densities rho (shape n, i-major, k fastest) at cell lo."""
+        r = np.ascontiguousarray(rho, dtype=np.float32)
+        self.design_shape = r.shape
+        a, b = (C.c_int * 3)(*[int(x) for x in lo]), (C.c_int * 3)(*r.shape)
+        self._check(self.lib.fdtd_set_design(self.h, C.byref(a), C.byref(b), _ptr(r.reshape(-1), C.c_float)))
+        self.info = self.get_info()
+
+    def adjoint(self, nsteps, r0, freqs, checkpoint=64):
+        """nsteps forward steps, then the discrete adjoint sweep of the point
+        objective J = sum_f sum_c |sum_t E_c^{t+1}(r0) exp(i 2 pi f (t+1) dt)|^2.
+        Returns (J, dJ/drho over the design block)."""
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        g = np.empty(int(np.prod(self.design_shape)))
+        J = C.c_double()
+        r = (C.c_int * 3)(*[int(x) for x in r0])
+        self._check(self.lib.fdtd_adjoint(self.h, int(nsteps), int(checkpoint), C.byref(r), len(f),
+                                          _ptr(f, C.c_double), C.byref(J), _ptr(g, C.c_double)))
+        return J.value, g.reshape(self.design_shape)
 
     def get_materials(self, comp):
         out = np.empty(self.shape, dtype=np.uint8)

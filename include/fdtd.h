@@ -136,6 +136,21 @@ int  fdtd_set_materials(fdtd_sim *sim, const uint8_t *id_x, const uint8_t *id_y,
  * the dimer and other multi-particle geometries; same validity rules as
  * fdtd_set_materials. */
 int  fdtd_add_sphere(fdtd_sim *sim, const double centre[3], double radius);
+/* Design region (no reference counterpart; inverse-design workloads): the block
+ * [lo, lo+n) of cells becomes dispersive for all three E components with the
+ * ADE coefficients blended by the density rho in [0, 1] (i-major, k fastest):
+ * eps_inf(rho) = 1 + rho (eps_inf - 1) and every pole drive scaled by rho,
+ * so rho = 0 is vacuum and rho = 1 the configured metal. */
+int  fdtd_set_design(fdtd_sim *sim, const int lo[3], const int n[3], const float *rho);
+/* Discrete adjoint (no reference counterpart).  From the current state (t = 0)
+ * runs nsteps forward with a full-state checkpoint every ckpt steps, then the
+ * transposed step in reverse time, recomputing each segment from its
+ * checkpoint.  Objective J = sum_f sum_c |sum_t E_c^{t+1}(r0) e^{i 2 pi f_j (t+1) dt}|^2
+ * over the nf frequencies freq (Hz) at cell r0.  Writes J and dJ/drho for the
+ * design block (same order as fdtd_set_design).  The field state afterwards is
+ * the forward state after nsteps. */
+int  fdtd_adjoint(fdtd_sim *sim, int nsteps, int ckpt, const int r0[3], int nf, const double *freq,
+                  double *J, double *grad);
 /* Copies a mask out (comp 0: mat_matrix, 1..3: X, Y, Z). */
 int  fdtd_get_materials(const fdtd_sim *sim, int comp, uint8_t *out);
 

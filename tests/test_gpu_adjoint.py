@@ -1,0 +1,101 @@
+"""The GPU discrete adjoint (fdtd_set_design / fdtd_adjoint) against the FP64
+oracle adjoint, whose own gradient is pinned by central differences in
+test_oracle_adjoint.py.
+
+FP32 on the GPU against FP64: J within 1e-3 relative, dJ/drho within 2e-3 of
+the largest gradient entry.  Checkpoint spacing must not change a single bit
+of the gradient (every segment is recomputed by the same kernels), and the
+state left behind is the plain forward state after nsteps.
+"""
+import numpy as np
+import pytest
+
+import fdtd_amd as G
+import oracle as O
+from conftest import small_params
+
+C0 = 2.99792458e8
+LAMS = np.array([633e-9, 785e-9, 850e-9])
+STEPS = 260
+
+
+def problem(mod, shape=(5, 4, 3), seed=3):
+    p = small_params(mod, n=20, ncpml=5, sphere=0, steps=STEPS)
+    p.inc_phi = 90.0
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.1, 0.9, size=shape)
+    T = 20 + 2 * 5 + 1
+    c = T // 2
+    lo = (c - shape[0] // 2, c - shape[1] // 2, c - shape[2] // 2)
+    r0 = (c, c + 1, lo[2] + shape[2] + 1)
+    return p, rho, lo, r0
+
+
+def oracle_adjoint(shape=(5, 4, 3)):
+    p, rho, lo, r0 = problem(O, shape)
+    o = O.Oracle(p)
+    o.set_design(lo, rho.astype(np.float32).astype(np.float64))
+    o.set_objective(r0, C0 / LAMS)
+    return o.adjoint(STEPS)
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_oracle():
+    p, rho, lo, r0 = problem(G)
+    sim = G.Simulation(p)
+    sim.set_design(lo, rho)
+    J, g = sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=50)
+    Jo, go = oracle_adjoint()
+    print("J", J, Jo, "max|g|", np.abs(go).max(), "err", np.abs(g - go).max(), flush=True)
+    assert J == pytest.approx(Jo, rel=1e-3)
+    assert np.abs(g - go).max() < 2e-3 * np.abs(go).max()
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_checkpoint_spacing_is_exact():
+    p, rho, lo, r0 = problem(G)
+    out = []
+    for ck in (7, 64, STEPS):
+        sim = G.Simulation(p)
+        sim.set_design(lo, rho)
+        out.append(sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=ck))
+        sim.close()
+    for J, g in out[1:]:
+        assert J == out[0][0]
+        assert np.array_equal(g, out[0][1])
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_leaves_forward_state():
+    p, rho, lo, r0 = problem(G)
+    a = G.Simulation(p)
+    a.set_design(lo, rho)
+    a.adjoint(STEPS, r0, C0 / LAMS, checkpoint=40)
+    b = G.Simulation(p)
+    b.set_design(lo, rho)
+    b.step(STEPS)
+    for c in range(6):
+        assert np.array_equal(a.get_field(c), b.get_field(c)), c
+    fa, fb = a.flux(), b.flux()
+    assert np.array_equal(fa["ext"], fb["ext"])
+    # and the simulation carries on from there
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.get_field(G.EX), b.get_field(G.EX))
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_autograd_wrapper():
+    torch = pytest.importorskip("torch")
+    from fdtd_amd.design import DesignObjective
+    p, rho, lo, r0 = problem(G)
+    fn = DesignObjective()
+    r = torch.tensor(rho, dtype=torch.float64, requires_grad=True)
+    J = fn.apply(r, p, lo, r0, C0 / LAMS, STEPS, 32)
+    (3.0 * J).backward()
+    sim = G.Simulation(p)
+    sim.set_design(lo, rho)
+    J2, g2 = sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=32)
+    assert float(J) == J2
+    assert np.array_equal(r.grad.numpy(), 3.0 * g2)
+
