@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB -> 1 GiB copy kernel 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs each simulation on a single MI355X "
@@ -120,6 +122,9 @@ def main():
     abytes = algorithmic_bytes(info, p)
     k_avg_s = (k_ms / 1e3) / max(k_n, 1)
     achieved = abytes / k_avg_s / 1e9
+
+    if args.calibrate_copy:
+        copy_calibration()
 
     cpu = None
     if not args.no_cpu:
@@ -160,6 +165,24 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(nbytes=1 << 30, reps=4):
+    """A plain device-to-device copy of nbytes (read nbytes, write nbytes; the
+    HIP runtime runs it as its copyBuffer kernel) so the FETCH_SIZE /
+    WRITE_SIZE counters can be scaled against a known size."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def cpu_baseline(p, info, nsteps):

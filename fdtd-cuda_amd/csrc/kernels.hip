@@ -66,31 +66,66 @@ __device__ inline float inc_interp(const TfsfDev &t, bool hline, float di, float
 //   follows c cyclically, else +/-.
 // Applied to H after the H update and to E^n before the E update, as the
 // reference's CORRECT kernels run between UPDATE_h* and UPDATE_e*.
+__device__ inline int pick3(int a, int x, int y, int z) { return a == 0 ? x : (a == 1 ? y : z); }
+__device__ inline float pick3f(int a, float x, float y, float z) { return a == 0 ? x : (a == 1 ? y : z); }
+
+// One face term: H component c (or E if efield) on the face normal to axis a.
+// Indices are picked with selects, not local arrays, so a runtime (c, a)
+// never puts the cell on the stack.
+__device__ inline float tfsf_term(const TfsfDev &t, bool efield, int c, int a, int i, int j, int k) {
+    const int b = 3 - a - c;
+    const int xa = pick3(a, i, j, k), xb = pick3(b, i, j, k), xc = pick3(c, i, j, k);
+    const int t0a = pick3(a, t.t0[0], t.t0[1], t.t0[2]), t1a = pick3(a, t.t1[0], t.t1[1], t.t1[2]);
+    const int t0b = pick3(b, t.t0[0], t.t0[1], t.t0[2]), t1b = pick3(b, t.t1[0], t.t1[1], t.t1[2]);
+    const int t0c = pick3(c, t.t0[0], t.t0[1], t.t0[2]), t1c = pick3(c, t.t1[0], t.t1[1], t.t1[2]);
+    const int near_plane = efield ? t0a : t0a - 1;
+    const bool nr = xa == near_plane, fr = xa == t1a;
+    if (!nr && !fr) return 0.f;
+    const bool c_ok = efield ? (xc >= t0c && xc < t1c) : (xc >= t0c && xc <= t1c);
+    const bool b_ok = efield ? (xb >= t0b && xb <= t1b) : (xb >= t0b && xb < t1b);
+    if (!c_ok || !b_ok) return 0.f;
+    const float ddc = (float)(xc - t0c) + (efield ? 0.5f : 0.f);
+    const float ddb = (float)(xb - t0b) + (efield ? 0.f : 0.5f);
+    const float span = (float)(t1a - t0a);
+    const float dda = efield ? (nr ? -0.5f : span + 0.5f) : (nr ? 0.f : span);
+    const float d0 = a == 0 ? dda : (b == 0 ? ddb : ddc);
+    const float d1 = a == 1 ? dda : (b == 1 ? ddb : ddc);
+    const float d2 = a == 2 ? dda : (b == 2 ? ddb : ddc);
+    const bool cyc = a == (c + 1) % 3;
+    float sg = (cyc != efield) ? 1.f : -1.f;
+    if (fr) sg = -sg;
+    const float pol = efield ? pick3f(b, t.hf[0], t.hf[1], t.hf[2]) : pick3f(b, t.ef[0], t.ef[1], t.ef[2]);
+    const float v = inc_interp(t, efield, d0, d1, d2) * pol;
+    const float term = sg * (efield ? t.cE : t.cH) * v * pick3f(a, t.inv_d[0], t.inv_d[1], t.inv_d[2]);
+    return term;
+}
+
 __device__ inline float tfsf_corr(const TfsfDev &t, bool efield, int c, int i, int j, int k) {
-    const int x[3] = {i, j, k};
     float r = 0.f;
 #pragma unroll 1
     for (int a = 0; a < 3; a++) {
         if (a == c) continue;
-        const int b = 3 - a - c;
-        const int near_plane = efield ? t.t0[a] : t.t0[a] - 1;
-        const bool nr = x[a] == near_plane, fr = x[a] == t.t1[a];
-        if (!nr && !fr) continue;
-        const bool c_ok = efield ? (x[c] >= t.t0[c] && x[c] < t.t1[c]) : (x[c] >= t.t0[c] && x[c] <= t.t1[c]);
-        const bool b_ok = efield ? (x[b] >= t.t0[b] && x[b] <= t.t1[b]) : (x[b] >= t.t0[b] && x[b] < t.t1[b]);
-        if (!c_ok || !b_ok) continue;
-        float dd[3];
-        dd[c] = (float)(x[c] - t.t0[c]) + (efield ? 0.5f : 0.f);
-        dd[b] = (float)(x[b] - t.t0[b]) + (efield ? 0.f : 0.5f);
-        const float span = (float)(t.t1[a] - t.t0[a]);
-        dd[a] = efield ? (nr ? -0.5f : span + 0.5f) : (nr ? 0.f : span);
-        const bool cyc = a == (c + 1) % 3;
-        float sg = (cyc != efield) ? 1.f : -1.f;
-        if (fr) sg = -sg;
-        const float v = inc_interp(t, efield, dd[0], dd[1], dd[2]) * (efield ? t.hf[b] : t.ef[b]);
-        r += sg * (efield ? t.cE : t.cH) * v * t.inv_d[a];
+        r += tfsf_term(t, efield, c, a, i, j, k);
     }
     return r;
+}
+
+// Corrections of all three components of (i,j,k) from the faces flagged per
+// normal axis (wave-uniform flags); per component the faces add in ascending
+// axis order, as in tfsf_corr.  One call site of tfsf_term keeps it cheap.
+__device__ inline void tfsf_faces(const TfsfDev &t, bool efield, bool fx, bool fy, bool fz, int i, int j, int k,
+                                  float &rx, float &ry, float &rz) {
+    rx = ry = rz = 0.f;
+#pragma unroll 1
+    for (int n = 0; n < 6; n++) {
+        const int a = n >> 1;
+        if (!(a == 0 ? fx : (a == 1 ? fy : fz))) continue;
+        const int c = (n & 1) ? (a + 2) % 3 : (a + 1) % 3;
+        const float v = tfsf_term(t, efield, c, a, i, j, k);
+        rx += c == 0 ? v : 0.f;
+        ry += c == 1 ? v : 0.f;
+        rz += c == 2 ? v : 0.f;
+    }
 }
 
 __device__ inline float tfsf_h(const TfsfDev &t, int c, int i, int j, int k) {
@@ -256,80 +291,283 @@ __device__ inline void dft_sample_cell(const FaceDev &f, float *row, int i, int 
 
 // ----------------------------------------------------------------- fused step kernel
 
-// Tile: 64 lanes along z (k = kt*63 - 1 + lane) by TJ wave rows along y
-// (j = jt*(TJ-1) - 1 + row); a block marches x over [i0, i1).  The first lane
-// and first row are halo: they recompute H there (from the read-only A
-// buffers) so E at the tile's low faces has its j-1 and k-1 neighbours, and
-// own nothing.  H planes are exchanged through LDS, double buffered, one
-// barrier per x step.
-template <int TJ>
-__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk) {
-    __shared__ float sH[2][3][TJ][65];
-    const Geom &g = s.g;
-    const int lane = threadIdx.x, row = threadIdx.y;
-    const int k = blockIdx.x * 63 - 1 + lane;
-    const int j = blockIdx.y * (TJ - 1) - 1 + row;
-    const int i0 = blockIdx.z * ichunk;
-    const int i1 = min(i0 + ichunk, g.NX);
-    const bool own = lane > 0 && row > 0 && k < g.NZ && j < g.NY;
-    // lanes past the grid read a clamped, in-bounds cell; their results are discarded
-    const int kc = min(k, g.NZ - 1), jc = min(j, g.NY - 1);
-    const long sz = g.sz, sxy = g.sxy;
-    const long cidx = (long)jc * sz + kc;   // in-plane offset
+// Lane l of a wave receives v from lane l-1 (DPP wave_shr:1); lane 0 gets 0.
+__device__ inline float from_lower_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
 
+// Tile: 64 lanes along z (k = kt*63 - 1 + lane) by TJ wave rows along y
+// (j = jt*(TJ-1) - 1 + row); a block marches x over [i0, i1).  Lane 0 and row
+// 0 are halo: they recompute H there (from the read-only input buffers) so E
+// at the tile's low faces has its k-1 and j-1 neighbours, and own nothing.
+// H at k-1 comes from the next-lower lane through DPP, H at j-1 from the row
+// below through LDS (double buffered, one barrier per x step), H at i-1 stays
+// in registers.
+//
+// Everything that depends only on (j,k) is hoisted out of the x march: the
+// component domains, 1/(kappa d) along y and z, the CPML layers and
+// recursion coefficients of the y and z slabs, psi offsets.  CPML and TFSF
+// work sits behind wave-uniform tests (a z-slab lane in the wave, a y-slab
+// row, an x-slab plane, a TFSF face), so a wave in the vacuum interior runs
+// the bare Yee update.  Field loads of plane i+1 are issued one x step ahead
+// of their use; psi_H of plane i+1 is fetched right after H of plane i is
+// done and psi_E of plane i at the top of its step.
+//
+// Blocks are renumbered so each XCD (blocks are dealt round-robin over 8)
+// receives a contiguous run of tiles: tiles adjacent in j, whose halo rows
+// overlap, then share that XCD's L2.
+template <int TJ>
+__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, int nkt, int njt, int nblk) {
+    __shared__ float sH[2][2][TJ][64];   // hz, hx of the plane for the j-1 neighbour
+    const Geom &g = s.g;
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+    int jt, kt, ch;
+    {
+        const int b = blockIdx.x, q = nblk >> 3, r = nblk & 7, x = b & 7, slot = b >> 3;
+        const int L = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+        jt = L % njt;
+        kt = (L / njt) % nkt;
+        ch = L / (njt * nkt);
+    }
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int k = kt * 63 - 1 + lane;
+    const int j = __builtin_amdgcn_readfirstlane(jt * (TJ - 1) - 1 + row);
+    const int i0 = ch * ichunk, i1 = min(i0 + ichunk, NX);
+    const bool own = lane > 0 && row > 0 && k < NZ && j < NY;
+    const int kc = min(k, NZ - 1), jc = min(j, NY - 1);
+    const int kk = k < NZ ? k : NZ, jj = j < NY ? j : NY;   // beyond the grid: outside every domain
+    const int sz = g.sz;
+    const long sxy = g.sxy;
+    const int cidx = jc * sz + kc;
+
+    // (j,k) parts of the component domains (reference loop bounds, main.cu:829-1014, 1172-1767)
+    const bool jk_in = jj >= 0 && kk >= 0;
+    const bool dhx = jk_in && jj < NY - 1 && kk < NZ - 1;   // also the psi (j,k) range
+    const bool dhy = jk_in && jj < NY && kk < NZ - 1;
+    const bool dhz = jk_in && jj < NY - 1 && kk < NZ;
+    const bool dex = dhx && jj > 0 && kk > 0;
+    const bool dey = dhx && kk > 0;
+    const bool dez = dhx && jj > 0;
+
+    const CpmlDev &cx = s.cp[0], &cy = s.cp[1], &cz = s.cp[2];
+    const int kq = max(kc, 0), jq = max(jc, 0);
+    const float ikz_h = cz.ik_h[kq], ikz_e = cz.ik_e[kq];
+    const float iky_h = cy.ik_h[jq], iky_e = cy.ik_e[jq];
+    // z slabs: per lane
+    const int lzh = dhx ? h_layer(cz, kk) : -1, lze = dhx ? e_layer(cz, kk) : -1;
+    const bool zh = __ballot(lzh >= 0) != 0, ze = __ballot(lze >= 0) != 0;
+    const int Lzh = 2 * cz.Nc, Lze = 2 * (cz.Nc + 1);
+    const float bzh = lzh >= 0 ? cz.bh[lzh] : 0.f, czh = lzh >= 0 ? cz.ch[lzh] : 0.f;
+    const float bze = lze >= 0 ? cz.be[lze] : 0.f, cze = lze >= 0 ? cz.ce[lze] : 0.f;
+    const int ozh = jq * Lzh + max(lzh, 0), oze = jq * Lze + max(lze, 0);
+    const long szh = (long)NY * Lzh, sze = (long)NY * Lze;
+    // y slabs: per row (wave-uniform)
+    const int lyh = (j >= 0 && j < NY - 1) ? h_layer(cy, j) : -1, lye = (j >= 0 && j < NY - 1) ? e_layer(cy, j) : -1;
+    const int Lyh = 2 * cy.Nc, Lye = 2 * (cy.Nc + 1);
+    const float byh = lyh >= 0 ? cy.bh[lyh] : 0.f, cyh = lyh >= 0 ? cy.ch[lyh] : 0.f;
+    const float bye = lye >= 0 ? cy.be[lye] : 0.f, cye = lye >= 0 ? cy.ce[lye] : 0.f;
+    const int oyh = max(lyh, 0) * NZ + kq, oye = max(lye, 0) * NZ + kq;
+    const long syh = (long)Lyh * NZ, sye = (long)Lye * NZ;
+    // x slabs: offset within a layer
+    const int ox = jq * NZ + kq;
+    const long sxl = (long)NY * NZ;
+    // TFSF faces touching this wave (H faces sit at t0-1 and t1, E faces at t0 and t1)
+    const TfsfDev &tf = s.tf;
+    const bool tf_on = s.tfsf_on != 0;
+    const bool tyh = tf_on && (j == tf.t0[1] - 1 || j == tf.t1[1]), tye = tf_on && (j == tf.t0[1] || j == tf.t1[1]);
+    const bool tzh = tf_on && __ballot(k == tf.t0[2] - 1 || k == tf.t1[2]) != 0;
+    const bool tze = tf_on && __ballot(k == tf.t0[2] || k == tf.t1[2]) != 0;
+
+    const float Chh = s.c.Chh, Che = s.c.Che, Cee = s.c.Cee, Ceh = s.c.Ceh;
     const float *Ex = s.Ein[0], *Ey = s.Ein[1], *Ez = s.Ein[2];
     const float *Hx = s.Hin[0], *Hy = s.Hin[1], *Hz = s.Hin[2];
+    const PsiSet &pHi = s.psiH_in, &pHo = s.psiH_out, &pE = s.psiE;
 
-    // prologue: H^{n+1/2} at plane i0-1 (values only) and E^n at plane i0
+    // H^{n+1/2} at plane i0-1 (values only)
     float hyP = 0.f, hzP = 0.f;
-    float exc, eyc, ezc;
-    {
-        long p = (long)i0 * sxy + cidx;
-        exc = Ex[p]; eyc = Ey[p]; ezc = Ez[p];
-    }
     if (i0 > 0) {
-        const int ip = i0 - 1;
-        const long p = (long)ip * sxy + cidx;
+        const long p = (long)(i0 - 1) * sxy + cidx;
         ENbr e;
         e.ex = Ex[p]; e.ey = Ey[p]; e.ez = Ez[p];
         e.ex_j1 = Ex[p + sz]; e.ez_j1 = Ez[p + sz];
         e.ex_k1 = Ex[p + 1]; e.ey_k1 = Ey[p + 1];
-        e.ey_i1 = eyc; e.ez_i1 = ezc;
+        e.ey_i1 = Ey[p + sxy]; e.ez_i1 = Ez[p + sxy];
         float hx, hy, hz;
-        h_cell(s, ip, jc == j ? j : g.NY, kc == k ? k : g.NZ, e, Hx[p], Hy[p], Hz[p], false, hx, hy, hz);
+        h_cell(s, i0 - 1, jj, kk, e, Hx[p], Hy[p], Hz[p], false, hx, hy, hz);
         hyP = hy; hzP = hz;
     }
+    // pipeline fill: E^n at i0 and i0+1, E^n neighbours and H at i0, psi_H at i0
+    float exo, eyo, ezo, e1x, e1y, e1z, nxj, nzj, nxk, nyk, hxo, hyo, hzo;
+    {
+        const long p = (long)i0 * sxy + cidx;
+        exo = Ex[p]; eyo = Ey[p]; ezo = Ez[p];
+        e1x = Ex[p + sxy]; e1y = Ey[p + sxy]; e1z = Ez[p + sxy];
+        nxj = Ex[p + sz]; nzj = Ez[p + sz];
+        nxk = Ex[p + 1]; nyk = Ey[p + 1];
+        hxo = Hx[p]; hyo = Hy[p]; hzo = Hz[p];
+    }
+    float qhxz = 0.f, qhyz = 0.f, qhxy = 0.f, qhzy = 0.f, qhyx = 0.f, qhzx = 0.f;   // psi_H of the current plane
+    auto fetch_psiH = [&](int ip) {
+        if (ip >= NX - 1) return;
+        if (zh && lzh >= 0) {
+            const long q = ip * szh + ozh;
+            qhxz = pHi.p[0][q]; qhyz = pHi.p[2][q];
+        }
+        if (lyh >= 0 && dhx) {
+            const long q = ip * syh + oyh;
+            qhxy = pHi.p[1][q]; qhzy = pHi.p[5][q];
+        }
+        const int lx = h_layer(cx, ip);
+        if (lx >= 0 && dhx) {
+            const long q = lx * sxl + ox;
+            qhyx = pHi.p[3][q]; qhzx = pHi.p[4][q];
+        }
+    };
+    fetch_psiH(i0);
+
     int buf = 0;
     for (int i = i0; i < i1; i++) {
         const long p = (long)i * sxy + cidx;
-        ENbr e;
-        e.ex = exc; e.ey = eyc; e.ez = ezc;
-        const float exn = Ex[p + sxy], eyn = Ey[p + sxy], ezn = Ez[p + sxy];
-        e.ey_i1 = eyn; e.ez_i1 = ezn;
-        e.ex_j1 = Ex[p + sz]; e.ez_j1 = Ez[p + sz];
-        e.ex_k1 = Ex[p + 1]; e.ey_k1 = Ey[p + 1];
+        const bool more = i + 1 < i1;
+        // fields of the next step
+        float f_e1x = 0.f, f_e1y = 0.f, f_e1z = 0.f, f_nxj = 0.f, f_nzj = 0.f, f_nxk = 0.f, f_nyk = 0.f;
+        float f_hx = 0.f, f_hy = 0.f, f_hz = 0.f;
+        if (more) {
+            const long p1 = p + sxy, p2 = min(i + 2, NX) * sxy + cidx;
+            f_e1x = Ex[p2]; f_e1y = Ey[p2]; f_e1z = Ez[p2];
+            f_nxj = Ex[p1 + sz]; f_nzj = Ez[p1 + sz];
+            f_nxk = Ex[p1 + 1]; f_nyk = Ey[p1 + 1];
+            f_hx = Hx[p1]; f_hy = Hy[p1]; f_hz = Hz[p1];
+        }
+        const bool pin = i < NX - 1;
+        // psi_E of this plane
+        const bool xin = i > 0 && pin;   // ey, ez need 0 < i < NX-1
+        const int lxe = pin ? e_layer(cx, i) : -1;
+        float qexz = 0.f, qeyz = 0.f, qexy = 0.f, qezy = 0.f, qeyx = 0.f, qezx = 0.f;
+        if (pin && ze && lze >= 0) {
+            const long q = i * sze + oze;
+            if (dex) qexz = pE.p[0][q];
+            if (dey && xin) qeyz = pE.p[2][q];
+        }
+        if (pin && lye >= 0) {
+            const long q = i * sye + oye;
+            if (dex) qexy = pE.p[1][q];
+            if (dez && xin) qezy = pE.p[4][q];
+        }
+        if (lxe >= 0 && xin) {
+            const long q = lxe * sxl + ox;
+            if (dey) qeyx = pE.p[3][q];
+            if (dez) qezx = pE.p[5][q];
+        }
+
+        // ---- H^{n+1/2} at plane i
+        const float ikx_h = cx.ik_h[i];
+        const int lxh = pin ? h_layer(cx, i) : -1;
         float hx, hy, hz;
-        h_cell(s, i, jc == j ? j : g.NY, kc == k ? k : g.NZ, e, Hx[p], Hy[p], Hz[p], own, hx, hy, hz);
+        {
+            const float dEy_z = nyk - eyo, dEz_y = nzj - ezo;   // hx
+            const float dEz_x = e1z - ezo, dEx_z = nxk - exo;   // hy
+            const float dEx_y = nxj - exo, dEy_x = e1y - eyo;   // hz
+            hx = Chh * hxo + Che * (dEy_z * ikz_h - dEz_y * iky_h);
+            hy = Chh * hyo + Che * (dEz_x * ikx_h - dEx_z * ikz_h);
+            hz = Chh * hzo + Che * (dEx_y * iky_h - dEy_x * ikx_h);
+            if (pin && zh && lzh >= 0) {
+                const long q = i * szh + ozh;
+                const float a = bzh * qhxz + czh * dEy_z;
+                const float b = bzh * qhyz + czh * dEx_z;
+                hx += Che * a;
+                hy -= Che * b;
+                if (own) { pHo.p[0][q] = a; pHo.p[2][q] = b; }
+            }
+            if (pin && lyh >= 0 && dhx) {
+                const long q = i * syh + oyh;
+                const float a = byh * qhxy + cyh * dEz_y;
+                hx -= Che * a;
+                if (own) pHo.p[1][q] = a;
+            }
+            if (lxh >= 0 && dhx) {
+                const long q = lxh * sxl + ox;
+                const float bx = cx.bh[lxh], cxv = cx.ch[lxh];
+                const float a = bx * qhyx + cxv * dEz_x;
+                const float b = bx * qhzx + cxv * dEy_x;
+                hy += Che * a;
+                hz -= Che * b;
+                if (own) { pHo.p[3][q] = a; pHo.p[4][q] = b; }
+            }
+            if (pin && lyh >= 0 && dhx) {
+                const long q = i * syh + oyh;
+                const float a = byh * qhzy + cyh * dEx_y;
+                hz += Che * a;
+                if (own) pHo.p[5][q] = a;
+            }
+            const bool txh = tf_on && (i == tf.t0[0] - 1 || i == tf.t1[0]);
+            if (txh || tyh || tzh) {
+                float rx, ry, rz;
+                tfsf_faces(tf, false, txh, tyh, tzh, i, jj, kk, rx, ry, rz);
+                hx += rx; hy += ry; hz += rz;
+            }
+            hx = dhx ? hx : hxo;
+            hy = (dhy && pin) ? hy : hyo;
+            hz = (dhz && pin) ? hz : hzo;
+        }
+        if (more) fetch_psiH(i + 1);
         if (own) { s.Hout[0][p] = hx; s.Hout[1][p] = hy; s.Hout[2][p] = hz; }
-        sH[buf][0][row][lane] = hx;
-        sH[buf][1][row][lane] = hy;
-        sH[buf][2][row][lane] = hz;
+        sH[buf][0][row][lane] = hz;
+        sH[buf][1][row][lane] = hx;
+        const float hy_km = from_lower_lane(hy), hx_km = from_lower_lane(hx);
         __syncthreads();
-        HNbr h;
-        h.hx = hx; h.hy = hy; h.hz = hz;
-        const int rm = row > 0 ? row - 1 : 0, lm = lane > 0 ? lane - 1 : 0;
-        h.hz_jm = sH[buf][2][rm][lane];
-        h.hx_jm = sH[buf][0][rm][lane];
-        h.hy_km = sH[buf][1][row][lm];
-        h.hx_km = sH[buf][0][row][lm];
-        h.hz_im = hzP; h.hy_im = hyP;
+        const int rm = row > 0 ? row - 1 : 0;
+        const float hz_jm = sH[buf][0][rm][lane], hx_jm = sH[buf][1][rm][lane];
+
+        // ---- E^{n+1} at plane i
         if (own) {
-            float ex, ey, ez;
-            e_cell(s, i, j, k, h, exc, eyc, ezc, ex, ey, ez);
-            s.Eout[0][p] = ex; s.Eout[1][p] = ey; s.Eout[2][p] = ez;
+            const float ikx_e = cx.ik_e[i];
+            float eox = exo, eoy = eyo, eoz = ezo;
+            const bool txe = tf_on && (i == tf.t0[0] || i == tf.t1[0]);
+            if (txe || tye || tze) {
+                float rx, ry, rz;
+                tfsf_faces(tf, true, txe, tye, tze, i, j, k, rx, ry, rz);
+                eox += rx; eoy += ry; eoz += rz;
+            }
+            const float dHz_y = hz - hz_jm, dHy_z = hy - hy_km;   // ex
+            const float dHx_z = hx - hx_km, dHz_x = hz - hzP;     // ey
+            const float dHy_x = hy - hyP, dHx_y = hx - hx_jm;     // ez
+            float ex = Cee * eox + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
+            float ey = Cee * eoy + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
+            float ez = Cee * eoz + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
+            const bool ux = dex && pin, uy = dey && xin, uz = dez && xin;
+            if (pin && ze && lze >= 0) {
+                const long q = i * sze + oze;
+                if (ux) { const float a = bze * qexz + cze * dHy_z; ex -= Ceh * a; pE.p[0][q] = a; }
+                if (uy) { const float a = bze * qeyz + cze * dHx_z; ey += Ceh * a; pE.p[2][q] = a; }
+            }
+            if (pin && lye >= 0) {
+                const long q = i * sye + oye;
+                if (ux) { const float a = bye * qexy + cye * dHz_y; ex += Ceh * a; pE.p[1][q] = a; }
+            }
+            if (lxe >= 0 && xin) {
+                const long q = lxe * sxl + ox;
+                const float bx = cx.be[lxe], cxv = cx.ce[lxe];
+                if (uy) { const float a = bx * qeyx + cxv * dHz_x; ey -= Ceh * a; pE.p[3][q] = a; }
+            }
+            if (pin && lye >= 0) {
+                const long q = i * sye + oye;
+                if (uz) { const float a = bye * qezy + cye * dHx_y; ez -= Ceh * a; pE.p[4][q] = a; }
+            }
+            if (lxe >= 0 && xin) {
+                const long q = lxe * sxl + ox;
+                const float bx = cx.be[lxe], cxv = cx.ce[lxe];
+                if (uz) { const float a = bx * qezx + cxv * dHy_x; ez += Ceh * a; pE.p[5][q] = a; }
+            }
+            s.Eout[0][p] = ux ? ex : exo;
+            s.Eout[1][p] = uy ? ey : eyo;
+            s.Eout[2][p] = uz ? ez : ezo;
         }
         hyP = hy; hzP = hz;
-        exc = exn; eyc = eyn; ezc = ezn;
+        exo = e1x; eyo = e1y; ezo = e1z;
+        e1x = f_e1x; e1y = f_e1y; e1z = f_e1z;
+        nxj = f_nxj; nzj = f_nzj; nxk = f_nxk; nyk = f_nyk;
+        hxo = f_hx; hyo = f_hy; hzo = f_hz;
         buf ^= 1;
     }
 }
@@ -1116,25 +1354,15 @@ __global__ void k_probe(const float *ex, const float *ey, const float *ez, long 
 
 hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, hipStream_t st) {
     const int nkt = (s.g.NZ + 62) / 63;
+    const int njt = (s.g.NY + tj - 2) / (tj - 1);
     const int nchunk = (s.g.NX + ichunk - 1) / ichunk;
+    const long nblk = (long)nkt * njt * nchunk;
+    if (nblk <= 0 || nblk > 0x7fffffff) return hipErrorInvalidValue;
     switch (tj) {
-        case 4: {
-            dim3 grid(nkt, (s.g.NY + 2) / 3, nchunk), block(64, 4);
-            hipLaunchKernelGGL(k_fused_step<4>, grid, block, 0, st, s, ichunk);
-            break;
-        }
-        case 8: {
-            dim3 grid(nkt, (s.g.NY + 6) / 7, nchunk), block(64, 8);
-            hipLaunchKernelGGL(k_fused_step<8>, grid, block, 0, st, s, ichunk);
-            break;
-        }
-        case 16: {
-            dim3 grid(nkt, (s.g.NY + 14) / 15, nchunk), block(64, 16);
-            hipLaunchKernelGGL(k_fused_step<16>, grid, block, 0, st, s, ichunk);
-            break;
-        }
-        default:
-            return hipErrorInvalidValue;
+        case 4: hipLaunchKernelGGL(k_fused_step<4>, dim3((unsigned)nblk), dim3(64, 4), 0, st, s, ichunk, nkt, njt, (int)nblk); break;
+        case 8: hipLaunchKernelGGL(k_fused_step<8>, dim3((unsigned)nblk), dim3(64, 8), 0, st, s, ichunk, nkt, njt, (int)nblk); break;
+        case 16: hipLaunchKernelGGL(k_fused_step<16>, dim3((unsigned)nblk), dim3(64, 16), 0, st, s, ichunk, nkt, njt, (int)nblk); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -71,20 +71,27 @@ def test_line_matches_oracle():
 
 @pytest.mark.parametrize("fused", [False, True])
 def test_dft_and_flux(fused):
+    """Running DFTs of the tangential fields on the 12 monitor faces and the
+    flux spectra.  A DFT error is normalised by the largest E (or H) DFT value
+    over all faces, as field errors are; spectra by their own maximum."""
     p = small_params(O, steps=450, block=8)
     o, s = run_pair(p, 449, fused)   # 113 samples: ends mid-block, exercises the partial contraction
-    worst = 0.0
-    for face in range(12):
-        for comp in range(4):
-            a, b = s.get_dft(face, comp), o.get_dft(face, comp)
-            worst = max(worst, rel(a, b))
-    print(f"dft worst rel={worst:.3e}", flush=True)
-    assert worst < 1e-3
+    da = {(f, c): s.get_dft(f, c) for f in range(12) for c in range(4)}
+    db = {(f, c): o.get_dft(f, c) for f in range(12) for c in range(4)}
+    me = max(np.abs(db[f, c]).max() for f in range(12) for c in (0, 1))
+    mh = max(np.abs(db[f, c]).max() for f in range(12) for c in (2, 3))
+    worst, own = 0.0, 0.0
+    for (f, c), a in da.items():
+        err = float(np.abs(a - db[f, c]).max())
+        worst = max(worst, err / (me if c < 2 else mh))
+        own = max(own, rel(a, db[f, c]))
+    print(f"dft worst rel={worst:.3e} (per face-component normalisation {own:.3e})", flush=True)
+    assert worst < 5e-5
     fo, fs = o.flux(), s.flux()
     for k in ("raw", "sca", "abs", "ext"):
         err = rel(fs[k], fo[k])
         print(f"flux {k} rel={err:.3e}", flush=True)
-        assert err < 1e-3, (k, err)
+        assert err < 1e-4, (k, err)
 
 
 def test_fused_equals_split():
