@@ -82,6 +82,17 @@ struct fdtd_sim {
     size_t hsz[6] = {}, esz[6] = {};
     int t = 0;
     int tj = 8, ichunk = 32;
+    // fused step: row-blocked kernel over the whole grid + TFSF fix-up, or the general kernel alone
+    bool use_rows = false;
+    bool has_clean = false;         // clean box non-empty: clean variant there, general variant on the rest
+    int cbox[4] = {};               // clean box {ilo, ihi, jlo, jhi}
+    int gbox[4][4] = {};            // general pieces
+    int ngbox = 0;
+    float ik0[2][2] = {};           // interior 1/(kappa d): [E, H][x, y]
+    unsigned *aux_d = nullptr;      // table image of the row-blocked kernel
+    std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
+    int rows = 2, cchunk = 8;       // own rows per thread and x chunk of the row-blocked kernel
+    int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     size_t ev_used = 0;
@@ -382,6 +393,140 @@ void fdtd_default_params(fdtd_params *p) { fdtd::default_params(p); }
 
 const char *fdtd_last_error(const fdtd_sim *sim) { return sim ? sim->err.c_str() : g_create_err.c_str(); }
 
+// The row-blocked kernel (k_step_rows) runs the whole grid with TFSF left
+// out; k_tfsf_fix then adds it, which needs the TFSF cells and their
+// neighbours free of CPML and with 1/(kappa d) at its interior value.  The
+// kernel's table image (layout in kernels.hip) needs 1/(kappa d) constant
+// outside 64 cells from either end of an axis and at most 31 CPML cells.
+// When any of this fails, or the fields exceed 32-bit byte offsets, the
+// general kernel (k_fused_step, TFSF inside) runs alone.
+int plan_rows(fdtd_sim *s) {
+    const Setup &S = s->S;
+    s->use_rows = false;
+    if (const char *v = std::getenv("FDTD_PATH"))
+        if (std::string(v) == "general") return FDTD_OK;
+    if (const char *v = std::getenv("FDTD_ROWS")) s->rows = std::min(3, std::max(2, std::atoi(v)));
+    if (const char *v = std::getenv("FDTD_CCHUNK")) s->cchunk = std::max(1, std::atoi(v));
+    if ((double)s->fsz * 4.0 >= 4294967296.0) return FDTD_OK;
+    std::vector<uint32_t> img(kRowsAuxWords, 0u);
+    auto put = [&](int q, float v) { std::memcpy(&img[q], &v, 4); };
+    for (int a = 0; a < 3; a++) {
+        const fdtd::CpmlAxis &c = S.cp[a];
+        const int N = S.N[a], mid = N / 2;
+        const std::vector<float> &ike = s->t_ike[a], &ikh = s->t_ikh[a];
+        const int ew = std::max(c.Nc + 2, N - c.F + 1);
+        if (ew > kRowsAuxMaxEdge || 2 * (c.Nc + 1) > kRowsAuxMaxLayers) return FDTD_OK;
+        for (int x = ew; x < N - ew; x++)
+            if (ike[x] != ike[mid] || ikh[x] != ikh[mid]) return FDTD_OK;
+        auto plain = [&](int x) {
+            const bool hl = x < c.Nc || x >= c.F, el = x <= c.Nc || x >= c.F;
+            return x > 0 && x < N - 1 && !hl && !el && ikh[x] == ikh[mid] && ike[x] == ike[mid];
+        };
+        for (int x = S.tf0[a] - 2; x <= S.tf1[a] + 1; x++)
+            if (!plain(x)) return FDTD_OK;
+        const int base = a * kRowsAuxAxis;
+        for (int x = 0; x < ew; x++) {
+            const int xn = std::min(x, N - 1), xf = std::max(0, N - ew + x);
+            put(base + x, ike[xn]); put(base + 64 + x, ike[xf]);
+            put(base + 128 + x, ikh[xn]); put(base + 192 + x, ikh[xf]);
+        }
+        for (size_t q = 0; q < s->t_be[a].size(); q++) { put(base + 256 + (int)q, s->t_be[a][q]); put(base + 320 + (int)q, s->t_ce[a][q]); }
+        for (size_t q = 0; q < s->t_bh[a].size(); q++) { put(base + 384 + (int)q, s->t_bh[a][q]); put(base + 448 + (int)q, s->t_ch[a][q]); }
+        put(1536 + a, ike[mid]);
+        put(1539 + a, ikh[mid]);
+        img[1542 + a] = (uint32_t)ew;
+    }
+    // clean box: planes and rows with no CPML, interior 1/(kappa d) and every
+    // domain test true, whose lower neighbour (the halo) is H-clean
+    int lo[2], hi[2];
+    bool clean = true;
+    for (int a = 0; a < 2 && clean; a++) {
+        const fdtd::CpmlAxis &c = S.cp[a];
+        const int N = S.N[a], mid = N / 2;
+        const std::vector<float> &ike = s->t_ike[a], &ikh = s->t_ikh[a];
+        auto hl = [&](int x) { return x < c.Nc || x >= c.F; };
+        auto el = [&](int x) { return x <= c.Nc || x >= c.F; };
+        auto hclean = [&](int x) { return x >= 0 && x < N - 1 && !hl(x) && ikh[x] == ikh[mid]; };
+        auto eclean = [&](int x) { return x > 0 && x < N - 1 && !el(x) && ike[x] == ike[mid]; };
+        if (!hclean(mid) || !eclean(mid)) { clean = false; break; }
+        int l = mid, h = mid + 1;
+        while (l - 1 >= 0 && hclean(l - 1) && eclean(l - 1)) l--;
+        while (h < N && hclean(h) && eclean(h)) h++;
+        if (!hclean(l - 1)) l++;
+        if (l >= h) clean = false;
+        lo[a] = l; hi[a] = h;
+        s->ik0[0][a] = ike[mid];
+        s->ik0[1][a] = ikh[mid];
+    }
+    const int NX = S.N[0], NY = S.N[1];
+    s->ngbox = 0;
+    auto addg = [&](int i0, int i1, int j0, int j1) {
+        if (i0 < i1 && j0 < j1) {
+            int *b = s->gbox[s->ngbox++];
+            b[0] = i0; b[1] = i1; b[2] = j0; b[3] = j1;
+        }
+    };
+    s->has_clean = clean;
+    if (clean) {
+        s->cbox[0] = lo[0]; s->cbox[1] = hi[0]; s->cbox[2] = lo[1]; s->cbox[3] = hi[1];
+        addg(0, lo[0], 0, NY);
+        addg(hi[0], NX, 0, NY);
+        addg(lo[0], hi[0], 0, lo[1]);
+        addg(lo[0], hi[0], hi[1], NY);
+    } else {
+        addg(0, NX, 0, NY);
+    }
+    float *ptrs[18];
+    for (int q = 0; q < 2; q++)
+        for (int x = 0; x < 6; x++) ptrs[q * 6 + x] = s->psiH[q][x];
+    for (int x = 0; x < 6; x++) ptrs[12 + x] = s->psiE[x];
+    std::memcpy(&img[1548], ptrs, sizeof(ptrs));
+    int rc = upload(s, &s->aux_d, img);
+    if (rc) return rc;
+    s->use_rows = true;
+    return FDTD_OK;
+}
+
+RowsArgs make_rows(fdtd_sim *s) {
+    const Setup &S = s->S;
+    RowsArgs r{};
+    const int q = s->p;
+    const long fb = s->fbase;
+    for (int c = 0; c < 3; c++) {
+        r.Ein[c] = (const char *)(s->E[q][c] - fb); r.Hin[c] = (const char *)(s->H[q][c] - fb);
+        r.Eout[c] = (char *)(s->E[1 - q][c] - fb); r.Hout[c] = (char *)(s->H[1 - q][c] - fb);
+    }
+    r.aux = s->aux_d;
+    r.Chh = (float)S.Chh; r.Che = (float)S.Che; r.Cee = (float)S.Cee; r.Ceh = (float)S.Ceh;
+    r.NX = s->g.NX; r.NY = s->g.NY; r.NZ = s->g.NZ; r.sz = s->g.sz;
+    r.sxy4 = (unsigned)(s->g.sxy * 4);
+    for (int a = 0; a < 3; a++) { r.Nc[a] = S.cp[a].Nc; r.F[a] = S.cp[a].F; }
+    r.pp = s->pp;
+    r.ichunk = s->cchunk;
+    for (int e = 0; e < 2; e++)
+        for (int a = 0; a < 2; a++) r.ik0[e][a] = s->ik0[e][a];
+    return r;
+}
+
+// The row-blocked field update: the clean variant on the clean box, the
+// general variant on the pieces around it (independent cell sets).
+int launch_rows_update(fdtd_sim *s, hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
+    RowsArgs r = make_rows(s);
+    if (s->ngbox) {
+        r.npiece = s->ngbox;
+        std::memcpy(r.box, s->gbox, sizeof(r.box));
+        HIPCHK(s, launch_step_rows(r, s->rows, true, s->stream));
+    }
+    if (s->has_clean) {
+        r.npiece = 1;
+        std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
+        if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
+        HIPCHK(s, launch_step_rows(r, s->rows, false, s->stream));
+        if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));
+    }
+    return FDTD_OK;
+}
+
 int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
     if (!p || !out) return set_err(nullptr, FDTD_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -449,7 +594,9 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
         if ((rc = upload(s, &s->ik_e[a], ike)) || (rc = upload(s, &s->ik_h[a], ikh)) || (rc = upload(s, &s->be[a], be)) ||
             (rc = upload(s, &s->ce[a], ce)) || (rc = upload(s, &s->bh[a], bh)) || (rc = upload(s, &s->ch[a], ch)))
             return fail(rc);
+        s->t_ike[a] = ike; s->t_ikh[a] = ikh; s->t_be[a] = be; s->t_ce[a] = ce; s->t_bh[a] = bh; s->t_ch[a] = ch;
     }
+    if ((rc = plan_rows(s))) return fail(rc);
     // 1D line
     if ((rc = dalloc(s, &s->line_e[0], S.L)) || (rc = dalloc(s, &s->line_e[1], S.L)) || (rc = dalloc(s, &s->line_h, S.L)) ||
         (rc = dalloc(s, &s->mur, 2)))
@@ -508,6 +655,8 @@ int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
     g->t = s->t;
     g->hydro = S.hydro;
     g->sample_count = s->samples;
+    g->fused_path = s->use_rows ? 1 : 0;
+    for (int a = 0; a < 4; a++) g->clean[a] = s->use_rows && s->has_clean ? s->cbox[a] : -1;
     return FDTD_OK;
 }
 
@@ -885,16 +1034,27 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         }
         for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
         const bool samp = s->sampling && sampling_step(s);
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (s->timing) {
-            hipEvent_t a = next_event(s, 0), b = next_event(s, 1);
-            if (!a || !b) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
-            HIPCHK(s, hipEventRecord(a, s->stream));
-            HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, s->stream));
-            HIPCHK(s, hipEventRecord(b, s->stream));
-            s->ev_used++;
-        } else {
-            HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, s->stream));
+            ev0 = next_event(s, 0);
+            ev1 = next_event(s, 1);
+            if (!ev0 || !ev1) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
         }
+        const bool only_clean = s->timing_what == 2 && s->use_rows && s->has_clean;
+        if (s->use_rows) {
+            d.tfsf_on = 0;
+            if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev0, s->stream));
+            int rc2 = launch_rows_update(s, only_clean ? ev0 : nullptr, only_clean ? ev1 : nullptr);
+            if (rc2) return rc2;
+            HIPCHK(s, launch_tfsf_fix(d, s->stream));
+            if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev1, s->stream));
+        } else {
+            const int full[1][4] = {{0, s->g.NX, 0, s->g.NY}};
+            if (s->timing) HIPCHK(s, hipEventRecord(ev0, s->stream));
+            HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, full, 1, s->stream));
+            if (s->timing) HIPCHK(s, hipEventRecord(ev1, s->stream));
+        }
+        if (s->timing) s->ev_used++;
         if (s->nbox) {
             DispDev dd = make_disp(s, 1 - q);
             HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
@@ -1031,7 +1191,9 @@ int fdtd_kernel_timing(fdtd_sim *s, int enable, double *total_ms, long *launches
     if (total_ms) *total_ms = tot;
     if (launches) *launches = (long)s->ev_used;
     s->ev_used = 0;
+    if (enable < 0 || enable > 2) return set_err(s, FDTD_ERR_INVALID, "kernel_timing: enable must be 0, 1 or 2");
     s->timing = enable != 0;
+    if (enable) s->timing_what = enable;
     return FDTD_OK;
 }
 

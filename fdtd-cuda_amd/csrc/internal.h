@@ -165,10 +165,55 @@ struct LineDev {
 // ---------------------------------------------------------------------------
 // Launch wrappers (kernels.hip).  All return hipError_t of the launch.
 // ---------------------------------------------------------------------------
-hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, hipStream_t st);
+// Sub-boxes of owned cells for the general fused kernel: x in [ia, ib), y in
+// [ja, jb), all z; blocks of piece q are [start[q], start[q+1]).
+struct Pieces {
+    int n;
+    int ia[4], ib[4], ja[4], jb[4];
+    int njt[4];
+    int start[5];
+};
+
+// Arguments of the row-blocked fused step (k_step_rows).  Field pointers are
+// allocation bases (element (-1,-1,-1)); aux is the table image (CPML tables,
+// interior 1/(kappa d), psi pointers; layout in kernels.hip, built by the
+// host).  The kernel runs the cells x in [box[q][0], box[q][1]) by y in
+// [box[q][2], box[q][3]), all z, of npiece boxes; the clean variant
+// (general = false) requires boxes with no x or y CPML, constant 1/(kappa d)
+// ik0[e][axis] (e: 0 E, 1 H) along x and y, every domain test true, and
+// H-clean planes and rows just below them.  nseg and start are filled in by
+// the launcher.
+struct RowsArgs {
+    const char *Ein[3], *Hin[3];
+    char *Eout[3], *Hout[3];
+    const unsigned *aux;
+    float Chh, Che, Cee, Ceh;
+    float ik0[2][2];
+    int NX, NY, NZ, sz;
+    unsigned sxy4;                      // plane pitch in bytes
+    int Nc[3], F[3];
+    int pp;                             // psi_H input set
+    int ichunk;
+    int npiece;
+    int box[4][4];
+    int nseg;
+    int start[5];
+};
+// Words of the table image and its layout (see kernels.hip).
+constexpr int kRowsAuxWords = 1548 + 36;
+constexpr int kRowsAuxAxis = 512, kRowsAuxMaxEdge = 64, kRowsAuxMaxLayers = 64;
+// Waves per block of the row-blocked kernel for a given NZ (one z segment when NZ < 64*16 - 2).
+int rows_waves(int NZ);
+// R: own rows per thread (2 or 3); TFSF is left to launch_tfsf_fix.
+hipError_t launch_step_rows(RowsArgs ra, int R, bool general, hipStream_t st);
+
+// box[q] = {ia, ib, ja, jb}; one to four disjoint boxes.
+hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st);
 hipError_t launch_split_h(const StepDev &s, hipStream_t st);
 hipError_t launch_split_e(const StepDev &s, hipStream_t st);
 hipError_t launch_tfsf_split(const StepDev &s, hipStream_t st);
+// TFSF terms added after a fused step run with tfsf_on = 0 (H faces, E faces, curl of dH).
+hipError_t launch_tfsf_fix(const StepDev &s, hipStream_t st);
 hipError_t launch_line(const LineDev &l, hipStream_t st);
 hipError_t launch_dispersive(const DispDev &d, hipStream_t st);
 hipError_t launch_dft_sample_split(const StepDev &s, hipStream_t st);

@@ -291,6 +291,13 @@ __device__ inline void dft_sample_cell(const FaceDev &f, float *row, int i, int 
 
 // ----------------------------------------------------------------- fused step kernel
 
+// Blocks are dealt round-robin over the 8 XCDs; this renumbering gives each
+// XCD a contiguous run of logical blocks, so neighbouring tiles share an L2.
+__device__ inline int xcd_order(int b, int nblk) {
+    const int q = nblk >> 3, r = nblk & 7, x = b & 7, slot = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+}
+
 // Lane l of a wave receives v from lane l-1 (DPP wave_shr:1); lane 0 gets 0.
 __device__ inline float from_lower_lane(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
@@ -317,23 +324,26 @@ __device__ inline float from_lower_lane(float v) {
 // receives a contiguous run of tiles: tiles adjacent in j, whose halo rows
 // overlap, then share that XCD's L2.
 template <int TJ>
-__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, int nkt, int njt, int nblk) {
+__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, int nkt, Pieces pc) {
     __shared__ float sH[2][2][TJ][64];   // hz, hx of the plane for the j-1 neighbour
     const Geom &g = s.g;
     const int NX = g.NX, NY = g.NY, NZ = g.NZ;
-    int jt, kt, ch;
+    int jt, kt, ch, ia, ib, ja, jb;
     {
-        const int b = blockIdx.x, q = nblk >> 3, r = nblk & 7, x = b & 7, slot = b >> 3;
-        const int L = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
-        jt = L % njt;
-        kt = (L / njt) % nkt;
-        ch = L / (njt * nkt);
+        int L = xcd_order(blockIdx.x, pc.start[pc.n]);
+        int q = 0;
+        while (q + 1 < pc.n && L >= pc.start[q + 1]) q++;
+        L -= pc.start[q];
+        jt = L % pc.njt[q];
+        kt = (L / pc.njt[q]) % nkt;
+        ch = L / (pc.njt[q] * nkt);
+        ia = pc.ia[q]; ib = pc.ib[q]; ja = pc.ja[q]; jb = pc.jb[q];
     }
     const int lane = threadIdx.x, row = threadIdx.y;
     const int k = kt * 63 - 1 + lane;
-    const int j = __builtin_amdgcn_readfirstlane(jt * (TJ - 1) - 1 + row);
-    const int i0 = ch * ichunk, i1 = min(i0 + ichunk, NX);
-    const bool own = lane > 0 && row > 0 && k < NZ && j < NY;
+    const int j = __builtin_amdgcn_readfirstlane(ja + jt * (TJ - 1) - 1 + row);
+    const int i0 = ia + ch * ichunk, i1 = min(i0 + ichunk, ib);
+    const bool own = lane > 0 && row > 0 && k < NZ && j < jb;
     const int kc = min(k, NZ - 1), jc = min(j, NY - 1);
     const int kk = k < NZ ? k : NZ, jj = j < NY ? j : NY;   // beyond the grid: outside every domain
     const int sz = g.sz;
@@ -572,6 +582,297 @@ __global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, i
     }
 }
 
+// ----------------------------------------------------------------- row-blocked step kernel
+
+__device__ inline float ldb(const char *b, unsigned o) { return *(const float *)(b + o); }
+__device__ inline void stb(char *b, unsigned o, float v) { *(float *)(b + o) = v; }
+// Lane l of a wave receives v from lane l+1 (DPP wave_shl:1); lane 63 gets 0.
+__device__ inline float from_upper_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// Layout of the row-blocked kernel's table image (32-bit words), copied to
+// LDS by every block: per axis a at a*512: 1/(kappa d) for E at x < ew
+// (near, @0) and x >= N-ew (far, @64), the same for H (@128, @192), CPML
+// recursion be, ce/d (@256, @320: near layers then far), bh, ch/d (@384,
+// @448); then the interior 1/(kappa d) for E (@1536+a) and H (@1539+a), ew
+// (@1542+a), and the psi pointers at word 1548: psi_H sets 0 and 1 (6 each),
+// psi_E (6), 64-bit each.
+constexpr int kAuxAxis = 512, kAuxIkE0 = 1536, kAuxIkH0 = 1539, kAuxEw = 1542, kAuxPsi = 1548;
+typedef __attribute__((address_space(1))) float gfloat;   // global memory (psi pointers read from LDS)
+
+template <bool NT>
+__device__ inline void stn(char *b, unsigned o, float v) {
+    float *p = (float *)(b + o);
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// One full leapfrog step of the whole grid (TFSF excluded: k_tfsf_fix adds
+// it).  A block spans a z segment with W waves, k = seg*(64W-2) - 1 + t for
+// t = 64w + lane; its first and last lanes are halo.  Each thread holds R
+// rows of y in registers (row 0 is halo, recomputed) and the block marches x
+// over one chunk.  Neighbours: E at k+1 and H at k-1 from the adjacent lane
+// (DPP) or, across a wave edge, from the adjacent wave through LDS; E at j+1
+// and H at j-1 from the adjacent row's registers; H at i-1 from the previous
+// x step.  One barrier per x step.  Loads of plane i+1 are issued while
+// plane i is computed; fields are stored non-temporally (each value is
+// written once and not read again in this step).  Field offsets are 32-bit
+// byte offsets from the allocation base, so every field access is a scalar
+// base plus a vector offset.  CPML and the domain tests (main.cu:829-1014,
+// 1172-1767) follow the uniformity of their axis: x per plane, y per row, z
+// per lane (only the edge waves of a segment hold z-slab lanes); per
+// component the psi terms add in the reference's order.  CPML tables and psi
+// pointers live in LDS (the table image above), which keeps scalar registers
+// for the field pointers.
+template <int W, int R, bool GEN>
+__global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
+    __shared__ float sK[2][2][W][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
+    __shared__ float sE[2][2][W][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
+    __shared__ __align__(16) unsigned sT[kAuxPsi + 36];
+    const int NX = ra.NX, NY = ra.NY, NZ = ra.NZ;
+    for (int q = threadIdx.y * 64 + threadIdx.x; q < kAuxPsi + 36; q += 64 * W) sT[q] = ra.aux[q];
+    const float *sTf = (const float *)sT;
+    int L = blockIdx.x, pc = 0;
+    while (pc + 1 < ra.npiece && L >= ra.start[pc + 1]) pc++;
+    L -= ra.start[pc];
+    const int ia = ra.box[pc][0], ib = ra.box[pc][1], ja = ra.box[pc][2], jbox = ra.box[pc][3];
+    const int njt = (jbox - ja + R - 1) / R;
+    const int jt = L % njt;
+    L /= njt;
+    const int seg = L % ra.nseg;
+    const int ch = L / ra.nseg;
+    const int lane = threadIdx.x, w = threadIdx.y, t = w * 64 + lane;
+    const int k = seg * (64 * W - 2) - 1 + t;
+    const int kc = min(k, NZ - 1);
+    const bool kown = t > 0 && t < 64 * W - 1 && k < NZ;
+    const int i0 = ia + ch * ra.ichunk, i1 = min(i0 + ra.ichunk, ib);
+    const int pp = ra.pp;
+    auto hlay = [&](int a, int x) { return x < ra.Nc[a] ? x : (x >= ra.F[a] ? ra.Nc[a] + x - ra.F[a] : -1); };
+    auto elay = [&](int a, int x) { return x <= ra.Nc[a] ? x : (x >= ra.F[a] ? ra.Nc[a] + 1 + x - ra.F[a] : -1); };
+    // 1/(kappa d) of axis a (length n) at x, E positions if e
+    auto ik = [&](int a, bool e, int x, int n) {
+        const int ew = (int)sT[kAuxEw + a];
+        const int base = a * kAuxAxis + (e ? 0 : 128);
+        const int q = x < ew ? base + x : (x >= n - ew ? base + 64 + x - (n - ew) : (e ? kAuxIkE0 : kAuxIkH0) + a);
+        return sTf[q];
+    };
+    auto coef = [&](int a, int which, int layer) { return sTf[a * kAuxAxis + 256 + which * 64 + layer]; };
+    auto psi = [&](int idx) { return *(gfloat *const *)&sT[kAuxPsi + 2 * idx]; };   // 0-5 set 0, 6-11 set 1, 12-17 psi_E
+    __syncthreads();
+
+    // z: per lane
+    const bool kA = k >= 0 && k < NZ - 1, kB = k >= 0 && k < NZ, kC = k > 0 && k < NZ - 1;
+    const int kq = max(kc, 0);
+    const int lzh = kA ? hlay(2, k) : -1, lze = kC ? elay(2, k) : -1;
+    const bool zh = __ballot(lzh >= 0) != 0, ze = __ballot(lze >= 0) != 0;
+    const float ikz_h = ik(2, false, kq, NZ), ikz_e = ik(2, true, kq, NZ);
+    const float bzh = lzh >= 0 ? coef(2, 2, lzh) : 0.f, czh = lzh >= 0 ? coef(2, 3, lzh) : 0.f;
+    const float bze = lze >= 0 ? coef(2, 0, lze) : 0.f, cze = lze >= 0 ? coef(2, 1, lze) : 0.f;
+    const int Lzh = 2 * ra.Nc[2], Lze = 2 * (ra.Nc[2] + 1), Lyh = 2 * ra.Nc[1], Lye = 2 * (ra.Nc[1] + 1);
+
+    // y: per row, j_r = ja + jt*R - 1 + r (row R+1 only feeds the j+1 neighbour); own rows < jbox
+    const int jb = ja + jt * R - 1;
+    const unsigned sz4 = (unsigned)ra.sz * 4u, sxy4 = ra.sxy4;
+    unsigned ro[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) ro[r] = (unsigned)(min(jb + r, NY) + 1) * sz4 + (unsigned)(kc + 1) * 4u;
+    auto jA = [&](int r) { return !GEN || (jb + r >= 0 && jb + r < NY - 1); };
+    auto jB = [&](int r) { return !GEN || (jb + r >= 0 && jb + r < NY); };
+    auto jC = [&](int r) { return !GEN || (jb + r > 0 && jb + r < NY - 1); };
+    auto jq = [&](int r) { return min(max(jb + r, 0), NY - 1); };
+
+    const float Chh = ra.Chh, Che = ra.Che, Cee = ra.Cee, Ceh = ra.Ceh;
+    const char *bEx = ra.Ein[0], *bEy = ra.Ein[1], *bEz = ra.Ein[2];
+    const char *bHx = ra.Hin[0], *bHy = ra.Hin[1], *bHz = ra.Hin[2];
+    char *oEx = ra.Eout[0], *oEy = ra.Eout[1], *oEz = ra.Eout[2];
+    char *oHx = ra.Hout[0], *oHy = ra.Hout[1], *oHz = ra.Hout[2];
+    const bool last_lane = lane == 63 && w < W - 1, first_lane = lane == 0 && w > 0;
+
+    auto e_up = [&](const float *x, const float *y, int b, float *xk, float *yk) {
+#pragma unroll
+        for (int r = 0; r <= R; r++) { xk[r] = from_upper_lane(x[r]); yk[r] = from_upper_lane(y[r]); }
+        if (last_lane) {
+#pragma unroll
+            for (int r = 0; r <= R; r++) { xk[r] = sE[b][0][w + 1][r]; yk[r] = sE[b][1][w + 1][r]; }
+        }
+    };
+    auto e_publish = [&](const float *x, const float *y, int b) {
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r <= R; r++) { sE[b][0][w][r] = x[r]; sE[b][1][w][r] = y[r]; }
+        }
+    };
+    // H of plane ip for rows 0..R from E^n of plane ip (rows 0..R, ex and ez of
+    // row R+1, ex and ey at k+1), ey and ez of plane ip+1 and H^{n-1/2} of
+    // plane ip.  wr: write psi_H of the own cells.
+    auto hplane = [&](int ip, const float *ex, const float *ey, const float *ez, float exj, float ezj,
+                      const float *exk, const float *eyk, const float *ny, const float *nz, const float (*ho)[R + 1],
+                      bool wr, float *hx, float *hy, float *hz) {
+        const bool iA = !GEN || ip < NX - 1;
+        const float ikx_h = GEN ? ik(0, false, ip, NX) : ra.ik0[1][0];
+        const int lxh = GEN && iA ? hlay(0, ip) : -1;
+        const int hin = pp * 6, hout = (1 - pp) * 6;
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            const int j = jb + r;
+            const bool own = wr && r > 0 && kown && j < jbox;
+            const float iky_h = GEN ? ik(1, false, jq(r), NY) : ra.ik0[1][1];
+            const float xj = r < R ? ex[r + 1] : exj, zj = r < R ? ez[r + 1] : ezj;
+            const float dEy_z = eyk[r] - ey[r], dEz_y = zj - ez[r];
+            const float dEz_x = nz[r] - ez[r], dEx_z = exk[r] - ex[r];
+            const float dEx_y = xj - ex[r], dEy_x = ny[r] - ey[r];
+            float vx = Chh * ho[0][r] + Che * (dEy_z * ikz_h - dEz_y * iky_h);
+            float vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - dEx_z * ikz_h);
+            float vz = Chh * ho[2][r] + Che * (dEx_y * iky_h - dEy_x * ikx_h);
+            if (zh && iA && jA(r) && lzh >= 0) {
+                const long q = ((long)ip * NY + j) * Lzh + lzh;
+                gfloat *pa = psi(hin + 0), *pb = psi(hin + 2);
+                const float a = bzh * pa[q] + czh * dEy_z;
+                const float b = bzh * pb[q] + czh * dEx_z;
+                vx += Che * a;
+                vy -= Che * b;
+                if (own) { psi(hout + 0)[q] = a; psi(hout + 2)[q] = b; }
+            }
+            float pyz = 0.f;
+            const int ly = GEN && jA(r) ? hlay(1, j) : -1;
+            if (iA && ly >= 0 && kA) {
+                const long q = ((long)ip * Lyh + ly) * NZ + k;
+                const float by = coef(1, 2, ly), cyv = coef(1, 3, ly);
+                const float a = by * psi(hin + 1)[q] + cyv * dEz_y;
+                pyz = by * psi(hin + 5)[q] + cyv * dEx_y;
+                vx -= Che * a;
+                if (own) { psi(hout + 1)[q] = a; psi(hout + 5)[q] = pyz; }
+            }
+            if (lxh >= 0 && jA(r) && kA) {
+                const long q = ((long)lxh * NY + j) * NZ + k;
+                const float bx = coef(0, 2, lxh), cxv = coef(0, 3, lxh);
+                const float a = bx * psi(hin + 3)[q] + cxv * dEz_x;
+                const float b = bx * psi(hin + 4)[q] + cxv * dEy_x;
+                vy += Che * a;
+                vz -= Che * b;
+                if (own) { psi(hout + 3)[q] = a; psi(hout + 4)[q] = b; }
+            }
+            if (iA && ly >= 0 && kA) vz += Che * pyz;
+            hx[r] = jA(r) && kA ? vx : ho[0][r];
+            hy[r] = iA && jB(r) && kA ? vy : ho[1][r];
+            hz[r] = iA && jA(r) && kB ? vz : ho[2][r];
+        }
+    };
+
+    float ex[R + 1], ey[R + 1], ez[R + 1], exj, ezj;     // E^n of plane i
+    float nx[R + 1], ny[R + 1], nz[R + 1];               // E^n of plane i+1
+    float ho[3][R + 1];                                   // H^{n-1/2} of plane i
+    float hyP[R + 1], hzP[R + 1];                         // H^{n+1/2} of plane i-1
+    auto load_e = [&](int ip, float *x, float *y, float *z) {
+        const unsigned pl = (unsigned)(ip + 1) * sxy4;
+#pragma unroll
+        for (int r = 0; r <= R; r++) { x[r] = ldb(bEx, pl + ro[r]); y[r] = ldb(bEy, pl + ro[r]); z[r] = ldb(bEz, pl + ro[r]); }
+    };
+    auto load_h = [&](int ip) {
+        const unsigned pl = (unsigned)(ip + 1) * sxy4;
+#pragma unroll
+        for (int r = 0; r <= R; r++) { ho[0][r] = ldb(bHx, pl + ro[r]); ho[1][r] = ldb(bHy, pl + ro[r]); ho[2][r] = ldb(bHz, pl + ro[r]); }
+    };
+    // prologue: H^{n+1/2} of plane i0-1 (not needed at i0 = 0: ey, ez are not
+    // updated there), E^n of planes i0 and i0+1, H of i0
+#pragma unroll
+    for (int r = 0; r <= R; r++) { hyP[r] = 0.f; hzP[r] = 0.f; }
+    load_e(i0, ex, ey, ez);
+    if (i0 > 0) {
+        float px[R + 1], py[R + 1], pz[R + 1], pxk[R + 1], pyk[R + 1], hx[R + 1], hy[R + 1], hz[R + 1];
+        load_e(i0 - 1, px, py, pz);
+        load_h(i0 - 1);
+        const unsigned pl = (unsigned)i0 * sxy4;
+        const float pxj = ldb(bEx, pl + ro[R + 1]), pzj = ldb(bEz, pl + ro[R + 1]);
+        e_publish(px, py, 1);
+        __syncthreads();
+        e_up(px, py, 1, pxk, pyk);
+        hplane(i0 - 1, px, py, pz, pxj, pzj, pxk, pyk, ey, ez, ho, false, hx, hy, hz);
+#pragma unroll
+        for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
+    }
+    e_publish(ex, ey, 0);
+    {
+        const unsigned pc = (unsigned)(i0 + 1) * sxy4;
+        exj = ldb(bEx, pc + ro[R + 1]);
+        ezj = ldb(bEz, pc + ro[R + 1]);
+    }
+    load_e(i0 + 1, nx, ny, nz);
+    load_h(i0);
+    __syncthreads();
+    int buf = 0;
+    for (int i = i0; i < i1; i++) {
+        const bool more = i + 1 < i1;
+        const unsigned pl = (unsigned)(i + 1) * sxy4;
+        float hx[R + 1], hy[R + 1], hz[R + 1], exk[R + 1], eyk[R + 1];
+        e_up(ex, ey, buf, exk, eyk);
+        hplane(i, ex, ey, ez, exj, ezj, exk, eyk, ny, nz, ho, true, hx, hy, hz);
+        if (more) load_h(i + 1);
+#pragma unroll
+        for (int r = 1; r <= R; r++) {
+            if (kown && jb + r < jbox) { stn<true>(oHx, pl + ro[r], hx[r]); stn<true>(oHy, pl + ro[r], hy[r]); stn<true>(oHz, pl + ro[r], hz[r]); }
+            if (lane == 63) { sK[buf][0][w][r] = hy[r]; sK[buf][1][w][r] = hx[r]; }
+        }
+        e_publish(nx, ny, buf ^ 1);
+        float hyk[R + 1], hxk[R + 1];
+#pragma unroll
+        for (int r = 1; r <= R; r++) { hyk[r] = from_lower_lane(hy[r]); hxk[r] = from_lower_lane(hx[r]); }
+        __syncthreads();
+        if (first_lane) {
+#pragma unroll
+            for (int r = 1; r <= R; r++) { hyk[r] = sK[buf][0][w - 1][r]; hxk[r] = sK[buf][1][w - 1][r]; }
+        }
+        const bool iA = !GEN || i < NX - 1, iC = !GEN || (i > 0 && i < NX - 1);
+        const float ikx_e = GEN ? ik(0, true, i, NX) : ra.ik0[0][0];
+        const int lxe = GEN && iC ? elay(0, i) : -1;
+#pragma unroll
+        for (int r = 1; r <= R; r++) {
+            const int j = jb + r;
+            const bool own = kown && j < jbox;
+            const float iky_e = GEN ? ik(1, true, jq(r), NY) : ra.ik0[0][1];
+            const float dHz_y = hz[r] - hz[r - 1], dHy_z = hy[r] - hyk[r];
+            const float dHx_z = hx[r] - hxk[r], dHz_x = hz[r] - hzP[r];
+            const float dHy_x = hy[r] - hyP[r], dHx_y = hx[r] - hx[r - 1];
+            float vx = Cee * ex[r] + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
+            float vy = Cee * ey[r] + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
+            float vz = Cee * ez[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
+            const bool ux = iA && jC(r) && kC, uy = iC && jA(r) && kC, uz = iC && jC(r) && kA;
+            if (ze && iA && lze >= 0) {
+                const long q = ((long)i * NY + j) * Lze + lze;
+                if (jC(r)) { gfloat *pe = psi(12 + 0); const float a = bze * pe[q] + cze * dHy_z; vx -= Ceh * a; if (own) pe[q] = a; }
+                if (iC && jA(r)) { gfloat *pe = psi(12 + 2); const float a = bze * pe[q] + cze * dHx_z; vy += Ceh * a; if (own) pe[q] = a; }
+            }
+            const int ly = GEN && jC(r) ? elay(1, j) : -1;
+            if (iA && ly >= 0) {
+                const long q = ((long)i * Lye + ly) * NZ + k;
+                const float by = coef(1, 0, ly), cyv = coef(1, 1, ly);
+                if (kC) { gfloat *pe = psi(12 + 1); const float a = by * pe[q] + cyv * dHz_y; vx += Ceh * a; if (own) pe[q] = a; }
+                if (iC && kA) { gfloat *pe = psi(12 + 4); const float a = by * pe[q] + cyv * dHx_y; vz -= Ceh * a; if (own) pe[q] = a; }
+            }
+            if (lxe >= 0) {
+                const long q = ((long)lxe * NY + j) * NZ + k;
+                const float bx = coef(0, 0, lxe), cxv = coef(0, 1, lxe);
+                if (jA(r) && kC) { gfloat *pe = psi(12 + 3); const float a = bx * pe[q] + cxv * dHz_x; vy -= Ceh * a; if (own) pe[q] = a; }
+                if (jC(r) && kA) { gfloat *pe = psi(12 + 5); const float a = bx * pe[q] + cxv * dHy_x; vz += Ceh * a; if (own) pe[q] = a; }
+            }
+            if (own) {
+                stn<true>(oEx, pl + ro[r], ux ? vx : ex[r]);
+                stn<true>(oEy, pl + ro[r], uy ? vy : ey[r]);
+                stn<true>(oEz, pl + ro[r], uz ? vz : ez[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; ex[r] = nx[r]; ey[r] = ny[r]; ez[r] = nz[r]; }
+        if (more) {
+            const unsigned pn = pl + sxy4;
+            exj = ldb(bEx, pn + ro[R + 1]); ezj = ldb(bEz, pn + ro[R + 1]);
+            load_e(i + 2, nx, ny, nz);
+        }
+        buf ^= 1;
+    }
+}
+
 // ----------------------------------------------------------------- split path kernels
 
 __global__ void k_split_h(StepDev s) {
@@ -626,6 +927,50 @@ __global__ void k_tfsf_split(StepDev s) {
         float de = tfsf_e(t, c, i, j, k);
         if (de != 0.f) s.Eout[c][p] += de;
     }
+}
+
+// TFSF corrections after a fused step that left them out (tfsf_on = 0):
+// H += dH on the H face layers and E += Cee dE + Ceh curl(dH) on the E cells
+// whose update saw those H values.  The reference adds dH to H before the E
+// update and dE to E^n (main.cu:2791-2901); the update is linear, so adding
+// the propagated terms afterwards gives the same fields up to rounding.
+// Cells: per normal axis a the planes t0-1, t0, t1, t1+1 spanning [t0-1, t1+1]
+// along the other axes; a cell lying on planes of several axes is taken by
+// the lowest one.  Requires the region free of CPML and inside every domain.
+__global__ void k_tfsf_fix(StepDev s) {
+    const TfsfDev &t = s.tf;
+    const int a = blockIdx.z >> 2, q = blockIdx.z & 3;
+    const int b0 = a == 0 ? 1 : 0, b1 = a == 2 ? 1 : 2;   // other axes, b1 fastest in memory
+    const int u = pick3(b0, t.t0[0], t.t0[1], t.t0[2]) - 1 + (int)blockIdx.y;
+    const int v = pick3(b1, t.t0[0], t.t0[1], t.t0[2]) - 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (u > pick3(b0, t.t1[0], t.t1[1], t.t1[2]) + 1 || v > pick3(b1, t.t1[0], t.t1[1], t.t1[2]) + 1) return;
+    const int t0a = pick3(a, t.t0[0], t.t0[1], t.t0[2]), t1a = pick3(a, t.t1[0], t.t1[1], t.t1[2]);
+    const int xa = q == 0 ? t0a - 1 : (q == 1 ? t0a : (q == 2 ? t1a : t1a + 1));
+    const int i = a == 0 ? xa : u;
+    const int j = a == 1 ? xa : (a == 0 ? u : v);
+    const int k = a == 2 ? xa : v;
+    auto on = [&](int ax, int x) {
+        const int lo = pick3(ax, t.t0[0], t.t0[1], t.t0[2]), hi = pick3(ax, t.t1[0], t.t1[1], t.t1[2]);
+        return x == lo - 1 || x == lo || x == hi || x == hi + 1;
+    };
+    if (a >= 1 && on(0, i)) return;
+    if (a >= 2 && on(1, j)) return;
+    const long p = gidx(s.g, i, j, k);
+    const float dhx = tfsf_h(t, 0, i, j, k), dhy = tfsf_h(t, 1, i, j, k), dhz = tfsf_h(t, 2, i, j, k);
+    if (dhx != 0.f) s.Hout[0][p] += dhx;
+    if (dhy != 0.f) s.Hout[1][p] += dhy;
+    if (dhz != 0.f) s.Hout[2][p] += dhz;
+    const float Cee = s.c.Cee, Ceh = s.c.Ceh;
+    const float ikx = s.cp[0].ik_e[i], iky = s.cp[1].ik_e[j], ikz = s.cp[2].ik_e[k];
+    const float dz_jm = tfsf_h(t, 2, i, j - 1, k), dx_jm = tfsf_h(t, 0, i, j - 1, k);
+    const float dy_km = tfsf_h(t, 1, i, j, k - 1), dx_km = tfsf_h(t, 0, i, j, k - 1);
+    const float dz_im = tfsf_h(t, 2, i - 1, j, k), dy_im = tfsf_h(t, 1, i - 1, j, k);
+    const float fx = Cee * tfsf_e(t, 0, i, j, k) + Ceh * ((dhz - dz_jm) * iky - (dhy - dy_km) * ikz);
+    const float fy = Cee * tfsf_e(t, 1, i, j, k) + Ceh * ((dhx - dx_km) * ikz - (dhz - dz_im) * ikx);
+    const float fz = Cee * tfsf_e(t, 2, i, j, k) + Ceh * ((dhy - dy_im) * ikx - (dhx - dx_jm) * iky);
+    if (fx != 0.f) s.Eout[0][p] += fx;
+    if (fy != 0.f) s.Eout[1][p] += fy;
+    if (fz != 0.f) s.Eout[2][p] += fz;
 }
 
 // One thread per sample column of the E half (H taken at the same cell).
@@ -1352,19 +1697,81 @@ __global__ void k_probe(const float *ex, const float *ey, const float *ez, long 
 
 // ================================================================ launch wrappers
 
-hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, hipStream_t st) {
+hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st) {
+    if (nbox < 1 || nbox > 4 || ichunk < 1) return hipErrorInvalidValue;
     const int nkt = (s.g.NZ + 62) / 63;
-    const int njt = (s.g.NY + tj - 2) / (tj - 1);
-    const int nchunk = (s.g.NX + ichunk - 1) / ichunk;
-    const long nblk = (long)nkt * njt * nchunk;
-    if (nblk <= 0 || nblk > 0x7fffffff) return hipErrorInvalidValue;
+    Pieces pc{};
+    pc.n = nbox;
+    long total = 0;
+    for (int q = 0; q < nbox; q++) {
+        pc.ia[q] = box[q][0]; pc.ib[q] = box[q][1]; pc.ja[q] = box[q][2]; pc.jb[q] = box[q][3];
+        if (pc.ia[q] < 0 || pc.ib[q] > s.g.NX || pc.ia[q] >= pc.ib[q] || pc.ja[q] < 0 || pc.jb[q] > s.g.NY ||
+            pc.ja[q] >= pc.jb[q])
+            return hipErrorInvalidValue;
+        pc.njt[q] = (pc.jb[q] - pc.ja[q] + tj - 2) / (tj - 1);
+        pc.start[q] = (int)total;
+        total += (long)pc.njt[q] * nkt * ((pc.ib[q] - pc.ia[q] + ichunk - 1) / ichunk);
+    }
+    if (total > 0x7fffffff) return hipErrorInvalidValue;
+    pc.start[nbox] = (int)total;
+    const dim3 grid((unsigned)total);
     switch (tj) {
-        case 4: hipLaunchKernelGGL(k_fused_step<4>, dim3((unsigned)nblk), dim3(64, 4), 0, st, s, ichunk, nkt, njt, (int)nblk); break;
-        case 8: hipLaunchKernelGGL(k_fused_step<8>, dim3((unsigned)nblk), dim3(64, 8), 0, st, s, ichunk, nkt, njt, (int)nblk); break;
-        case 16: hipLaunchKernelGGL(k_fused_step<16>, dim3((unsigned)nblk), dim3(64, 16), 0, st, s, ichunk, nkt, njt, (int)nblk); break;
+        case 4: hipLaunchKernelGGL(k_fused_step<4>, grid, dim3(64, 4), 0, st, s, ichunk, nkt, pc); break;
+        case 8: hipLaunchKernelGGL(k_fused_step<8>, grid, dim3(64, 8), 0, st, s, ichunk, nkt, pc); break;
+        case 16: hipLaunchKernelGGL(k_fused_step<16>, grid, dim3(64, 16), 0, st, s, ichunk, nkt, pc); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+
+template <int W, bool GEN>
+static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
+    switch (R) {
+        case 2: hipLaunchKernelGGL((k_step_rows<W, 2, GEN>), grid, dim3(64, W), 0, st, ra); break;
+        case 3: hipLaunchKernelGGL((k_step_rows<W, 3, GEN>), grid, dim3(64, W), 0, st, ra); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <bool GEN>
+static hipError_t launch_rows_g(const RowsArgs &ra, int R, int W, dim3 grid, hipStream_t st) {
+    switch (W) {
+        case 2: return launch_rows_w<2, GEN>(ra, R, grid, st);
+        case 3: return launch_rows_w<3, GEN>(ra, R, grid, st);
+        case 4: return launch_rows_w<4, GEN>(ra, R, grid, st);
+        case 5: return launch_rows_w<5, GEN>(ra, R, grid, st);
+        case 6: return launch_rows_w<6, GEN>(ra, R, grid, st);
+        case 8: return launch_rows_w<8, GEN>(ra, R, grid, st);
+        case 12: return launch_rows_w<12, GEN>(ra, R, grid, st);
+        default: return launch_rows_w<16, GEN>(ra, R, grid, st);
+    }
+}
+
+int rows_waves(int NZ) {
+    static const int ws[] = {2, 3, 4, 5, 6, 8, 12, 16};
+    for (int w : ws)
+        if (64 * w - 2 >= NZ + 1) return w;   // one segment: lanes k = -1 .. 64w-2 cover the owned 0 .. NZ-1
+    return 16;
+}
+
+hipError_t launch_step_rows(RowsArgs ra, int R, bool general, hipStream_t st) {
+    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4) return hipErrorInvalidValue;
+    const int W = rows_waves(ra.NZ);
+    ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
+    long total = 0;
+    for (int q = 0; q < ra.npiece; q++) {
+        const int *b = ra.box[q];
+        if (b[0] < 0 || b[1] > ra.NX || b[0] >= b[1] || b[2] < 0 || b[3] > ra.NY || b[2] >= b[3]) return hipErrorInvalidValue;
+        if (!general && (b[0] < 1 || b[1] > ra.NX - 1 || b[2] < 1 || b[3] > ra.NY - 1)) return hipErrorInvalidValue;
+        ra.start[q] = (int)total;
+        total += (long)((b[3] - b[2] + R - 1) / R) * ra.nseg * ((b[1] - b[0] + ra.ichunk - 1) / ra.ichunk);
+    }
+    if (total > 0x7fffffff) return hipErrorInvalidValue;
+    ra.start[ra.npiece] = (int)total;
+    const dim3 grid((unsigned)total);
+    return general ? launch_rows_g<true>(ra, R, W, grid, st) : launch_rows_g<false>(ra, R, W, grid, st);
 }
 
 hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, hipStream_t st) {
@@ -1413,6 +1820,15 @@ hipError_t launch_tfsf_split(const StepDev &s, hipStream_t st) {
     const int nk = t.t1[2] - t.t0[2] + 2, nj = t.t1[1] - t.t0[1] + 2, ni = t.t1[0] - t.t0[0] + 2;
     dim3 grid((nk + 63) / 64, nj, ni), block(64);
     hipLaunchKernelGGL(k_tfsf_split, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_tfsf_fix(const StepDev &s, hipStream_t st) {
+    const TfsfDev &t = s.tf;
+    int span = 0;
+    for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
+    dim3 grid((span + 63) / 64, span, 12), block(64);
+    hipLaunchKernelGGL(k_tfsf_fix, grid, block, 0, st, s);
     return hipGetLastError();
 }
 

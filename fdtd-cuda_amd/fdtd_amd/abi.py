@@ -70,6 +70,8 @@ class GridInfo(C.Structure):
         ("n_dispersive", C.c_int * 3),
         ("box_lo", C.c_int * 3), ("box_hi", C.c_int * 3),
         ("sample_count", C.c_int),
+        ("fused_path", C.c_int),
+        ("clean", C.c_int * 4),
     ]
 
 

@@ -103,6 +103,10 @@ typedef struct fdtd_grid_info {
     int    n_dispersive[3];    /* E-component cells on the dispersive (ADE) branch             */
     int    box_lo[3], box_hi[3];       /* bounding box of dispersive cells, [lo, hi)          */
     int    sample_count;       /* DFT samples taken so far                                     */
+    int    fused_path;         /* fdtd_step's field update: 1 row-blocked kernel + TFSF fix-up,
+                                  0 general kernel with TFSF inside (no reference counterpart) */
+    int    clean[4];           /* x [clean0, clean1) by y [clean2, clean3), all z: cells run by the
+                                  CPML-free variant of the row-blocked kernel; -1 when none       */
 } fdtd_grid_info;
 
 typedef struct fdtd_sim fdtd_sim;
@@ -191,8 +195,11 @@ int  fdtd_get_dft(fdtd_sim *sim, int face, int comp, double *out);
  * be NULL. */
 int  fdtd_flux(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *raw);
 
-/* Device time of the fused step kernel accumulated since the last reset
- * (HIP events on the solver stream); enable=1 starts recording. */
+/* Device time of the field update of fdtd_step accumulated since the last
+ * reset (HIP events on the solver stream; no reference counterpart).  Returns
+ * the time and count recorded so far, then resets and sets the mode: 0 off,
+ * 1 all field-update kernels of a step, 2 the clean-box launch of the
+ * row-blocked kernel alone (the dominant one; same as 1 when there is none). */
 int  fdtd_kernel_timing(fdtd_sim *sim, int enable, double *total_ms, long *launches);
 
 #ifdef __cplusplus

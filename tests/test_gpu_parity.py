@@ -94,30 +94,65 @@ def test_dft_and_flux(fused):
         assert err < 1e-4, (k, err)
 
 
+def group_errors(fs, fo):
+    """L-inf differences normalised by the largest E (or H) value, as field_errors."""
+    me = max(np.abs(f).max() for f in fo[:3])
+    mh = max(np.abs(f).max() for f in fo[3:])
+    return [float(np.abs(fs[c] - fo[c]).max() / (me if c < 3 else mh)) for c in range(6)]
+
+
 def test_fused_equals_split():
+    """The fused step (TFSF added after the update, linear in the fields)
+    against the split path (TFSF between the H and E updates, reference
+    order): the same fields up to FP32 rounding."""
     p = small_params(O, steps=400)
     a = fdtd_amd.Simulation(p)
     b = fdtd_amd.Simulation(p)
     a.step(400)
     b.split_step(400)
-    for c in range(6):
-        fa, fb = a.get_field(c), b.get_field(c)
-        err = rel(fa, fb)
-        print(f"fused vs split comp={c} rel={err:.3e} bitwise={np.array_equal(fa, fb)}", flush=True)
-        assert err < 1e-5
+    errs = group_errors([a.get_field(c) for c in range(6)], [b.get_field(c) for c in range(6)])
+    print(f"fused vs split errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < 1e-5
 
 
-@pytest.mark.parametrize("tj", [4, 8, 16])
-def test_tile_shapes_agree(tj, monkeypatch):
+@pytest.mark.parametrize("tj,ichunk,rows,cchunk", [(4, 7, 2, 5), (8, 7, 3, 3), (16, 5, 4, 9), (8, 32, 2, 64)])
+def test_tile_shapes_agree(tj, ichunk, rows, cchunk, monkeypatch):
+    """Tile shapes of both step kernels (general pieces: TJ rows x 63 lanes,
+    x chunk; clean region: rows per thread, x chunk) give the split path's
+    fields."""
     p = small_params(O, steps=350)
     ref = fdtd_amd.Simulation(p)
     ref.split_step(350)
-    monkeypatch.setenv("FDTD_TJ", str(tj))
-    monkeypatch.setenv("FDTD_ICHUNK", "7")
+    for k, v in (("FDTD_TJ", tj), ("FDTD_ICHUNK", ichunk), ("FDTD_ROWS", rows), ("FDTD_CCHUNK", cchunk)):
+        monkeypatch.setenv(k, str(v))
     s = fdtd_amd.Simulation(p)
     s.step(350)
-    for c in range(6):
-        assert rel(s.get_field(c), ref.get_field(c)) < 1e-5
+    errs = group_errors([s.get_field(c) for c in range(6)], [ref.get_field(c) for c in range(6)])
+    print(f"tile tj={tj} ichunk={ichunk} rows={rows} cchunk={cchunk} errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < 1e-5
+
+
+@pytest.mark.parametrize("sphere", [1, 0])
+def test_general_kernel_alone(sphere, monkeypatch):
+    """FDTD_PATH=general: the general fused kernel over the whole grid with
+    TFSF inside, against the oracle."""
+    monkeypatch.setenv("FDTD_PATH", "general")
+    p = small_params(O, steps=400, sphere=sphere, phi=0.0 if sphere else 90.0)
+    o, s = run_pair(p, 400, True)
+    errs = field_errors(s, o)
+    print(f"general path sphere={sphere} errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < TOL, errs
+
+
+def test_uneven_grid_and_cpml():
+    """Unequal edges and CPML thicknesses (clean region bounds differ per axis)."""
+    p = small_params(O, steps=360)
+    p.NCELLX, p.NCELLY, p.NCELLZ = 22, 26, 30
+    p.NcpmlX, p.NcpmlY, p.NcpmlZ = 5, 7, 6
+    o, s = run_pair(p, 360, True)
+    errs = field_errors(s, o)
+    print(f"uneven errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < TOL, errs
 
 
 def test_mfma_split_bf16_matches_fp64_contraction():
