@@ -86,10 +86,13 @@ struct fdtd_sim {
     bool use_rows = false;
     bool has_clean = false;         // clean box non-empty: clean variant there, general variant on the rest
     int cbox[4] = {};               // clean box {ilo, ihi, jlo, jhi}
-    int gbox[4][4] = {};            // general pieces
-    int ngbox = 0;
+    int xbox[2][4] = {}, ybox[2][4] = {}, cbox4[4][4] = {};   // x-general, y-general and corner pieces
+    int nxbox = 0, nybox = 0, ncorner = 0;
     float ik0[2][2] = {};           // interior 1/(kappa d): [E, H][x, y]
     unsigned *aux_d = nullptr;      // table image of the row-blocked kernel
+    bool concurrent = false;        // general pieces on a second stream beside the clean box (FDTD_CONCURRENT=1)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
     int rows = 2, cchunk = 8;       // own rows per thread and x chunk of the row-blocked kernel
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
@@ -459,23 +462,24 @@ int plan_rows(fdtd_sim *s) {
         s->ik0[1][a] = ikh[mid];
     }
     const int NX = S.N[0], NY = S.N[1];
-    s->ngbox = 0;
-    auto addg = [&](int i0, int i1, int j0, int j1) {
-        if (i0 < i1 && j0 < j1) {
-            int *b = s->gbox[s->ngbox++];
-            b[0] = i0; b[1] = i1; b[2] = j0; b[3] = j1;
-        }
+    s->nxbox = s->nybox = s->ncorner = 0;
+    s->has_clean = false;
+    if (!clean) return FDTD_OK;   // the general kernel runs alone
+    for (int a = 0; a < 3; a++)
+        if (S.cp[a].Nc < 1) return FDTD_OK;
+    s->has_clean = true;
+    s->cbox[0] = lo[0]; s->cbox[1] = hi[0]; s->cbox[2] = lo[1]; s->cbox[3] = hi[1];
+    auto add = [&](int (*b)[4], int &n, int i0, int i1, int j0, int j1) {
+        if (i0 < i1 && j0 < j1) { b[n][0] = i0; b[n][1] = i1; b[n][2] = j0; b[n][3] = j1; n++; }
     };
-    s->has_clean = clean;
-    if (clean) {
-        s->cbox[0] = lo[0]; s->cbox[1] = hi[0]; s->cbox[2] = lo[1]; s->cbox[3] = hi[1];
-        addg(0, lo[0], 0, NY);
-        addg(hi[0], NX, 0, NY);
-        addg(lo[0], hi[0], 0, lo[1]);
-        addg(lo[0], hi[0], hi[1], NY);
-    } else {
-        addg(0, NX, 0, NY);
-    }
+    add(s->xbox, s->nxbox, 0, lo[0], lo[1], hi[1]);
+    add(s->xbox, s->nxbox, hi[0], NX, lo[1], hi[1]);
+    add(s->ybox, s->nybox, lo[0], hi[0], 0, lo[1]);
+    add(s->ybox, s->nybox, lo[0], hi[0], hi[1], NY);
+    add(s->cbox4, s->ncorner, 0, lo[0], 0, lo[1]);
+    add(s->cbox4, s->ncorner, 0, lo[0], hi[1], NY);
+    add(s->cbox4, s->ncorner, hi[0], NX, 0, lo[1]);
+    add(s->cbox4, s->ncorner, hi[0], NX, hi[1], NY);
     float *ptrs[18];
     for (int q = 0; q < 2; q++)
         for (int x = 0; x < 6; x++) ptrs[q * 6 + x] = s->psiH[q][x];
@@ -483,7 +487,14 @@ int plan_rows(fdtd_sim *s) {
     std::memcpy(&img[1548], ptrs, sizeof(ptrs));
     int rc = upload(s, &s->aux_d, img);
     if (rc) return rc;
-    s->use_rows = true;
+    if (const char *v = std::getenv("FDTD_CONCURRENT")) s->concurrent = std::atoi(v) != 0;
+    if (s->concurrent && s->has_clean) {
+        if (hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess)
+            return set_err(s, FDTD_ERR_DEVICE, "second stream or events could not be created");
+    }
+    s->use_rows = s->has_clean;
     return FDTD_OK;
 }
 
@@ -508,22 +519,38 @@ RowsArgs make_rows(fdtd_sim *s) {
     return r;
 }
 
-// The row-blocked field update: the clean variant on the clean box, the
-// general variant on the pieces around it (independent cell sets).
-int launch_rows_update(fdtd_sim *s, hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
+// The row-blocked field update over independent cell sets: the clean
+// variant on the clean box, the x- and y-general variants on the slabs beside
+// it, and the general kernel (TFSF off) on the four corner columns.  With
+// FDTD_CONCURRENT=1 the side pieces run on a second stream.
+int launch_rows_update(fdtd_sim *s, const StepDev &d, hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
     RowsArgs r = make_rows(s);
-    if (s->ngbox) {
-        r.npiece = s->ngbox;
-        std::memcpy(r.box, s->gbox, sizeof(r.box));
-        HIPCHK(s, launch_step_rows(r, s->rows, true, s->stream));
+    const bool fork = s->stream2 != nullptr;
+    hipStream_t side = fork ? s->stream2 : s->stream;
+    if (fork) {
+        HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
     }
-    if (s->has_clean) {
-        r.npiece = 1;
-        std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
-        if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
-        HIPCHK(s, launch_step_rows(r, s->rows, false, s->stream));
-        if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));
+    if (s->nxbox) {
+        RowsArgs rx = r;
+        rx.npiece = s->nxbox;
+        std::memcpy(rx.box, s->xbox, sizeof(s->xbox));
+        HIPCHK(s, launch_step_rows(rx, 2, 1, side));
     }
+    if (s->nybox) {
+        RowsArgs ry = r;
+        ry.npiece = s->nybox;
+        std::memcpy(ry.box, s->ybox, sizeof(s->ybox));
+        HIPCHK(s, launch_step_rows(ry, 2, 2, side));
+    }
+    if (s->ncorner) HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, s->cbox4, s->ncorner, side));
+    if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
+    r.npiece = 1;
+    std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
+    if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
+    HIPCHK(s, launch_step_rows(r, s->rows, 0, s->stream));
+    if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));
+    if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     return FDTD_OK;
 }
 
@@ -628,7 +655,11 @@ int fdtd_destroy(fdtd_sim *s) {
     if (!s) return FDTD_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto &e : s->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    if (s->stream2) (void)hipStreamSynchronize(s->stream2);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     for (void *p : s->allocs) (void)hipFree(p);
+    if (s->stream2) (void)hipStreamDestroy(s->stream2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return FDTD_OK;
@@ -1040,11 +1071,11 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
             ev1 = next_event(s, 1);
             if (!ev0 || !ev1) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
         }
-        const bool only_clean = s->timing_what == 2 && s->use_rows && s->has_clean;
+        const bool only_clean = s->timing_what == 2 && s->use_rows;
         if (s->use_rows) {
             d.tfsf_on = 0;
             if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev0, s->stream));
-            int rc2 = launch_rows_update(s, only_clean ? ev0 : nullptr, only_clean ? ev1 : nullptr);
+            int rc2 = launch_rows_update(s, d, only_clean ? ev0 : nullptr, only_clean ? ev1 : nullptr);
             if (rc2) return rc2;
             HIPCHK(s, launch_tfsf_fix(d, s->stream));
             if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev1, s->stream));

@@ -178,11 +178,11 @@ struct Pieces {
 // allocation bases (element (-1,-1,-1)); aux is the table image (CPML tables,
 // interior 1/(kappa d), psi pointers; layout in kernels.hip, built by the
 // host).  The kernel runs the cells x in [box[q][0], box[q][1]) by y in
-// [box[q][2], box[q][3]), all z, of npiece boxes; the clean variant
-// (general = false) requires boxes with no x or y CPML, constant 1/(kappa d)
-// ik0[e][axis] (e: 0 E, 1 H) along x and y, every domain test true, and
-// H-clean planes and rows just below them.  nseg and start are filled in by
-// the launcher.
+// [box[q][2], box[q][3]), all z, of npiece boxes; an axis taken as clean
+// (x in variants 0 and 2, y in 0 and 1) must carry no CPML there, have
+// 1/(kappa d) = ik0[e][axis] (e: 0 E, 1 H), pass every domain test, and
+// have an H-clean plane or row just below the box.  nseg and start are
+// filled in by the launcher.
 struct RowsArgs {
     const char *Ein[3], *Hin[3];
     char *Eout[3], *Hout[3];
@@ -205,7 +205,9 @@ constexpr int kRowsAuxAxis = 512, kRowsAuxMaxEdge = 64, kRowsAuxMaxLayers = 64;
 // Waves per block of the row-blocked kernel for a given NZ (one z segment when NZ < 64*16 - 2).
 int rows_waves(int NZ);
 // R: own rows per thread (2 or 3); TFSF is left to launch_tfsf_fix.
-hipError_t launch_step_rows(RowsArgs ra, int R, bool general, hipStream_t st);
+// variant 0: clean boxes (no x or y CPML); 1: x general (x CPML and x
+// boundaries per plane), y clean; 2: y general (per row), x clean.
+hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st);
 
 // box[q] = {ia, ib, ja, jb}; one to four disjoint boxes.
 hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st);

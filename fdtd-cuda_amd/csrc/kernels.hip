@@ -601,6 +601,14 @@ __device__ inline float from_upper_lane(float v) {
 constexpr int kAuxAxis = 512, kAuxIkE0 = 1536, kAuxIkH0 = 1539, kAuxEw = 1542, kAuxPsi = 1548;
 typedef __attribute__((address_space(1))) float gfloat;   // global memory (psi pointers read from LDS)
 
+// Workgroup barrier ordering LDS only: global loads in flight (prefetches)
+// are not waited for, as a full __syncthreads() fence would do.
+__device__ inline void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <bool NT>
 __device__ inline void stn(char *b, unsigned o, float v) {
     float *p = (float *)(b + o);
@@ -625,7 +633,7 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // component the psi terms add in the reference's order.  CPML tables and psi
 // pointers live in LDS (the table image above), which keeps scalar registers
 // for the field pointers.
-template <int W, int R, bool GEN>
+template <int W, int R, bool XG, bool YG>
 __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     __shared__ float sK[2][2][W][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
     __shared__ float sE[2][2][W][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
@@ -659,7 +667,7 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     };
     auto coef = [&](int a, int which, int layer) { return sTf[a * kAuxAxis + 256 + which * 64 + layer]; };
     auto psi = [&](int idx) { return *(gfloat *const *)&sT[kAuxPsi + 2 * idx]; };   // 0-5 set 0, 6-11 set 1, 12-17 psi_E
-    __syncthreads();
+    lds_sync();
 
     // z: per lane
     const bool kA = k >= 0 && k < NZ - 1, kB = k >= 0 && k < NZ, kC = k > 0 && k < NZ - 1;
@@ -677,10 +685,14 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     unsigned ro[R + 2];
 #pragma unroll
     for (int r = 0; r < R + 2; r++) ro[r] = (unsigned)(min(jb + r, NY) + 1) * sz4 + (unsigned)(kc + 1) * 4u;
-    auto jA = [&](int r) { return !GEN || (jb + r >= 0 && jb + r < NY - 1); };
-    auto jB = [&](int r) { return !GEN || (jb + r >= 0 && jb + r < NY); };
-    auto jC = [&](int r) { return !GEN || (jb + r > 0 && jb + r < NY - 1); };
+    auto jA = [&](int r) { return !YG || (jb + r >= 0 && jb + r < NY - 1); };
+    auto jB = [&](int r) { return !YG || (jb + r >= 0 && jb + r < NY); };
+    auto jC = [&](int r) { return !YG || (jb + r > 0 && jb + r < NY - 1); };
     auto jq = [&](int r) { return min(max(jb + r, 0), NY - 1); };
+    auto lyh = [&](int r) { return YG && jA(r) ? hlay(1, jb + r) : -1; };
+    auto lye = [&](int r) { return YG && jC(r) ? elay(1, jb + r) : -1; };
+    auto lxh = [&](int ip) { return XG && ip < NX - 1 ? hlay(0, ip) : -1; };
+    auto lxe = [&](int ip) { return XG && ip > 0 && ip < NX - 1 ? elay(0, ip) : -1; };
 
     const float Chh = ra.Chh, Che = ra.Che, Cee = ra.Cee, Ceh = ra.Ceh;
     const char *bEx = ra.Ein[0], *bEy = ra.Ein[1], *bEz = ra.Ein[2];
@@ -703,21 +715,90 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
             for (int r = 0; r <= R; r++) { sE[b][0][w][r] = x[r]; sE[b][1][w][r] = y[r]; }
         }
     };
+
+    // psi of one plane: z (psi_H hx_z, hy_z rows 0..R; psi_E ex_z, ey_z rows
+    // 1..R), y (hx_y, hz_y; ex_y, ez_y) when YG, x (hy_x, hz_x; ey_x, ez_x)
+    // when XG.  The y and x loads are issued unconditionally (indices clamped
+    // into the slab arrays, values unused where no slab applies) so the wait
+    // counts after them stay exact.
+    struct PSet {
+        float hxz[R + 1], hyz[R + 1], exz[R], eyz[R];
+        float hxy[R + 1], hzy[R + 1], exy[R], ezy[R];
+        float hyx[R + 1], hzx[R + 1], eyx[R], ezx[R];
+    };
+    auto load_psi = [&](int ip, PSet &P) {
+#pragma unroll
+        for (int r = 0; r <= R; r++) { P.hxz[r] = 0.f; P.hyz[r] = 0.f; }
+#pragma unroll
+        for (int r = 0; r < R; r++) { P.exz[r] = 0.f; P.eyz[r] = 0.f; }
+        const bool iA = !XG || ip < NX - 1, iC = !XG || (ip > 0 && ip < NX - 1);
+        if (zh && iA && lzh >= 0) {
+            const gfloat *pa = psi(pp * 6 + 0), *pb = psi(pp * 6 + 2);
+#pragma unroll
+            for (int r = 0; r <= R; r++)
+                if (jA(r)) {
+                    const long q = ((long)ip * NY + jb + r) * Lzh + lzh;
+                    P.hxz[r] = pa[q];
+                    P.hyz[r] = pb[q];
+                }
+        }
+        if (ze && iA && lze >= 0) {
+            const gfloat *pa = psi(12 + 0), *pb = psi(12 + 2);
+#pragma unroll
+            for (int r = 1; r <= R; r++) {
+                const long q = ((long)ip * NY + jb + r) * Lze + lze;
+                if (jC(r)) P.exz[r - 1] = pa[q];
+                if (iC && jA(r)) P.eyz[r - 1] = pb[q];
+            }
+        }
+        if (YG) {
+            const gfloat *h1 = psi(pp * 6 + 1), *h5 = psi(pp * 6 + 5), *e1 = psi(12 + 1), *e4 = psi(12 + 4);
+#pragma unroll
+            for (int r = 0; r <= R; r++) {
+                const long q = ((long)ip * Lyh + max(lyh(r), 0)) * NZ + kq;
+                P.hxy[r] = h1[q];
+                P.hzy[r] = h5[q];
+            }
+#pragma unroll
+            for (int r = 1; r <= R; r++) {
+                const long q = ((long)ip * Lye + max(lye(r), 0)) * NZ + kq;
+                P.exy[r - 1] = e1[q];
+                P.ezy[r - 1] = e4[q];
+            }
+        }
+        if (XG) {
+            const gfloat *h3 = psi(pp * 6 + 3), *h4 = psi(pp * 6 + 4), *e3 = psi(12 + 3), *e5 = psi(12 + 5);
+            const int lh = max(lxh(ip), 0), le = max(lxe(ip), 0);
+#pragma unroll
+            for (int r = 0; r <= R; r++) {
+                const long q = ((long)lh * NY + jq(r)) * NZ + kq;
+                P.hyx[r] = h3[q];
+                P.hzx[r] = h4[q];
+            }
+#pragma unroll
+            for (int r = 1; r <= R; r++) {
+                const long q = ((long)le * NY + jq(r)) * NZ + kq;
+                P.eyx[r - 1] = e3[q];
+                P.ezx[r - 1] = e5[q];
+            }
+        }
+    };
+
     // H of plane ip for rows 0..R from E^n of plane ip (rows 0..R, ex and ez of
-    // row R+1, ex and ey at k+1), ey and ez of plane ip+1 and H^{n-1/2} of
-    // plane ip.  wr: write psi_H of the own cells.
+    // row R+1, ex and ey at k+1), ey and ez of plane ip+1, H^{n-1/2} and psi_H
+    // of plane ip.  wr: write psi_H of the own cells.
     auto hplane = [&](int ip, const float *ex, const float *ey, const float *ez, float exj, float ezj,
                       const float *exk, const float *eyk, const float *ny, const float *nz, const float (*ho)[R + 1],
-                      bool wr, float *hx, float *hy, float *hz) {
-        const bool iA = !GEN || ip < NX - 1;
-        const float ikx_h = GEN ? ik(0, false, ip, NX) : ra.ik0[1][0];
-        const int lxh = GEN && iA ? hlay(0, ip) : -1;
-        const int hin = pp * 6, hout = (1 - pp) * 6;
+                      const PSet &P, bool wr, float *hx, float *hy, float *hz) {
+        const bool iA = !XG || ip < NX - 1;
+        const float ikx_h = XG ? ik(0, false, ip, NX) : ra.ik0[1][0];
+        const int lx = lxh(ip);
+        const int hout = (1 - pp) * 6;
 #pragma unroll
         for (int r = 0; r <= R; r++) {
             const int j = jb + r;
             const bool own = wr && r > 0 && kown && j < jbox;
-            const float iky_h = GEN ? ik(1, false, jq(r), NY) : ra.ik0[1][1];
+            const float iky_h = YG ? ik(1, false, jq(r), NY) : ra.ik0[1][1];
             const float xj = r < R ? ex[r + 1] : exj, zj = r < R ? ez[r + 1] : ezj;
             const float dEy_z = eyk[r] - ey[r], dEz_y = zj - ez[r];
             const float dEz_x = nz[r] - ez[r], dEx_z = exk[r] - ex[r];
@@ -725,151 +806,167 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
             float vx = Chh * ho[0][r] + Che * (dEy_z * ikz_h - dEz_y * iky_h);
             float vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - dEx_z * ikz_h);
             float vz = Chh * ho[2][r] + Che * (dEx_y * iky_h - dEy_x * ikx_h);
+            // per component the psi terms add in the reference's order:
+            // hx +z -y, hy -z +x, hz -x +y (main.cu:834-1037)
             if (zh && iA && jA(r) && lzh >= 0) {
                 const long q = ((long)ip * NY + j) * Lzh + lzh;
-                gfloat *pa = psi(hin + 0), *pb = psi(hin + 2);
-                const float a = bzh * pa[q] + czh * dEy_z;
-                const float b = bzh * pb[q] + czh * dEx_z;
+                const float a = bzh * P.hxz[r] + czh * dEy_z;
+                const float b = bzh * P.hyz[r] + czh * dEx_z;
                 vx += Che * a;
                 vy -= Che * b;
                 if (own) { psi(hout + 0)[q] = a; psi(hout + 2)[q] = b; }
             }
+            const int ly = lyh(r);
             float pyz = 0.f;
-            const int ly = GEN && jA(r) ? hlay(1, j) : -1;
-            if (iA && ly >= 0 && kA) {
+            if (YG && iA && ly >= 0 && kA) {
                 const long q = ((long)ip * Lyh + ly) * NZ + k;
                 const float by = coef(1, 2, ly), cyv = coef(1, 3, ly);
-                const float a = by * psi(hin + 1)[q] + cyv * dEz_y;
-                pyz = by * psi(hin + 5)[q] + cyv * dEx_y;
+                const float a = by * P.hxy[r] + cyv * dEz_y;
+                pyz = by * P.hzy[r] + cyv * dEx_y;
                 vx -= Che * a;
                 if (own) { psi(hout + 1)[q] = a; psi(hout + 5)[q] = pyz; }
             }
-            if (lxh >= 0 && jA(r) && kA) {
-                const long q = ((long)lxh * NY + j) * NZ + k;
-                const float bx = coef(0, 2, lxh), cxv = coef(0, 3, lxh);
-                const float a = bx * psi(hin + 3)[q] + cxv * dEz_x;
-                const float b = bx * psi(hin + 4)[q] + cxv * dEy_x;
+            if (XG && lx >= 0 && jA(r) && kA) {
+                const long q = ((long)lx * NY + j) * NZ + k;
+                const float bx = coef(0, 2, lx), cxv = coef(0, 3, lx);
+                const float a = bx * P.hyx[r] + cxv * dEz_x;
+                const float b = bx * P.hzx[r] + cxv * dEy_x;
                 vy += Che * a;
                 vz -= Che * b;
                 if (own) { psi(hout + 3)[q] = a; psi(hout + 4)[q] = b; }
             }
-            if (iA && ly >= 0 && kA) vz += Che * pyz;
+            if (YG && iA && ly >= 0 && kA) vz += Che * pyz;
             hx[r] = jA(r) && kA ? vx : ho[0][r];
             hy[r] = iA && jB(r) && kA ? vy : ho[1][r];
             hz[r] = iA && jA(r) && kB ? vz : ho[2][r];
         }
     };
 
-    float ex[R + 1], ey[R + 1], ez[R + 1], exj, ezj;     // E^n of plane i
-    float nx[R + 1], ny[R + 1], nz[R + 1];               // E^n of plane i+1
-    float ho[3][R + 1];                                   // H^{n-1/2} of plane i
+    // Register sets: E^n of one plane (rows 0..R, and ex, ez of row R+1),
+    // H^{n-1/2} and psi of one plane.  Step i uses E of planes i and i+1, H and
+    // psi of plane i, and loads E of plane i+2, H and psi of plane i+1 for
+    // later steps; the x march is unrolled six times over three E sets and two
+    // H and psi sets so no set is copied while its loads are in flight.
+    struct ESet { float x[R + 1], y[R + 1], z[R + 1], xj, zj; };
+    struct HSet { float h[3][R + 1]; };
     float hyP[R + 1], hzP[R + 1];                         // H^{n+1/2} of plane i-1
-    auto load_e = [&](int ip, float *x, float *y, float *z) {
+    auto load_e = [&](int ip, ESet &e) {
         const unsigned pl = (unsigned)(ip + 1) * sxy4;
 #pragma unroll
-        for (int r = 0; r <= R; r++) { x[r] = ldb(bEx, pl + ro[r]); y[r] = ldb(bEy, pl + ro[r]); z[r] = ldb(bEz, pl + ro[r]); }
+        for (int r = 0; r <= R; r++) { e.x[r] = ldb(bEx, pl + ro[r]); e.y[r] = ldb(bEy, pl + ro[r]); e.z[r] = ldb(bEz, pl + ro[r]); }
+        e.xj = ldb(bEx, pl + ro[R + 1]);
+        e.zj = ldb(bEz, pl + ro[R + 1]);
     };
-    auto load_h = [&](int ip) {
+    auto load_h = [&](int ip, HSet &h) {
         const unsigned pl = (unsigned)(ip + 1) * sxy4;
 #pragma unroll
-        for (int r = 0; r <= R; r++) { ho[0][r] = ldb(bHx, pl + ro[r]); ho[1][r] = ldb(bHy, pl + ro[r]); ho[2][r] = ldb(bHz, pl + ro[r]); }
+        for (int r = 0; r <= R; r++) { h.h[0][r] = ldb(bHx, pl + ro[r]); h.h[1][r] = ldb(bHy, pl + ro[r]); h.h[2][r] = ldb(bHz, pl + ro[r]); }
     };
+    ESet EA, EB, EC;
+    HSet H0, H1;
+    PSet P0, P1;
     // prologue: H^{n+1/2} of plane i0-1 (not needed at i0 = 0: ey, ez are not
-    // updated there), E^n of planes i0 and i0+1, H of i0
+    // updated there), E^n of planes i0 and i0+1, H and psi of plane i0
 #pragma unroll
     for (int r = 0; r <= R; r++) { hyP[r] = 0.f; hzP[r] = 0.f; }
-    load_e(i0, ex, ey, ez);
+    load_e(i0, EA);
     if (i0 > 0) {
-        float px[R + 1], py[R + 1], pz[R + 1], pxk[R + 1], pyk[R + 1], hx[R + 1], hy[R + 1], hz[R + 1];
-        load_e(i0 - 1, px, py, pz);
-        load_h(i0 - 1);
-        const unsigned pl = (unsigned)i0 * sxy4;
-        const float pxj = ldb(bEx, pl + ro[R + 1]), pzj = ldb(bEz, pl + ro[R + 1]);
-        e_publish(px, py, 1);
-        __syncthreads();
-        e_up(px, py, 1, pxk, pyk);
-        hplane(i0 - 1, px, py, pz, pxj, pzj, pxk, pyk, ey, ez, ho, false, hx, hy, hz);
+        ESet P;
+        float pxk[R + 1], pyk[R + 1], hx[R + 1], hy[R + 1], hz[R + 1];
+        load_e(i0 - 1, P);
+        load_h(i0 - 1, H1);
+        load_psi(i0 - 1, P1);
+        e_publish(P.x, P.y, 1);
+        lds_sync();
+        e_up(P.x, P.y, 1, pxk, pyk);
+        hplane(i0 - 1, P.x, P.y, P.z, P.xj, P.zj, pxk, pyk, EA.y, EA.z, H1.h, P1, false, hx, hy, hz);
 #pragma unroll
         for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
     }
-    e_publish(ex, ey, 0);
-    {
-        const unsigned pc = (unsigned)(i0 + 1) * sxy4;
-        exj = ldb(bEx, pc + ro[R + 1]);
-        ezj = ldb(bEz, pc + ro[R + 1]);
-    }
-    load_e(i0 + 1, nx, ny, nz);
-    load_h(i0);
-    __syncthreads();
-    int buf = 0;
-    for (int i = i0; i < i1; i++) {
+    e_publish(EA.x, EA.y, 0);
+    load_psi(i0, P0);
+    load_e(i0 + 1, EB);
+    load_h(i0, H0);
+    lds_sync();
+
+    // one x step: C = E of plane i, N = E of plane i+1, Fr <- E of plane i+2,
+    // Hc, Pc = H, psi of plane i, Hn, Pn <- H, psi of plane i+1
+    auto step = [&](int i, ESet &C, ESet &N, ESet &Fr, HSet &Hc, HSet &Hn, PSet &Pc, PSet &Pn) {
         const bool more = i + 1 < i1;
+        const int buf = (i - i0) & 1;
         const unsigned pl = (unsigned)(i + 1) * sxy4;
+        // issued on every step (the last one of the chunk reloads plane i, which
+        // is cached) so the wait counts after them are exact
+        load_psi(more ? i + 1 : i, Pn);
+        load_e(more ? i + 2 : i, Fr);
+        load_h(more ? i + 1 : i, Hn);
         float hx[R + 1], hy[R + 1], hz[R + 1], exk[R + 1], eyk[R + 1];
-        e_up(ex, ey, buf, exk, eyk);
-        hplane(i, ex, ey, ez, exj, ezj, exk, eyk, ny, nz, ho, true, hx, hy, hz);
-        if (more) load_h(i + 1);
+        e_up(C.x, C.y, buf, exk, eyk);
+        hplane(i, C.x, C.y, C.z, C.xj, C.zj, exk, eyk, N.y, N.z, Hc.h, Pc, true, hx, hy, hz);
 #pragma unroll
         for (int r = 1; r <= R; r++) {
             if (kown && jb + r < jbox) { stn<true>(oHx, pl + ro[r], hx[r]); stn<true>(oHy, pl + ro[r], hy[r]); stn<true>(oHz, pl + ro[r], hz[r]); }
             if (lane == 63) { sK[buf][0][w][r] = hy[r]; sK[buf][1][w][r] = hx[r]; }
         }
-        e_publish(nx, ny, buf ^ 1);
+        e_publish(N.x, N.y, buf ^ 1);
         float hyk[R + 1], hxk[R + 1];
 #pragma unroll
         for (int r = 1; r <= R; r++) { hyk[r] = from_lower_lane(hy[r]); hxk[r] = from_lower_lane(hx[r]); }
-        __syncthreads();
+        lds_sync();
         if (first_lane) {
 #pragma unroll
             for (int r = 1; r <= R; r++) { hyk[r] = sK[buf][0][w - 1][r]; hxk[r] = sK[buf][1][w - 1][r]; }
         }
-        const bool iA = !GEN || i < NX - 1, iC = !GEN || (i > 0 && i < NX - 1);
-        const float ikx_e = GEN ? ik(0, true, i, NX) : ra.ik0[0][0];
-        const int lxe = GEN && iC ? elay(0, i) : -1;
+        const bool iA = !XG || i < NX - 1, iC = !XG || (i > 0 && i < NX - 1);
+        const float ikx_e = XG ? ik(0, true, i, NX) : ra.ik0[0][0];
+        const int lx = lxe(i);
 #pragma unroll
         for (int r = 1; r <= R; r++) {
             const int j = jb + r;
             const bool own = kown && j < jbox;
-            const float iky_e = GEN ? ik(1, true, jq(r), NY) : ra.ik0[0][1];
+            const float iky_e = YG ? ik(1, true, jq(r), NY) : ra.ik0[0][1];
             const float dHz_y = hz[r] - hz[r - 1], dHy_z = hy[r] - hyk[r];
             const float dHx_z = hx[r] - hxk[r], dHz_x = hz[r] - hzP[r];
             const float dHy_x = hy[r] - hyP[r], dHx_y = hx[r] - hx[r - 1];
-            float vx = Cee * ex[r] + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
-            float vy = Cee * ey[r] + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
-            float vz = Cee * ez[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
+            float vx = Cee * C.x[r] + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
+            float vy = Cee * C.y[r] + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
+            float vz = Cee * C.z[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
             const bool ux = iA && jC(r) && kC, uy = iC && jA(r) && kC, uz = iC && jC(r) && kA;
+            // ex -z +y, ey +z -x, ez -y +x (main.cu:1172-1906)
             if (ze && iA && lze >= 0) {
                 const long q = ((long)i * NY + j) * Lze + lze;
-                if (jC(r)) { gfloat *pe = psi(12 + 0); const float a = bze * pe[q] + cze * dHy_z; vx -= Ceh * a; if (own) pe[q] = a; }
-                if (iC && jA(r)) { gfloat *pe = psi(12 + 2); const float a = bze * pe[q] + cze * dHx_z; vy += Ceh * a; if (own) pe[q] = a; }
+                if (jC(r)) { const float a = bze * Pc.exz[r - 1] + cze * dHy_z; vx -= Ceh * a; if (own) psi(12 + 0)[q] = a; }
+                if (iC && jA(r)) { const float a = bze * Pc.eyz[r - 1] + cze * dHx_z; vy += Ceh * a; if (own) psi(12 + 2)[q] = a; }
             }
-            const int ly = GEN && jC(r) ? elay(1, j) : -1;
-            if (iA && ly >= 0) {
+            const int ly = lye(r);
+            if (YG && iA && ly >= 0) {
                 const long q = ((long)i * Lye + ly) * NZ + k;
                 const float by = coef(1, 0, ly), cyv = coef(1, 1, ly);
-                if (kC) { gfloat *pe = psi(12 + 1); const float a = by * pe[q] + cyv * dHz_y; vx += Ceh * a; if (own) pe[q] = a; }
-                if (iC && kA) { gfloat *pe = psi(12 + 4); const float a = by * pe[q] + cyv * dHx_y; vz -= Ceh * a; if (own) pe[q] = a; }
+                if (kC) { const float a = by * Pc.exy[r - 1] + cyv * dHz_y; vx += Ceh * a; if (own) psi(12 + 1)[q] = a; }
+                if (iC && kA) { const float a = by * Pc.ezy[r - 1] + cyv * dHx_y; vz -= Ceh * a; if (own) psi(12 + 4)[q] = a; }
             }
-            if (lxe >= 0) {
-                const long q = ((long)lxe * NY + j) * NZ + k;
-                const float bx = coef(0, 0, lxe), cxv = coef(0, 1, lxe);
-                if (jA(r) && kC) { gfloat *pe = psi(12 + 3); const float a = bx * pe[q] + cxv * dHz_x; vy -= Ceh * a; if (own) pe[q] = a; }
-                if (jC(r) && kA) { gfloat *pe = psi(12 + 5); const float a = bx * pe[q] + cxv * dHy_x; vz += Ceh * a; if (own) pe[q] = a; }
+            if (XG && lx >= 0) {
+                const long q = ((long)lx * NY + j) * NZ + k;
+                const float bx = coef(0, 0, lx), cxv = coef(0, 1, lx);
+                if (jA(r) && kC) { const float a = bx * Pc.eyx[r - 1] + cxv * dHz_x; vy -= Ceh * a; if (own) psi(12 + 3)[q] = a; }
+                if (jC(r) && kA) { const float a = bx * Pc.ezx[r - 1] + cxv * dHy_x; vz += Ceh * a; if (own) psi(12 + 5)[q] = a; }
             }
             if (own) {
-                stn<true>(oEx, pl + ro[r], ux ? vx : ex[r]);
-                stn<true>(oEy, pl + ro[r], uy ? vy : ey[r]);
-                stn<true>(oEz, pl + ro[r], uz ? vz : ez[r]);
+                stn<true>(oEx, pl + ro[r], ux ? vx : C.x[r]);
+                stn<true>(oEy, pl + ro[r], uy ? vy : C.y[r]);
+                stn<true>(oEz, pl + ro[r], uz ? vz : C.z[r]);
             }
         }
 #pragma unroll
-        for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; ex[r] = nx[r]; ey[r] = ny[r]; ez[r] = nz[r]; }
-        if (more) {
-            const unsigned pn = pl + sxy4;
-            exj = ldb(bEx, pn + ro[R + 1]); ezj = ldb(bEz, pn + ro[R + 1]);
-            load_e(i + 2, nx, ny, nz);
-        }
-        buf ^= 1;
+        for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
+    };
+    for (int i = i0;;) {
+        step(i, EA, EB, EC, H0, H1, P0, P1); if (++i >= i1) break;
+        step(i, EB, EC, EA, H1, H0, P1, P0); if (++i >= i1) break;
+        step(i, EC, EA, EB, H0, H1, P0, P1); if (++i >= i1) break;
+        step(i, EA, EB, EC, H1, H0, P1, P0); if (++i >= i1) break;
+        step(i, EB, EC, EA, H0, H1, P0, P1); if (++i >= i1) break;
+        step(i, EC, EA, EB, H1, H0, P1, P0); if (++i >= i1) break;
     }
 }
 
@@ -1725,27 +1822,27 @@ hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, const int (*b
 }
 
 
-template <int W, bool GEN>
+template <int W, bool XG, bool YG>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
     switch (R) {
-        case 2: hipLaunchKernelGGL((k_step_rows<W, 2, GEN>), grid, dim3(64, W), 0, st, ra); break;
-        case 3: hipLaunchKernelGGL((k_step_rows<W, 3, GEN>), grid, dim3(64, W), 0, st, ra); break;
+        case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG>), grid, dim3(64, W), 0, st, ra); break;
+        case 3: hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG>), grid, dim3(64, W), 0, st, ra); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <bool GEN>
+template <bool XG, bool YG>
 static hipError_t launch_rows_g(const RowsArgs &ra, int R, int W, dim3 grid, hipStream_t st) {
     switch (W) {
-        case 2: return launch_rows_w<2, GEN>(ra, R, grid, st);
-        case 3: return launch_rows_w<3, GEN>(ra, R, grid, st);
-        case 4: return launch_rows_w<4, GEN>(ra, R, grid, st);
-        case 5: return launch_rows_w<5, GEN>(ra, R, grid, st);
-        case 6: return launch_rows_w<6, GEN>(ra, R, grid, st);
-        case 8: return launch_rows_w<8, GEN>(ra, R, grid, st);
-        case 12: return launch_rows_w<12, GEN>(ra, R, grid, st);
-        default: return launch_rows_w<16, GEN>(ra, R, grid, st);
+        case 2: return launch_rows_w<2, XG, YG>(ra, R, grid, st);
+        case 3: return launch_rows_w<3, XG, YG>(ra, R, grid, st);
+        case 4: return launch_rows_w<4, XG, YG>(ra, R, grid, st);
+        case 5: return launch_rows_w<5, XG, YG>(ra, R, grid, st);
+        case 6: return launch_rows_w<6, XG, YG>(ra, R, grid, st);
+        case 8: return launch_rows_w<8, XG, YG>(ra, R, grid, st);
+        case 12: return launch_rows_w<12, XG, YG>(ra, R, grid, st);
+        default: return launch_rows_w<16, XG, YG>(ra, R, grid, st);
     }
 }
 
@@ -1756,22 +1853,28 @@ int rows_waves(int NZ) {
     return 16;
 }
 
-hipError_t launch_step_rows(RowsArgs ra, int R, bool general, hipStream_t st) {
-    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4) return hipErrorInvalidValue;
+hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
+    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 2)
+        return hipErrorInvalidValue;
     const int W = rows_waves(ra.NZ);
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
     long total = 0;
     for (int q = 0; q < ra.npiece; q++) {
         const int *b = ra.box[q];
         if (b[0] < 0 || b[1] > ra.NX || b[0] >= b[1] || b[2] < 0 || b[3] > ra.NY || b[2] >= b[3]) return hipErrorInvalidValue;
-        if (!general && (b[0] < 1 || b[1] > ra.NX - 1 || b[2] < 1 || b[3] > ra.NY - 1)) return hipErrorInvalidValue;
+        if (variant != 1 && (b[0] < 1 || b[1] > ra.NX - 1)) return hipErrorInvalidValue;   // x must be clean
+        if (variant != 2 && (b[2] < 1 || b[3] > ra.NY - 1)) return hipErrorInvalidValue;   // y must be clean
         ra.start[q] = (int)total;
         total += (long)((b[3] - b[2] + R - 1) / R) * ra.nseg * ((b[1] - b[0] + ra.ichunk - 1) / ra.ichunk);
     }
     if (total > 0x7fffffff) return hipErrorInvalidValue;
     ra.start[ra.npiece] = (int)total;
     const dim3 grid((unsigned)total);
-    return general ? launch_rows_g<true>(ra, R, W, grid, st) : launch_rows_g<false>(ra, R, W, grid, st);
+    switch (variant) {
+        case 0: return launch_rows_g<false, false>(ra, R, W, grid, st);
+        case 1: return launch_rows_g<true, false>(ra, R, W, grid, st);
+        default: return launch_rows_g<false, true>(ra, R, W, grid, st);
+    }
 }
 
 hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, hipStream_t st) {
