@@ -175,7 +175,9 @@ def main():
             "grid": [info.NX, info.NY, info.NZ],
             "dispersive_cells": list(info.n_dispersive[:]),
             "tile": {"tj": int(os.environ.get("FDTD_TJ", 8)), "ichunk": int(os.environ.get("FDTD_ICHUNK", 32)),
-                     "rows": int(os.environ.get("FDTD_ROWS", 2)), "cchunk": int(os.environ.get("FDTD_CCHUNK", 8))},
+                     "rows": int(os.environ.get("FDTD_ROWS", 3)), "cchunk": int(os.environ.get("FDTD_CCHUNK", 16)),
+                     "schunk": int(os.environ.get("FDTD_SCHUNK", 8)),
+                     "side_stream": int(os.environ.get("FDTD_CONCURRENT", 1)), "xcd_order": int(os.environ.get("FDTD_XCD", 0))},
         },
         "roofline": {
             "bound": "hbm",

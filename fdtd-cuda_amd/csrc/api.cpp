@@ -90,11 +90,13 @@ struct fdtd_sim {
     int nxbox = 0, nybox = 0, ncorner = 0;
     float ik0[2][2] = {};           // interior 1/(kappa d): [E, H][x, y]
     unsigned *aux_d = nullptr;      // table image of the row-blocked kernel
-    bool concurrent = false;        // general pieces on a second stream beside the clean box (FDTD_CONCURRENT=1)
+    bool concurrent = true;         // side slabs on a second stream beside the clean box (FDTD_CONCURRENT=0 to serialise)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
-    int rows = 2, cchunk = 8;       // own rows per thread and x chunk of the row-blocked kernel
+    int rows = 3, cchunk = 16;      // own rows per thread and x chunk of the row-blocked kernel (clean box)
+    int schunk = 8;                 // x chunk of the side slabs
+    int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -410,6 +412,8 @@ int plan_rows(fdtd_sim *s) {
         if (std::string(v) == "general") return FDTD_OK;
     if (const char *v = std::getenv("FDTD_ROWS")) s->rows = std::min(3, std::max(2, std::atoi(v)));
     if (const char *v = std::getenv("FDTD_CCHUNK")) s->cchunk = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_SCHUNK")) s->schunk = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_XCD")) s->xcd = std::atoi(v) != 0;
     if ((double)s->fsz * 4.0 >= 4294967296.0) return FDTD_OK;
     std::vector<uint32_t> img(kRowsAuxWords, 0u);
     auto put = [&](int q, float v) { std::memcpy(&img[q], &v, 4); };
@@ -514,6 +518,7 @@ RowsArgs make_rows(fdtd_sim *s) {
     for (int a = 0; a < 3; a++) { r.Nc[a] = S.cp[a].Nc; r.F[a] = S.cp[a].F; }
     r.pp = s->pp;
     r.ichunk = s->cchunk;
+    r.xcd = s->xcd;
     for (int e = 0; e < 2; e++)
         for (int a = 0; a < 2; a++) r.ik0[e][a] = s->ik0[e][a];
     return r;
@@ -533,17 +538,19 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, hipEvent_t ev_clean0, hipE
     }
     if (s->nxbox) {
         RowsArgs rx = r;
+        rx.ichunk = s->schunk;
         rx.npiece = s->nxbox;
         std::memcpy(rx.box, s->xbox, sizeof(s->xbox));
         HIPCHK(s, launch_step_rows(rx, 2, 1, side));
     }
     if (s->nybox) {
         RowsArgs ry = r;
+        ry.ichunk = s->schunk;
         ry.npiece = s->nybox;
         std::memcpy(ry.box, s->ybox, sizeof(s->ybox));
         HIPCHK(s, launch_step_rows(ry, 2, 2, side));
     }
-    if (s->ncorner) HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, s->cbox4, s->ncorner, side));
+    if (s->ncorner) HIPCHK(s, launch_fused_step(d, s->tj, 2, s->cbox4, s->ncorner, side));   // thin columns: short x chunks
     if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));

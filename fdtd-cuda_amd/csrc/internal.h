@@ -194,6 +194,7 @@ struct RowsArgs {
     int Nc[3], F[3];
     int pp;                             // psi_H input set
     int ichunk;
+    int xcd;                            // 1: XCD-contiguous block order (j-neighbour tiles share an L2)
     int npiece;
     int box[4][4];
     int nseg;

@@ -641,7 +641,7 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     const int NX = ra.NX, NY = ra.NY, NZ = ra.NZ;
     for (int q = threadIdx.y * 64 + threadIdx.x; q < kAuxPsi + 36; q += 64 * W) sT[q] = ra.aux[q];
     const float *sTf = (const float *)sT;
-    int L = blockIdx.x, pc = 0;
+    int L = ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : blockIdx.x, pc = 0;
     while (pc + 1 < ra.npiece && L >= ra.start[pc + 1]) pc++;
     L -= ra.start[pc];
     const int ia = ra.box[pc][0], ib = ra.box[pc][1], ja = ra.box[pc][2], jbox = ra.box[pc][3];
