@@ -842,11 +842,12 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
         }
     };
 
-    // Register sets: E^n of one plane (rows 0..R, and ex, ez of row R+1),
-    // H^{n-1/2} and psi of one plane.  Step i uses E of planes i and i+1, H and
-    // psi of plane i, and loads E of plane i+2, H and psi of plane i+1 for
-    // later steps; the x march is unrolled six times over three E sets and two
-    // H and psi sets so no set is copied while its loads are in flight.
+    // Register sets: E^n of one plane (rows 0..R, and ex, ez of row R+1) and
+    // H^{n-1/2} of one plane.  Step i uses E of planes i and i+1 and H of
+    // plane i, and loads E of plane i+2 and H of plane i+1 for later steps;
+    // the x march is unrolled six times over three E sets and two H sets so no
+    // set is copied while its loads are in flight.  psi of plane i is loaded at
+    // the top of step i (one set).
     struct ESet { float x[R + 1], y[R + 1], z[R + 1], xj, zj; };
     struct HSet { float h[3][R + 1]; };
     float hyP[R + 1], hzP[R + 1];                         // H^{n+1/2} of plane i-1
@@ -864,7 +865,7 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     };
     ESet EA, EB, EC;
     HSet H0, H1;
-    PSet P0, P1;
+    PSet P0, P1;   // P1 only for the prologue plane
     // prologue: H^{n+1/2} of plane i0-1 (not needed at i0 = 0: ey, ez are not
     // updated there), E^n of planes i0 and i0+1, H and psi of plane i0
 #pragma unroll
@@ -884,20 +885,21 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
         for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
     }
     e_publish(EA.x, EA.y, 0);
-    load_psi(i0, P0);
     load_e(i0 + 1, EB);
     load_h(i0, H0);
     lds_sync();
 
     // one x step: C = E of plane i, N = E of plane i+1, Fr <- E of plane i+2,
-    // Hc, Pc = H, psi of plane i, Hn, Pn <- H, psi of plane i+1
-    auto step = [&](int i, ESet &C, ESet &N, ESet &Fr, HSet &Hc, HSet &Hn, PSet &Pc, PSet &Pn) {
+    // Hc = H of plane i, Hn <- H of plane i+1, Pc <- psi of plane i
+    auto step = [&](int i, ESet &C, ESet &N, ESet &Fr, HSet &Hc, HSet &Hn, PSet &Pc) {
         const bool more = i + 1 < i1;
         const int buf = (i - i0) & 1;
         const unsigned pl = (unsigned)(i + 1) * sxy4;
-        // issued on every step (the last one of the chunk reloads plane i, which
-        // is cached) so the wait counts after them are exact
-        load_psi(more ? i + 1 : i, Pn);
+        // psi of this plane first (waited for after the field loads issued
+        // behind it, not with them), then the fields of the next step; all
+        // loads are issued on every step (the last one of the chunk reloads
+        // plane i, which is cached) so the wait counts after them are exact
+        load_psi(i, Pc);
         load_e(more ? i + 2 : i, Fr);
         load_h(more ? i + 1 : i, Hn);
         float hx[R + 1], hy[R + 1], hz[R + 1], exk[R + 1], eyk[R + 1];
@@ -961,12 +963,12 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
         for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
     };
     for (int i = i0;;) {
-        step(i, EA, EB, EC, H0, H1, P0, P1); if (++i >= i1) break;
-        step(i, EB, EC, EA, H1, H0, P1, P0); if (++i >= i1) break;
-        step(i, EC, EA, EB, H0, H1, P0, P1); if (++i >= i1) break;
-        step(i, EA, EB, EC, H1, H0, P1, P0); if (++i >= i1) break;
-        step(i, EB, EC, EA, H0, H1, P0, P1); if (++i >= i1) break;
-        step(i, EC, EA, EB, H1, H0, P1, P0); if (++i >= i1) break;
+        step(i, EA, EB, EC, H0, H1, P0); if (++i >= i1) break;
+        step(i, EB, EC, EA, H1, H0, P0); if (++i >= i1) break;
+        step(i, EC, EA, EB, H0, H1, P0); if (++i >= i1) break;
+        step(i, EA, EB, EC, H1, H0, P0); if (++i >= i1) break;
+        step(i, EB, EC, EA, H0, H1, P0); if (++i >= i1) break;
+        step(i, EC, EA, EB, H1, H0, P0); if (++i >= i1) break;
     }
 }
 
