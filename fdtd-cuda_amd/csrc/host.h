@@ -64,6 +64,8 @@ bool build_setup(const fdtd_params &p, Setup &s, std::string &err);
 void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]);
 // Same rule around an arbitrary centre (cells).
 void sphere_masks_at(const Setup &s, const double centre[3], double radius, std::vector<uint8_t> m[4]);
+void rod_masks_at(const Setup &s, const double centre[3], int axis, double length, double radius,
+                  std::vector<uint8_t> m[4]);
 double source_pulse(const Setup &s, int t);
 
 }  // namespace fdtd

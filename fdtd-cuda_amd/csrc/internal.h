@@ -236,6 +236,36 @@ hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t 
 // Hydrodynamic model: velocity update, then per component (x, y, z) continuity and E update.
 hipError_t launch_hydro(const DispDev &d, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Batched launches: one launch runs the same kernel over many independent
+// simulations.  Per-simulation arguments sit in device memory (one item per
+// simulation, read with scalar loads); map[b] names the item of block b and
+// the block's index within that simulation's cell set: item << 22 | local.
+// ---------------------------------------------------------------------------
+constexpr int kBatchLocalBits = 22;
+struct FusedItem {
+    StepDev s;
+    Pieces pc;
+    int ichunk, nkt;
+};
+struct SampleItem {
+    Geom g;
+    FaceDev fd;
+    const float *E[3], *H[3];
+    float *X;                       // sample ring [K][mtot]
+    long mtot;
+};
+// dyn = {t, slot}: the step index of the first step of this launch sequence
+// and the ring row of its first sample; k_batch_advance moves them on.
+hipError_t launch_fused_batch(const FusedItem *items, const unsigned *map, int nblocks, int tj, hipStream_t st);
+hipError_t launch_line_batch(const LineDev *items, int n, const int *dyn, int step_off, hipStream_t st);
+hipError_t launch_disp_batch(const DispDev *items, const unsigned *map, int nblocks, hipStream_t st);
+hipError_t launch_sample_batch(const SampleItem *items, const unsigned *map, int nblocks, const int *dyn, int sidx,
+                               hipStream_t st);
+hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st);
+// Blocks of one simulation in a batched launch of the general kernel (tile tj x ichunk over the whole grid).
+long fused_batch_blocks(const Geom &g, int tj, int ichunk, Pieces *pc, int *nkt);
+
 // Adjoint: inject dJ/dE^{n+1}(r0), transposed dispersive update, fused transposed step.
 hipError_t launch_adj_inject(float *const E[3], long p0, float ax, float ay, float az, hipStream_t st);
 hipError_t launch_adj_disp(const AdjDispDev &a, hipStream_t st);

@@ -323,14 +323,16 @@ __device__ inline float from_lower_lane(float v) {
 // Blocks are renumbered so each XCD (blocks are dealt round-robin over 8)
 // receives a contiguous run of tiles: tiles adjacent in j, whose halo rows
 // overlap, then share that XCD's L2.
+//
+// The body takes the block's index within its own cell set (L), so batched
+// launches run the same code over many simulations.
 template <int TJ>
-__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, int nkt, Pieces pc) {
+__device__ __forceinline__ void fused_body(const StepDev &s, int ichunk, int nkt, const Pieces &pc, int L) {
     __shared__ float sH[2][2][TJ][64];   // hz, hx of the plane for the j-1 neighbour
     const Geom &g = s.g;
     const int NX = g.NX, NY = g.NY, NZ = g.NZ;
     int jt, kt, ch, ia, ib, ja, jb;
     {
-        int L = xcd_order(blockIdx.x, pc.start[pc.n]);
         int q = 0;
         while (q + 1 < pc.n && L >= pc.start[q + 1]) q++;
         L -= pc.start[q];
@@ -580,6 +582,11 @@ __global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, i
         hxo = f_hx; hyo = f_hy; hzo = f_hz;
         buf ^= 1;
     }
+}
+
+template <int TJ>
+__global__ __launch_bounds__(64 * TJ) void k_fused_step(StepDev s, int ichunk, int nkt, Pieces pc) {
+    fused_body<TJ>(s, ichunk, nkt, pc, xcd_order(blockIdx.x, pc.start[pc.n]));
 }
 
 // ----------------------------------------------------------------- row-blocked step kernel
@@ -1073,9 +1080,8 @@ __global__ void k_tfsf_fix(StepDev s) {
 }
 
 // One thread per sample column of the E half (H taken at the same cell).
-__global__ void k_dft_sample_split(StepDev s) {
-    const FaceDev &f = s.fd;
-    long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sample_body(const Geom &g, const FaceDev &f, const float *const *E, const float *const *H,
+                                            float *row, long col) {
     if (col >= f.mE) return;
     int q = 0;
     while (q < 11 && col >= f.off[q + 1]) q++;
@@ -1089,12 +1095,16 @@ __global__ void k_dft_sample_split(StepDev s) {
     if (axis == 0) { i = f.plane[q]; j = a; k = b; }
     else if (axis == 1) { j = f.plane[q]; i = a; k = b; }
     else { k = f.plane[q]; i = a; j = b; }
-    const long p = gidx(s.g, i, j, k);
+    const long p = gidx(g, i, j, k);
     // E_a/E_b and H_a/H_b components per normal axis: x (y,z), y (z,x), z (x,y)
     const int ca = (axis + 1) % 3, cb = (axis + 2) % 3;
     const int c = comp ? cb : ca;
-    s.sample_row[col] = s.Ein[c][p];
-    s.sample_row[f.mE_pad + col] = s.Hin[c][p];
+    row[col] = E[c][p];
+    row[f.mE_pad + col] = H[c][p];
+}
+
+__global__ void k_dft_sample_split(StepDev s) {
+    sample_body(s.g, s.fd, s.Ein, s.Hin, s.sample_row, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ----------------------------------------------------------------- dispersive box (main.cu:1178-1219)
@@ -1106,9 +1116,8 @@ __global__ void k_dft_sample_split(StepDev s) {
 // the Drude pole k1+k2-1 = 0 exactly, so the secularly growing P drops out and
 // the FP32 update avoids the near-total cancellation of the two-level form.
 
-__global__ void k_dispersive(DispDev d) {
+__device__ __forceinline__ void disp_body(const DispDev &d, long idx) {
     const long nb = (long)d.n[0] * d.n[1] * d.n[2];
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nb) return;
     const int bk = (int)(idx % d.n[2]);
     const int bj = (int)((idx / d.n[2]) % d.n[1]);
@@ -1156,6 +1165,8 @@ __global__ void k_dispersive(DispDev d) {
         d.E[c][p] = e;
     }
 }
+
+__global__ void k_dispersive(DispDev d) { disp_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // ----------------------------------------------------------------- hydrodynamic model (main.cu:1916-2276)
 
@@ -1316,7 +1327,7 @@ __global__ void k_disp_sync(DispDev d, int from_field) {
 
 // One block: h_inc (main.cu:2328-2336), e_inc (main.cu:2280-2326) into the
 // other buffer, first-order Mur ends and the additive source (main.cu:2831-2859).
-__global__ __launch_bounds__(1024) void k_line(LineDev l) {
+__device__ __forceinline__ void line_body(const LineDev &l, int t) {
     const int L = l.L;
     for (int i = threadIdx.x; i < L - 1; i += blockDim.x) l.h[i] = l.h[i] - l.ch * (l.e_old[i + 1] - l.e_old[i]);
     __syncthreads();
@@ -1330,16 +1341,18 @@ __global__ __launch_bounds__(1024) void k_line(LineDev l) {
         l.mur[1] = l.e_new[L - 2];
         double pulse;
         if (l.static_field) {
-            pulse = (l.t < 1000) ? l.amp * l.t / 1000.0 : l.amp;  // main.cu:2852-2857
+            pulse = (t < 1000) ? l.amp * t / 1000.0 : l.amp;  // main.cu:2852-2857
         } else {
             // powf on a float-converted argument, as the reference (main.cu:2848)
-            const float b = (float)(((double)l.t - l.delay) / l.width);
+            const float b = (float)(((double)t - l.delay) / l.width);
             const float b2 = b * b;
-            pulse = l.amp * exp(-(double)b2 / 2.0) * sin(l.w0dt * (double)l.t);
+            pulse = l.amp * exp(-(double)b2 / 2.0) * sin(l.w0dt * (double)t);
         }
         l.e_new[l.src] += pulse;
     }
 }
+
+__global__ __launch_bounds__(1024) void k_line(LineDev l) { line_body(l, l.t); }
 
 // ----------------------------------------------------------------- DFT contraction
 
@@ -1823,6 +1836,106 @@ hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, const int (*b
     return hipGetLastError();
 }
 
+
+// ----------------------------------------------------------------- batched launches
+
+// Items are read through the constant address space: every block reads its
+// item at a uniform address, so the fields land in scalar registers exactly
+// as kernel arguments do.
+#define CONST_AS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const T &batch_item(const T *items, unsigned idx) {
+    const CONST_AS T *c = (const CONST_AS T *)items;
+    return *(const T *)(c + idx);
+}
+__device__ __forceinline__ unsigned batch_map(const unsigned *map) {
+    const CONST_AS unsigned *c = (const CONST_AS unsigned *)map;
+    return c[blockIdx.x];
+}
+
+template <int TJ>
+__global__ __launch_bounds__(64 * TJ) void k_fused_batch(const FusedItem *items, const unsigned *map) {
+    const unsigned m = batch_map(map);
+    const FusedItem &it = batch_item(items, m >> kBatchLocalBits);
+    fused_body<TJ>(it.s, it.ichunk, it.nkt, it.pc, (int)(m & ((1u << kBatchLocalBits) - 1)));
+}
+
+__global__ __launch_bounds__(1024) void k_line_batch(const LineDev *items, const int *dyn, int step_off) {
+    const CONST_AS int *d = (const CONST_AS int *)dyn;
+    line_body(batch_item(items, blockIdx.x), d[0] + step_off);
+}
+
+__global__ __launch_bounds__(256) void k_disp_batch(const DispDev *items, const unsigned *map) {
+    const unsigned m = batch_map(map);
+    const DispDev &d = batch_item(items, m >> kBatchLocalBits);
+    disp_body(d, (long)(m & ((1u << kBatchLocalBits) - 1)) * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_sample_batch(const SampleItem *items, const unsigned *map, const int *dyn, int sidx) {
+    const unsigned m = batch_map(map);
+    const SampleItem &it = batch_item(items, m >> kBatchLocalBits);
+    const CONST_AS int *d = (const CONST_AS int *)dyn;
+    float *row = it.X + (size_t)(d[1] + sidx) * it.mtot;
+    sample_body(it.g, it.fd, it.E, it.H, row, (long)(m & ((1u << kBatchLocalBits) - 1)) * 256 + threadIdx.x);
+}
+
+__global__ void k_batch_advance(int *dyn, int dt, int dslot) {
+    if (threadIdx.x == 0) {
+        dyn[0] += dt;
+        dyn[1] += dslot;
+    }
+}
+
+long fused_batch_blocks(const Geom &g, int tj, int ichunk, Pieces *pc, int *nkt) {
+    if (tj != 4 && tj != 8 && tj != 16) return -1;
+    if (ichunk < 1) return -1;
+    Pieces q{};
+    q.n = 1;
+    q.ia[0] = 0; q.ib[0] = g.NX; q.ja[0] = 0; q.jb[0] = g.NY;
+    q.njt[0] = (g.NY + tj - 2) / (tj - 1);
+    q.start[0] = 0;
+    const int nk = (g.NZ + 62) / 63;
+    const long total = (long)q.njt[0] * nk * ((g.NX + ichunk - 1) / ichunk);
+    q.start[1] = (int)total;
+    if (pc) *pc = q;
+    if (nkt) *nkt = nk;
+    return total;
+}
+
+hipError_t launch_fused_batch(const FusedItem *items, const unsigned *map, int nblocks, int tj, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    switch (tj) {
+        case 4: hipLaunchKernelGGL(k_fused_batch<4>, dim3(nblocks), dim3(64, 4), 0, st, items, map); break;
+        case 8: hipLaunchKernelGGL(k_fused_batch<8>, dim3(nblocks), dim3(64, 8), 0, st, items, map); break;
+        case 16: hipLaunchKernelGGL(k_fused_batch<16>, dim3(nblocks), dim3(64, 16), 0, st, items, map); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_line_batch(const LineDev *items, int n, const int *dyn, int step_off, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_line_batch, dim3(n), dim3(1024), 0, st, items, dyn, step_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_disp_batch(const DispDev *items, const unsigned *map, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_disp_batch, dim3(nblocks), dim3(256), 0, st, items, map);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_batch(const SampleItem *items, const unsigned *map, int nblocks, const int *dyn, int sidx,
+                               hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sample_batch, dim3(nblocks), dim3(256), 0, st, items, map, dyn, sidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, st, dyn, dt, dslot);
+    return hipGetLastError();
+}
 
 template <int W, bool XG, bool YG>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {

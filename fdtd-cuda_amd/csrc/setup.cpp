@@ -113,6 +113,41 @@ void sphere_masks_at(const Setup &s, const double centre[3], double radius, std:
             }
 }
 
+// Nanorod: cylinder with hemispherical caps, total length `length`, along
+// `axis`.  A Yee position is inside when its distance to the axis segment
+// between the two cap centres is below radius/dx (cells; not truncated, so
+// continuous diameters stay distinct).  No reference counterpart: the
+// reference paints only spheres (setup.cu:219-248); the test per component
+// position is the same.
+void rod_masks_at(const Setup &s, const double centre[3], int axis, double length, double radius,
+                  std::vector<uint8_t> m[4]) {
+    const int N[3] = {s.N[0], s.N[1], s.N[2]};
+    const double r = radius / s.P.dx;
+    const double half = std::max(0.0, 0.5 * length / s.P.dx - r);   // half-length of the cylindrical part
+    const uint8_t mat = (uint8_t)s.P.material;
+    auto in = [&](double x, double y, double z) {
+        double d[3] = {x - centre[0], y - centre[1], z - centre[2]};
+        const double t = std::min(half, std::max(-half, d[axis]));
+        d[axis] -= t;
+        return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) < r;
+    };
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        const double ext = r + (a == axis ? half : 0.0) + 2.0;
+        lo[a] = std::max(0, (int)std::floor(centre[a] - ext));
+        hi[a] = std::min(N[a], (int)std::ceil(centre[a] + ext) + 1);
+    }
+    for (int i = lo[0]; i < hi[0]; i++)
+        for (int j = lo[1]; j < hi[1]; j++)
+            for (int k = lo[2]; k < hi[2]; k++) {
+                const size_t id = ((size_t)i * N[1] + j) * N[2] + k;
+                if (in(i, j, k)) m[0][id] = mat;
+                if (in(i + 0.5, j, k)) m[1][id] = mat;
+                if (in(i, j + 0.5, k)) m[2][id] = mat;
+                if (in(i, j, k + 0.5)) m[3][id] = mat;
+            }
+}
+
 // The reference centres its sphere on the cell NCELL/2 of the augmented grid.
 void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]) {
     const double c[3] = {(double)(s.N[0] / 2), (double)(s.N[1] / 2), (double)(s.N[2] / 2)};

@@ -1,0 +1,148 @@
+// The simulation handle behind the C ABI and the host helpers that build
+// kernel arguments from it, shared by the single-simulation API (api.cpp) and
+// batched sweeps (batch.cpp).  Not part of the public ABI.
+#ifndef FDTD_SIM_H_
+#define FDTD_SIM_H_
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fdtd.h"
+#include "host.h"
+#include "internal.h"
+
+struct fdtd_sim {
+    fdtd::Setup S;
+    std::string err = "no error";
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    Geom g{};
+    // fields: [parity][component]
+    float *E[2][3] = {}, *H[2][3] = {};
+    int p = 0;                      // current field parity
+    float *psiH[2][6] = {}, *psiE[6] = {};
+    int pp = 0;                     // current psi_H parity
+    // CPML tables (float)
+    float *ik_e[3] = {}, *ik_h[3] = {}, *be[3] = {}, *ce[3] = {}, *bh[3] = {}, *ch[3] = {};
+    // 1D line
+    double *line_e[2] = {}, *line_h = nullptr, *mur = nullptr;
+    int le = 0;
+    // dispersive box
+    int box_lo[3] = {0, 0, 0}, box_n[3] = {0, 0, 0};
+    long nbox = 0;
+    uint8_t *disp_id[3] = {};
+    float *disp_st[3][8] = {};
+    float *h_vnew[3] = {}, *h_nd[3] = {}, *h_ndp[3] = {};
+    int n_disp[3] = {0, 0, 0};
+    // DFT
+    FaceDev fd{};
+    long mtot = 0;
+    float *X = nullptr;             // [K][mtot]
+    double *acc = nullptr;          // [2F][mtot]
+    double *W = nullptr;            // [2][K][2F] FP64 twiddles (exact mode)
+    void *Wf = nullptr;             // split-BF16 twiddle fragments (MFMA mode)
+    double *freq_d = nullptr;
+    double *flux_d = nullptr;       // [12][F]
+    int K = 64, slot = 0;
+    std::vector<int> slot_t;
+    int samples = 0;
+    // design region and adjoint
+    int des_lo[3] = {0, 0, 0}, des_n[3] = {0, 0, 0};
+    std::vector<float> des_rho;     // design densities (i-major, k fastest)
+    float *rho_box = nullptr;       // density per dispersive-box cell (1 outside the design block)
+    bool sampling = true;           // DFT sampling on (off while the adjoint recomputes a segment)
+    float *probe = nullptr;         // E(r0) per step while the adjoint's forward runs
+    long probe_p = 0;
+    int probe_t0 = 0;               // step whose E^{n+1} lands in probe[0..2]
+    float *traj_step[3] = {};       // dispersive kernel writes E^{n+1} here when set
+    float *traj_first[2][3] = {};   // and E^{n-1}, E^n here for the first recorded step
+    struct Adj {
+        bool ready = false;
+        float *E[2][3] = {}, *H[2][3] = {};
+        float *psiE[2][6] = {}, *psiH[2][6] = {};
+        float *st[3][7] = {};
+        float *Qb[3] = {}, *Eb[3] = {};
+        double *grad = nullptr;
+    } adj;
+    // misc
+    float *dense = nullptr;         // NX*NY*NZ staging for field copies
+    size_t fsz = 0;                 // floats per padded field allocation
+    long fbase = 0;                 // offset of element (0,0,0) in it
+    size_t hsz[6] = {}, esz[6] = {};
+    int t = 0;
+    int tj = 8, ichunk = 32;
+    // fused step: row-blocked kernel over the whole grid + TFSF fix-up, or the general kernel alone
+    bool use_rows = false;
+    bool has_clean = false;         // clean box non-empty: clean variant there, general variant on the rest
+    int cbox[4] = {};               // clean box {ilo, ihi, jlo, jhi}
+    int xbox[2][4] = {}, ybox[2][4] = {}, cbox4[4][4] = {};   // x-general, y-general and corner pieces
+    int nxbox = 0, nybox = 0, ncorner = 0;
+    float ik0[2][2] = {};           // interior 1/(kappa d): [E, H][x, y]
+    unsigned *aux_d = nullptr;      // table image of the row-blocked kernel
+    bool concurrent = true;         // side slabs on a second stream beside the clean box (FDTD_CONCURRENT=0 to serialise)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
+    int rows = 3, cchunk = 16;      // own rows per thread and x chunk of the row-blocked kernel (clean box)
+    int schunk = 8;                 // x chunk of the side slabs
+    int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
+    int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t ev_used = 0;
+};
+
+namespace fdtd_host {
+
+int set_err(fdtd_sim *s, int code, const std::string &m);
+
+#define HIPCHK(sim, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fdtd_host::set_err(sim, FDTD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(fdtd_sim *s, T **ptr, size_t count) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        s->err = std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e);
+        return FDTD_ERR_ALLOC;
+    }
+    s->allocs.push_back(p);
+    e = hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(T), s->stream);
+    if (e != hipSuccess) { s->err = hipGetErrorString(e); return FDTD_ERR_DEVICE; }
+    *ptr = (T *)p;
+    return FDTD_OK;
+}
+
+// Allocation zero-fills run on the solver stream, so uploads are queued on the
+// same stream (and waited for, the host vector being temporary) to keep them
+// ordered after the fill.
+template <typename T>
+int upload(fdtd_sim *s, T **ptr, const std::vector<T> &v) {
+    int rc = dalloc(s, ptr, v.size());
+    if (rc) return rc;
+    HIPCHK(s, hipMemcpyAsync(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+// Kernel arguments at the handle's current parity.
+StepDev make_step(fdtd_sim *s, bool fused);
+LineDev make_line(fdtd_sim *s);
+DispDev make_disp(fdtd_sim *s, int parity);
+// DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.
+bool sampling_step(const fdtd_sim *s);
+int after_sample(fdtd_sim *s);
+int contract(fdtd_sim *s);
+
+}  // namespace fdtd_host
+
+#endif

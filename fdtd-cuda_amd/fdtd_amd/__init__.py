@@ -19,8 +19,12 @@ EXPORTS = [
     "fdtd_set_materials", "fdtd_add_sphere", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
     "fdtd_dft_accumulate", "fdtd_step", "fdtd_sync", "fdtd_get_field", "fdtd_set_field",
     "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
-    "fdtd_set_design", "fdtd_adjoint",
+    "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod",
+    "fdtd_batch_create", "fdtd_batch_run", "fdtd_batch_tile", "fdtd_batch_last_error", "fdtd_batch_destroy",
 ]
+
+BATCH_NO_GRAPH = 1
+BATCH_NO_TUNE = 2
 
 _lib = None
 
@@ -58,6 +62,12 @@ def load_library(path=LIB_PATH):
         "fdtd_kernel_timing": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
         "fdtd_set_design": (ip, [vp, C.POINTER(C.c_int * 3), C.POINTER(C.c_int * 3), fp]),
         "fdtd_adjoint": (ip, [vp, ip, ip, C.POINTER(C.c_int * 3), ip, dp, dp, dp]),
+        "fdtd_add_rod": (ip, [vp, C.POINTER(C.c_double * 3), ip, C.c_double, C.c_double]),
+        "fdtd_batch_create": (ip, [C.POINTER(vp), ip, ip, C.POINTER(vp)]),
+        "fdtd_batch_run": (ip, [vp, ip]),
+        "fdtd_batch_tile": (ip, [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "fdtd_batch_last_error": (C.c_char_p, [vp]),
+        "fdtd_batch_destroy": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -124,6 +134,12 @@ class Simulation:
     def add_sphere(self, centre, radius):
         c = (C.c_double * 3)(*[float(x) for x in centre])
         self._check(self.lib.fdtd_add_sphere(self.h, C.byref(c), float(radius)))
+        self.info = self.get_info()
+
+    def add_rod(self, centre, axis, length, radius):
+        """Nanorod with hemispherical caps (length, radius in metres) along axis 0..2."""
+        c = (C.c_double * 3)(*[float(x) for x in centre])
+        self._check(self.lib.fdtd_add_rod(self.h, C.byref(c), int(axis), float(length), float(radius)))
         self.info = self.get_info()
 
     def set_design(self, lo, rho):
@@ -213,3 +229,41 @@ class Simulation:
         ms, n = C.c_double(), C.c_long()
         self._check(self.lib.fdtd_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Batch:
+    """Many simulations advanced together (fdtd_batch_*): one launch per
+    kernel per step for all of them, a HIP graph per sampling period, tiles
+    picked by the autotuner.  Keeps references to the simulations."""
+
+    def __init__(self, sims, flags=0):
+        self.lib = load_library()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        self.h = C.c_void_p()
+        rc = self.lib.fdtd_batch_create(arr, len(self.sims), int(flags), C.byref(self.h))
+        if rc != FDTD_OK:
+            raise FDTDError(rc, self.lib.fdtd_batch_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != FDTD_OK:
+            raise FDTDError(rc, self.lib.fdtd_batch_last_error(self.h).decode())
+
+    def run(self, nsteps):
+        self._check(self.lib.fdtd_batch_run(self.h, int(nsteps)))
+
+    def tile(self, i):
+        tj, ic = C.c_int(), C.c_int()
+        self._check(self.lib.fdtd_batch_tile(self.h, int(i), C.byref(tj), C.byref(ic)))
+        return tj.value, ic.value
+
+    def close(self):
+        if self.h:
+            self.lib.fdtd_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

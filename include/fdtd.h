@@ -140,6 +140,12 @@ int  fdtd_set_materials(fdtd_sim *sim, const uint8_t *id_x, const uint8_t *id_y,
  * the dimer and other multi-particle geometries; same validity rules as
  * fdtd_set_materials. */
 int  fdtd_add_sphere(fdtd_sim *sim, const double centre[3], double radius);
+/* Nanorod (no reference counterpart; parameter-sweep workloads): a cylinder
+ * with hemispherical caps, total length `length` and radius `radius`
+ * (metres), along axis 0..2 (x..z), centre in cells.  An E component at Yee
+ * position x is inside when its distance to the axis segment between the cap
+ * centres is < radius/dx; same validity rules as fdtd_set_materials. */
+int  fdtd_add_rod(fdtd_sim *sim, const double centre[3], int axis, double length, double radius);
 /* Design region (no reference counterpart; inverse-design workloads): the block
  * [lo, lo+n) of cells becomes dispersive for all three E components with the
  * ADE coefficients blended by the density rho in [0, 1] (i-major, k fastest):
@@ -201,6 +207,33 @@ int  fdtd_flux(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *ra
  * 1 all field-update kernels of a step, 2 the clean-box launch of the
  * row-blocked kernel alone (the dominant one; same as 1 when there is none). */
 int  fdtd_kernel_timing(fdtd_sim *sim, int enable, double *total_ms, long *launches);
+
+/*
+ * Batched sweeps (no reference counterpart: the reference runs one
+ * simulation per process, main.cu:2724-2934).  A batch advances many
+ * independent simulations of any grid sizes together: each kernel of a step
+ * runs once for all of them (the blocks of every simulation in one grid), and
+ * the launches of one DFT sampling period are captured in a HIP graph and
+ * replayed.  The field update of a batch is the general fused kernel (TFSF
+ * inside) with a tile per grid size picked by the autotuner (timed on the
+ * device at fdtd_batch_create, remembered per process and, when the
+ * environment variable FDTD_TUNE_FILE names a file, across processes).
+ *
+ * The batch does not own the simulations, which must outlive it.  They must
+ * be at the same step t, share dft_stride and dft_block, and have no
+ * hydrodynamic model.  Between fdtd_batch_run calls each simulation can be
+ * read (fdtd_flux, fdtd_get_field ...) or stepped on its own.
+ */
+typedef struct fdtd_batch fdtd_batch;
+#define FDTD_BATCH_NO_GRAPH 1   /* launch every step directly instead of replaying a graph */
+#define FDTD_BATCH_NO_TUNE  2   /* default tile (8 rows, x chunk 32) for every grid size   */
+int  fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out);
+/* Advances every simulation by nsteps full steps (as fdtd_step does). */
+int  fdtd_batch_run(fdtd_batch *b, int nsteps);
+/* Tile used for simulation i: block rows (tj - 1 owned) and x chunk. */
+int  fdtd_batch_tile(const fdtd_batch *b, int i, int *tj, int *ichunk);
+const char *fdtd_batch_last_error(const fdtd_batch *b);
+int  fdtd_batch_destroy(fdtd_batch *b);
 
 #ifdef __cplusplus
 }
