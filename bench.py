@@ -98,6 +98,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=128,
+                    help="steps of the 32-rod sweep window, batched and one by one (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=400,
+                    help="steps of the adjoint/forward timing on the inverse-design grid (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB copy kernel 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -149,8 +153,11 @@ def main():
     achieved = kbytes / k_avg_s / 1e9
     u_avg_s = (u_ms / 1e3) / max(u_n, 1)
 
+    sim.close()
     if args.calibrate_copy:
         copy_calibration()
+    sweep = sweep_bench(args.sweep_steps) if args.sweep_steps > 0 else None
+    adjoint = adjoint_bench(args.adjoint_steps) if args.adjoint_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -199,6 +206,8 @@ def main():
             "achieved_GBs": round(abytes / u_avg_s / 1e9, 1),
             "frac": round(abytes / u_avg_s / 1e9 / HBM_PEAK_GBS, 4),
         },
+        "sweep": sweep,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -232,6 +241,115 @@ def copy_calibration(nbytes=1 << 30, reps=4):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
+
+
+def sweep_bench(steps, seed=20240601):
+    """Batched sweep of 32 seeded nanorod simulations of uneven sizes: a
+    window of `steps` steps run one simulation at a time (each on its own
+    fastest path, fdtd_step) and then batched (fdtd_batch_run, graph replay),
+    as aggregate Mcell-updates/s over all cells of all simulations."""
+    import fdtd_amd
+    from fdtd_amd import sweep
+    steps = max(4, steps // 4 * 4)
+    cfgs, redraws = sweep.draw(seed)
+    t0 = time.perf_counter()
+    sims = [sweep.make_sim(fdtd_amd.Simulation, fdtd_amd, c) for c in cfgs]
+    t_create = time.perf_counter() - t0
+    cells = sweep.total_cells(cfgs)
+    for s in sims:
+        s.step(8)
+        s.sync()
+    t_alone = 0.0
+    for s in sims:
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t_alone += time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = fdtd_amd.Batch(sims)
+    t_tune = time.perf_counter() - t0
+    b.run(8)                      # warm-up, graph capture
+    t0 = time.perf_counter()
+    b.run(steps)
+    t_batch = time.perf_counter() - t0
+    tiles = sorted({b.tile(i) for i in range(len(sims))})
+    b.close()
+    for s in sims:
+        s.close()
+    return {
+        "workload": "32 seeded gold nanorods (splitmix64 seed %d), edges 96..192 step 8, dx=1.5 nm, 10-cell CPML, "
+                    "Drude+2CP, 32 wavelengths 500-1200 nm" % seed,
+        "sims": len(cfgs),
+        "redraws": redraws,
+        "cells": cells,
+        "steps": steps,
+        "batched_mcells_per_s": round(cells * steps / t_batch / 1e6, 1),
+        "one_by_one_mcells_per_s": round(cells * steps / t_alone / 1e6, 1),
+        "speedup": round(t_alone / t_batch, 3),
+        "ms_per_step_batched": round(t_batch * 1e3 / steps, 3),
+        "tiles_tj_ichunk": [list(t) for t in tiles],
+        "autotune_s": round(t_tune, 2),
+        "create_s": round(t_create, 2),
+    }
+
+
+def adjoint_bench(steps, ckpt=None, seed=7):
+    """Adjoint/forward wall-time ratio on the inverse-design grid: 256^3
+    interior at dx = 2 nm (281^3 cells), 12-cell CPML, a 64x64x32 density
+    block at the centre with rho ~ U[0, 1) (seeded), objective |E|^2 at 633,
+    785 and 850 nm at the cell 5 nm above the block's top face, x-polarised
+    plane wave along z.  Forward = fdtd_step over `steps`; adjoint =
+    fdtd_adjoint over the same steps (checkpointed forward, recompute,
+    transposed sweep)."""
+    import fdtd_amd
+    ckpt = ckpt or max(1, steps // 4)
+    p = fdtd_amd.default_params()
+    p.NCELLX = p.NCELLY = p.NCELLZ = 256
+    p.NcpmlX = p.NcpmlY = p.NcpmlZ = 12
+    p.dx = p.dy = p.dz = 2e-9
+    p.nano_sphere = 0
+    p.material = 3
+    p.NUM_freq = 3
+    p.lam_min, p.lam_max = 633e-9, 850e-9
+    p.inc_phi = 90.0
+    p.TimeFactor = 3 ** 0.5
+    p.tfsf_inset = 10
+    T = 256 + 25
+    n = (64, 64, 32)
+    lo = tuple(T // 2 - k // 2 for k in n)
+    r0 = (T // 2, T // 2, lo[2] + n[2] + 2)      # cell centre 2.5 cells = 5 nm above the top face
+    rho = np.random.default_rng(seed).uniform(0.0, 1.0, size=n)
+    freqs = 2.99792458e8 / np.array([633e-9, 785e-9, 850e-9])
+    f = fdtd_amd.Simulation(p)
+    f.set_design(lo, rho)
+    f.step(4)
+    f.sync()
+    t0 = time.perf_counter()
+    f.step(steps)
+    f.sync()
+    t_fwd = time.perf_counter() - t0
+    f.close()
+    res = {}
+    for mode, ck in (("recompute", ckpt), ("stored", 0)):
+        a = fdtd_amd.Simulation(p)
+        a.set_design(lo, rho)
+        t0 = time.perf_counter()
+        J, g = a.adjoint(steps, r0, freqs, checkpoint=ck)
+        res[mode] = (time.perf_counter() - t0, J, float(np.linalg.norm(g)))
+        a.close()
+    t_rec, t_sto = res["recompute"][0], res["stored"][0]
+    return {
+        "workload": "inverse design: 256^3 interior dx=2 nm, 64x64x32 density block, |E|^2 at 633/785/850 nm",
+        "steps": steps,
+        "checkpoint": ckpt,
+        "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
+        "adjoint_ms_per_step": round(t_rec * 1e3 / steps, 4),
+        "ratio": round(t_rec / t_fwd, 3),
+        "stored_trajectory_ms_per_step": round(t_sto * 1e3 / steps, 4),
+        "stored_trajectory_ratio": round(t_sto / t_fwd, 3),
+        "J": res["recompute"][1],
+        "grad_norm": res["recompute"][2],
+    }
 
 
 def cpu_baseline(p, info, nsteps):

@@ -371,6 +371,7 @@ int plan_rows(fdtd_sim *s) {
         for (int x = 0; x < 6; x++) ptrs[q * 6 + x] = s->psiH[q][x];
     for (int x = 0; x < 6; x++) ptrs[12 + x] = s->psiE[x];
     std::memcpy(&img[1548], ptrs, sizeof(ptrs));
+    s->aux_img = img;
     int rc = upload(s, &s->aux_d, img);
     if (rc) return rc;
     if (const char *v = std::getenv("FDTD_CONCURRENT")) s->concurrent = std::atoi(v) != 0;
@@ -723,6 +724,38 @@ int adj_alloc(fdtd_sim *s) {
         if ((rc = dalloc(s, &s->adj.Qb[c], s->nbox)) || (rc = dalloc(s, &s->adj.Eb[c], s->nbox))) return rc;
     }
     if ((rc = dalloc(s, &s->adj.grad, s->nbox))) return rc;
+    // transposed step on the clean box with the row-blocked kernel (variant 3):
+    // needs the row path, wave-edge lanes outside the z slabs of E (their
+    // published z weights carry no psi), and the dispersive box grown by two
+    // cells inside the clean box and free of z CPML (launch_adj_box_fix)
+    s->adj.rows = false;
+    if (const char *v = std::getenv("FDTD_ADJ_SCHUNK")) s->adj_schunk = std::max(1, std::atoi(v));
+    const char *ap = std::getenv("FDTD_ADJ_PATH");
+    if (s->use_rows && !(ap && std::string(ap) == "general") && s->S.Chh == 1.0) {
+        const Setup &S = s->S;
+        const int NZ = S.N[2], Ncz = S.cp[2].Nc, Fz = S.cp[2].F;
+        const int W = rows_waves(NZ), nseg = (NZ + 1 + 64 * W - 3) / (64 * W - 2);
+        bool ok = true;
+        for (int seg = 0; seg < nseg && ok; seg++)
+            for (int w = 1; w < W && ok; w++) {
+                const int k = seg * (64 * W - 2) - 1 + 64 * w;
+                if (k >= 0 && k < NZ && (k <= Ncz + 1 || k >= Fz - 1)) ok = false;
+            }
+        if (ok && s->nbox > 0) {
+            for (int a = 0; a < 2; a++)
+                if (s->box_lo[a] - 2 < s->cbox[2 * a] || s->box_lo[a] + s->box_n[a] + 2 > s->cbox[2 * a + 1]) ok = false;
+            if (s->box_lo[2] - 2 <= Ncz + 1 || s->box_lo[2] + s->box_n[2] + 2 >= Fz - 1) ok = false;
+        }
+        if (ok) {
+            std::vector<uint32_t> img = s->aux_img;
+            float *ptrs[24];
+            for (int q = 0; q < 2; q++)
+                for (int x = 0; x < 6; x++) { ptrs[q * 6 + x] = s->adj.psiH[q][x]; ptrs[12 + q * 6 + x] = s->adj.psiE[q][x]; }
+            std::memcpy(&img[1548], ptrs, sizeof(ptrs));
+            if ((rc = upload(s, &s->adj.aux_d, img))) return rc;
+            s->adj.rows = true;
+        }
+    }
     s->adj.ready = true;
     return FDTD_OK;
 }
@@ -761,7 +794,7 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
 
 int fdtd_adjoint(fdtd_sim *s, int nsteps, int ckpt, const int r0[3], int nf, const double *freq, double *J,
                  double *grad) {
-    if (!s || !r0 || !freq || !J || !grad || nsteps < 1 || ckpt < 1 || nf < 1) return FDTD_ERR_INVALID;
+    if (!s || !r0 || !freq || !J || !grad || nsteps < 1 || nf < 1) return FDTD_ERR_INVALID;
     const Setup &S = s->S;
     if (s->des_rho.empty() || !s->rho_box) return set_err(s, FDTD_ERR_INVALID, "adjoint: no design region set");
     if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "adjoint: hydrodynamic model not supported");
@@ -787,31 +820,112 @@ int fdtd_adjoint(fdtd_sim *s, int nsteps, int ckpt, const int r0[3], int nf, con
 
 namespace {
 
+// One transposed step: adjoint level pa -> 1 - pa, psi_E pe, psi_H ph.  On
+// the row path the clean box runs on the row-blocked kernel (variant 3) with
+// the dispersive-cell correction after it, and the side slabs and corner
+// columns on the general transposed kernel beside it on the second stream.
+int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
+    const Setup &S = s->S;
+    AdjDev av{};
+    const StepDev sd = make_step(s, false);
+    av.g = sd.g; av.c = sd.c;
+    for (int x = 0; x < 3; x++) av.cp[x] = sd.cp[x];
+    for (int c = 0; c < 3; c++) {
+        av.Ein[c] = s->adj.E[pa][c]; av.Hin[c] = s->adj.H[pa][c];
+        av.Eout[c] = s->adj.E[1 - pa][c]; av.Hout[c] = s->adj.H[1 - pa][c];
+        av.bid[c] = s->disp_id[c]; av.Qb[c] = s->adj.Qb[c]; av.Eb[c] = s->adj.Eb[c];
+    }
+    for (int x = 0; x < 6; x++) {
+        av.psiE_in.p[x] = s->adj.psiE[pe][x]; av.psiE_out.p[x] = s->adj.psiE[1 - pe][x];
+        av.psiH_in.p[x] = s->adj.psiH[ph][x]; av.psiH_out.p[x] = s->adj.psiH[1 - ph][x];
+    }
+    for (int x = 0; x < 3; x++) { av.blo[x] = s->box_lo[x]; av.bn[x] = s->box_n[x]; }
+    if (!s->adj.rows) {
+        const int full[1][4] = {{0, s->g.NX, 0, s->g.NY}};
+        HIPCHK(s, launch_adj_step(av, s->tj, s->ichunk, full, 1, s->stream));
+        return FDTD_OK;
+    }
+    const bool fork = s->stream2 != nullptr;
+    hipStream_t side = fork ? s->stream2 : s->stream;
+    if (fork) {
+        HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+    }
+    // the slabs are thin: one launch for all four, short x chunks so they fill the device
+    int slabs[4][4], ns = 0;
+    for (int q = 0; q < s->nxbox; q++) std::memcpy(slabs[ns++], s->xbox[q], sizeof(slabs[0]));
+    for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
+    if (ns) HIPCHK(s, launch_adj_step(av, 8, s->adj_schunk, slabs, ns, side));
+    if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, side));
+    if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
+    RowsArgs r = make_rows(s);
+    const long fb = s->fbase;
+    for (int c = 0; c < 3; c++) {
+        r.Ein[c] = (const char *)(s->adj.E[pa][c] - fb); r.Hin[c] = (const char *)(s->adj.H[pa][c] - fb);
+        r.Eout[c] = (char *)(s->adj.E[1 - pa][c] - fb); r.Hout[c] = (char *)(s->adj.H[1 - pa][c] - fb);
+    }
+    r.aux = s->adj.aux_d;
+    r.Chh = 1.0f;
+    r.Che = (float)-S.Ceh;
+    r.Cee = (float)S.Cee;
+    r.Ceh = (float)-S.Che;
+    r.pp = ph;
+    r.pe = pe;
+    r.npiece = 1;
+    std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
+    HIPCHK(s, launch_step_rows(r, s->rows, 3, s->stream));
+    if (s->nbox) HIPCHK(s, launch_adj_box_fix(av, s->stream));
+    if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    return FDTD_OK;
+}
+
 int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3], int nf, const double *freq,
                 double *J, double *grad) {
     const Setup &S = s->S;
     int rc;
     if ((rc = adj_alloc(s))) return rc;
     const int t0 = s->t;
-    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    // ckpt <= 0: the box trajectory of every step is kept (one forward pass);
+    // else checkpoints every ckpt steps and the trajectory of one segment at a time
+    const bool store = ckpt <= 0;
+    const int seglen = store ? nsteps : ckpt;
+    const int nseg = store ? 1 : (nsteps + ckpt - 1) / ckpt;
     const long nbox = s->nbox;
     if (hipMalloc((void **)&W.probe, (size_t)3 * nsteps * sizeof(float)) != hipSuccess ||
-        hipMalloc((void **)&W.traj, (size_t)(ckpt + 2) * 3 * nbox * sizeof(float)) != hipSuccess)
+        hipMalloc((void **)&W.traj, (size_t)(seglen + 2) * 3 * std::max(nbox, 1L) * sizeof(float)) != hipSuccess)
         return set_err(s, FDTD_ERR_ALLOC, "adjoint: scratch allocation failed");
     float *const probe = W.probe, *const traj = W.traj;
     std::vector<Checkpoint> &cks = W.cks;
-    cks.resize(nseg + 1);
-    // forward with checkpoints and the objective probe
+    cks.resize(store ? 0 : nseg + 1);
+    auto record = [&](int st) {
+        for (int c = 0; c < 3; c++) {
+            s->traj_step[c] = traj + ((size_t)(st + 2) * 3 + c) * nbox;
+            s->traj_first[0][c] = st == 0 ? traj + ((size_t)0 * 3 + c) * nbox : nullptr;
+            s->traj_first[1][c] = st == 0 ? traj + ((size_t)1 * 3 + c) * nbox : nullptr;
+        }
+    };
+    auto unrecord = [&]() {
+        for (int c = 0; c < 3; c++) s->traj_step[c] = s->traj_first[0][c] = s->traj_first[1][c] = nullptr;
+    };
+    // forward with checkpoints (or the full trajectory) and the objective probe
     s->probe = probe;
     s->probe_p = gidx(s->g, r0[0], r0[1], r0[2]);
     s->probe_t0 = t0;
-    for (int seg = 0; seg < nseg; seg++) {
-        if ((rc = ckpt_copy(s, cks[seg], true))) return rc;
-        const int len = std::min(ckpt, nsteps - seg * ckpt);
-        if ((rc = fdtd_step(s, len))) return rc;
+    if (store) {
+        for (int st = 0; st < nsteps; st++) {
+            record(st);
+            if ((rc = fdtd_step(s, 1))) return rc;
+        }
+        unrecord();
+    } else {
+        for (int seg = 0; seg < nseg; seg++) {
+            if ((rc = ckpt_copy(s, cks[seg], true))) return rc;
+            const int len = std::min(ckpt, nsteps - seg * ckpt);
+            if ((rc = fdtd_step(s, len))) return rc;
+        }
     }
     s->probe = nullptr;
-    if ((rc = ckpt_copy(s, cks[nseg], true))) return rc;
+    if (!store && (rc = ckpt_copy(s, cks[nseg], true))) return rc;
     std::vector<float> pr((size_t)3 * nsteps);
     if (hipStreamSynchronize(s->stream) != hipSuccess ||
         hipMemcpy(pr.data(), probe, pr.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
@@ -832,21 +946,21 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
     int pa = 0, pe = 0, ph = 0;
     const long p0 = gidx(s->g, r0[0], r0[1], r0[2]);
     for (int seg = nseg - 1; seg >= 0; seg--) {
-        Checkpoint &ck = cks[seg];
-        s->t = ck.t; s->p = ck.p; s->pp = ck.pp; s->le = ck.le;
-        if ((rc = ckpt_copy(s, ck, false))) return rc;
-        const int len = std::min(ckpt, nsteps - seg * ckpt);
-        for (int st = 0; st < len; st++) {
-            for (int c = 0; c < 3; c++) {
-                s->traj_step[c] = traj + ((size_t)(st + 2) * 3 + c) * nbox;
-                s->traj_first[0][c] = st == 0 ? traj + ((size_t)0 * 3 + c) * nbox : nullptr;
-                s->traj_first[1][c] = st == 0 ? traj + ((size_t)1 * 3 + c) * nbox : nullptr;
+        int seg_t0 = t0, len = nsteps;
+        if (!store) {
+            Checkpoint &ck = cks[seg];
+            s->t = ck.t; s->p = ck.p; s->pp = ck.pp; s->le = ck.le;
+            if ((rc = ckpt_copy(s, ck, false))) return rc;
+            seg_t0 = ck.t;
+            len = std::min(ckpt, nsteps - seg * ckpt);
+            for (int st = 0; st < len; st++) {
+                record(st);
+                if ((rc = fdtd_step(s, 1))) return rc;
             }
-            if ((rc = fdtd_step(s, 1))) return rc;
+            unrecord();
         }
-        for (int c = 0; c < 3; c++) s->traj_step[c] = s->traj_first[0][c] = s->traj_first[1][c] = nullptr;
         for (int st = len - 1; st >= 0; st--) {
-            const int t = ck.t + st;
+            const int t = seg_t0 + st;
             double a[3] = {0, 0, 0};
             for (int f = 0; f < nf; f++) {
                 const std::complex<double> w = std::exp(std::complex<double>(0, 2 * fdtd::kPI * freq[f] * (t + 1.0) * S.dt));
@@ -866,21 +980,7 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
             }
             ad.grad = s->adj.grad;
             HIPCHK(s, launch_adj_disp(ad, s->stream));
-            AdjDev av{};
-            const StepDev sd = make_step(s, false);
-            av.g = sd.g; av.c = sd.c;
-            for (int x = 0; x < 3; x++) av.cp[x] = sd.cp[x];
-            for (int c = 0; c < 3; c++) {
-                av.Ein[c] = s->adj.E[pa][c]; av.Hin[c] = s->adj.H[pa][c];
-                av.Eout[c] = s->adj.E[1 - pa][c]; av.Hout[c] = s->adj.H[1 - pa][c];
-                av.bid[c] = s->disp_id[c]; av.Qb[c] = s->adj.Qb[c]; av.Eb[c] = s->adj.Eb[c];
-            }
-            for (int x = 0; x < 6; x++) {
-                av.psiE_in.p[x] = s->adj.psiE[pe][x]; av.psiE_out.p[x] = s->adj.psiE[1 - pe][x];
-                av.psiH_in.p[x] = s->adj.psiH[ph][x]; av.psiH_out.p[x] = s->adj.psiH[1 - ph][x];
-            }
-            for (int x = 0; x < 3; x++) { av.blo[x] = s->box_lo[x]; av.bn[x] = s->box_n[x]; }
-            HIPCHK(s, launch_adj_step(av, s->tj, s->ichunk, s->stream));
+            if ((rc = adjoint_step(s, pa, pe, ph))) return rc;
             pa ^= 1; pe ^= 1; ph ^= 1;
         }
     }
@@ -895,6 +995,7 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
                 const int bi = s->des_lo[0] + a - s->box_lo[0], bj = s->des_lo[1] + b - s->box_lo[1], bk = s->des_lo[2] + c - s->box_lo[2];
                 grad[v++] = gb[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk];
             }
+    if (store) return FDTD_OK;   // the forward pass left the final state in place
     // leave the forward state after nsteps
     Checkpoint &fin = cks[nseg];
     s->t = fin.t; s->p = fin.p; s->pp = fin.pp; s->le = fin.le;

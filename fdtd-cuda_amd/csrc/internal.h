@@ -193,6 +193,7 @@ struct RowsArgs {
     unsigned sxy4;                      // plane pitch in bytes
     int Nc[3], F[3];
     int pp;                             // psi_H input set
+    int pe;                             // adjoint variant: psi_E input set
     int ichunk;
     int xcd;                            // 1: XCD-contiguous block order (j-neighbour tiles share an L2)
     int npiece;
@@ -201,13 +202,16 @@ struct RowsArgs {
     int start[5];
 };
 // Words of the table image and its layout (see kernels.hip).
-constexpr int kRowsAuxWords = 1548 + 36;
+constexpr int kRowsAuxWords = 1548 + 48;
 constexpr int kRowsAuxAxis = 512, kRowsAuxMaxEdge = 64, kRowsAuxMaxLayers = 64;
 // Waves per block of the row-blocked kernel for a given NZ (one z segment when NZ < 64*16 - 2).
 int rows_waves(int NZ);
 // R: own rows per thread (2 or 3); TFSF is left to launch_tfsf_fix.
 // variant 0: clean boxes (no x or y CPML); 1: x general (x CPML and x
-// boundaries per plane), y clean; 2: y general (per row), x clean.
+// boundaries per plane), y clean; 2: y general (per row), x clean; 3: the
+// transposed step of the adjoint on clean boxes (coefficients passed as
+// Chh = 1, Che = -Ceh, Cee, Ceh = -Che; the table image holds both adjoint
+// psi_E sets; dispersive cells are corrected afterwards by launch_adj_box_fix).
 hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st);
 
 // box[q] = {ia, ib, ja, jb}; one to four disjoint boxes.
@@ -269,7 +273,10 @@ long fused_batch_blocks(const Geom &g, int tj, int ichunk, Pieces *pc, int *nkt)
 // Adjoint: inject dJ/dE^{n+1}(r0), transposed dispersive update, fused transposed step.
 hipError_t launch_adj_inject(float *const E[3], long p0, float ax, float ay, float az, hipStream_t st);
 hipError_t launch_adj_disp(const AdjDispDev &a, hipStream_t st);
-hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, hipStream_t st);
+// box[q] = {ia, ib, ja, jb}: one to four disjoint x-y boxes of cells (all z).
+hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st);
+// Dispersive-cell correction after the adjoint row kernel (variant 3 of launch_step_rows).
+hipError_t launch_adj_box_fix(const AdjDev &a, hipStream_t st);
 // E at one cell for every step (objective probe).
 hipError_t launch_probe(const float *const E[3], long p0, float *out, hipStream_t st);
 

@@ -604,7 +604,7 @@ __device__ inline float from_upper_lane(float v) {
 // recursion be, ce/d (@256, @320: near layers then far), bh, ch/d (@384,
 // @448); then the interior 1/(kappa d) for E (@1536+a) and H (@1539+a), ew
 // (@1542+a), and the psi pointers at word 1548: psi_H sets 0 and 1 (6 each),
-// psi_E (6), 64-bit each.
+// psi_E (6), and for the adjoint a second psi_E set (6), 64-bit each.
 constexpr int kAuxAxis = 512, kAuxIkE0 = 1536, kAuxIkH0 = 1539, kAuxEw = 1542, kAuxPsi = 1548;
 typedef __attribute__((address_space(1))) float gfloat;   // global memory (psi pointers read from LDS)
 
@@ -640,13 +640,27 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // component the psi terms add in the reference's order.  CPML tables and psi
 // pointers live in LDS (the table image above), which keeps scalar registers
 // for the field pointers.
-template <int W, int R, bool XG, bool YG>
+//
+// ADJ: the transposed step of the discrete adjoint on a clean box (no x or y
+// CPML).  Written with the weights of k_adj_step, the adjoint of a step is
+//   Hhat_tot = Hhat + forward differences of V,  V = Q ik (+ z CPML), Q = Ceh Ehat
+//   Ehat'    = Cee Ehat + backward differences of W,  W = G ik (+ z CPML), G = Che Hhat_tot
+// which is the forward step with Che -> -Ceh and Ceh -> -Che (the host passes
+// them swapped) wherever 1/(kappa d) is constant; along z each lane first
+// forms its own z weights (Vyz/Ceh, Vxz/Ceh before the H half, Wyz/Che,
+// Wxz/Che before the E half, with the adjoint psi recursions of its z-slab
+// cells) and the z differences are taken of those.  Both adjoint psi sets are
+// double buffered (pp: psi_H set read, pe: psi_E set read).  The lanes at
+// wave edges must lie outside the z slabs (checked by the host).
+template <int W, int R, bool XG, bool YG, bool ADJ = false>
 __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
+    static_assert(!ADJ || (!XG && !YG), "the adjoint variant runs clean boxes only");
     __shared__ float sK[2][2][W][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
     __shared__ float sE[2][2][W][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
-    __shared__ __align__(16) unsigned sT[kAuxPsi + 36];
+    constexpr int kTab = kAuxPsi + (ADJ ? 48 : 36);
+    __shared__ __align__(16) unsigned sT[kTab];
     const int NX = ra.NX, NY = ra.NY, NZ = ra.NZ;
-    for (int q = threadIdx.y * 64 + threadIdx.x; q < kAuxPsi + 36; q += 64 * W) sT[q] = ra.aux[q];
+    for (int q = threadIdx.y * 64 + threadIdx.x; q < kTab; q += 64 * W) sT[q] = ra.aux[q];
     const float *sTf = (const float *)sT;
     int L = ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : blockIdx.x, pc = 0;
     while (pc + 1 < ra.npiece && L >= ra.start[pc + 1]) pc++;
@@ -674,6 +688,9 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     };
     auto coef = [&](int a, int which, int layer) { return sTf[a * kAuxAxis + 256 + which * 64 + layer]; };
     auto psi = [&](int idx) { return *(gfloat *const *)&sT[kAuxPsi + 2 * idx]; };   // 0-5 set 0, 6-11 set 1, 12-17 psi_E
+    // adjoint: psi_E read from set pe (12 + 6 pe), written to the other
+    const int ein = ADJ ? 12 + 6 * ra.pe : 12, eout = ADJ ? 18 - 6 * ra.pe : 12;
+    const int hout = (1 - pp) * 6;
     lds_sync();
 
     // z: per lane
@@ -702,6 +719,8 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     auto lxe = [&](int ip) { return XG && ip > 0 && ip < NX - 1 ? elay(0, ip) : -1; };
 
     const float Chh = ra.Chh, Che = ra.Che, Cee = ra.Cee, Ceh = ra.Ceh;
+    // adjoint: the forward Ceh and Che (Q = cE Ehat, G = cH Hhat_tot) and their inverses
+    const float cE = -Che, cH = -Ceh, icE = ADJ ? 1.f / cE : 0.f, icH = ADJ ? 1.f / cH : 0.f;
     const char *bEx = ra.Ein[0], *bEy = ra.Ein[1], *bEz = ra.Ein[2];
     const char *bHx = ra.Hin[0], *bHy = ra.Hin[1], *bHz = ra.Hin[2];
     char *oEx = ra.Eout[0], *oEy = ra.Eout[1], *oEz = ra.Eout[2];
@@ -716,10 +735,14 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
             for (int r = 0; r <= R; r++) { xk[r] = sE[b][0][w + 1][r]; yk[r] = sE[b][1][w + 1][r]; }
         }
     };
+    // adjoint: the first lane (outside the z slabs) publishes its z weights
     auto e_publish = [&](const float *x, const float *y, int b) {
         if (lane == 0) {
 #pragma unroll
-            for (int r = 0; r <= R; r++) { sE[b][0][w][r] = x[r]; sE[b][1][w][r] = y[r]; }
+            for (int r = 0; r <= R; r++) {
+                sE[b][0][w][r] = ADJ ? (kC ? x[r] * ikz_e : 0.f) : x[r];
+                sE[b][1][w][r] = ADJ ? (kC ? y[r] * ikz_e : 0.f) : y[r];
+            }
         }
     };
 
@@ -729,16 +752,36 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     // into the slab arrays, values unused where no slab applies) so the wait
     // counts after them stay exact.
     struct PSet {
-        float hxz[R + 1], hyz[R + 1], exz[R], eyz[R];
+        float hxz[R + 1], hyz[R + 1], exz[R + 1], eyz[R + 1];
         float hxy[R + 1], hzy[R + 1], exy[R], ezy[R];
         float hyx[R + 1], hzx[R + 1], eyx[R], ezx[R];
     };
     auto load_psi = [&](int ip, PSet &P) {
 #pragma unroll
-        for (int r = 0; r <= R; r++) { P.hxz[r] = 0.f; P.hyz[r] = 0.f; }
-#pragma unroll
-        for (int r = 0; r < R; r++) { P.exz[r] = 0.f; P.eyz[r] = 0.f; }
+        for (int r = 0; r <= R; r++) { P.hxz[r] = 0.f; P.hyz[r] = 0.f; P.exz[r] = 0.f; P.eyz[r] = 0.f; }
         const bool iA = !XG || ip < NX - 1, iC = !XG || (ip > 0 && ip < NX - 1);
+        if (ADJ) {
+            // adjoint: psi_E of rows 0..R (z weights of the H half), psi_H of rows 1..R (E half)
+            if (ze && lze >= 0) {
+                const gfloat *pa = psi(ein + 0), *pb = psi(ein + 2);
+#pragma unroll
+                for (int r = 0; r <= R; r++) {
+                    const long q = ((long)ip * NY + jb + r) * Lze + lze;
+                    P.exz[r] = pa[q];
+                    P.eyz[r] = pb[q];
+                }
+            }
+            if (zh && lzh >= 0) {
+                const gfloat *pa = psi(pp * 6 + 0), *pb = psi(pp * 6 + 2);
+#pragma unroll
+                for (int r = 1; r <= R; r++) {
+                    const long q = ((long)ip * NY + jb + r) * Lzh + lzh;
+                    P.hxz[r] = pa[q];
+                    P.hyz[r] = pb[q];
+                }
+            }
+            return;
+        }
         if (zh && iA && lzh >= 0) {
             const gfloat *pa = psi(pp * 6 + 0), *pb = psi(pp * 6 + 2);
 #pragma unroll
@@ -791,16 +834,56 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
         }
     };
 
+    // Adjoint z weights.  E side (rows 0..R): Vxz/Ceh = ex ik - c (psi/Ceh - ex),
+    // Vyz/Ceh = ey ik + c (psi/Ceh + ey), and the own cells' psi_E <- b (psi -+ Ceh e).
+    // H side (rows 1..R, from Hhat_tot): Wxz/Che = hx ik + c (psi/Che + hx),
+    // Wyz/Che = hy ik - c (psi/Che - hy), psi_H <- b (psi +- Che h).  Zero
+    // outside the component domains (k_adj_step's adj_v, adj_w).
+    auto zweights = [&](int ip, const float *x, const float *y, const PSet &P, bool wr, float *zx, float *zy) {
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            float ax = kC ? x[r] * ikz_e : 0.f, ay = kC ? y[r] * ikz_e : 0.f;
+            if (ze && lze >= 0) {
+                ax -= cze * (P.exz[r] * icE - x[r]);
+                ay += cze * (P.eyz[r] * icE + y[r]);
+                if (wr && r > 0 && kown && jb + r < jbox) {
+                    const long q = ((long)ip * NY + jb + r) * Lze + lze;
+                    psi(eout + 0)[q] = bze * (P.exz[r] - cE * x[r]);
+                    psi(eout + 2)[q] = bze * (P.eyz[r] + cE * y[r]);
+                }
+            }
+            zx[r] = ax;
+            zy[r] = ay;
+        }
+    };
+    auto hweights = [&](int ip, const float *hx, const float *hy, const PSet &P, float *wx, float *wy) {
+#pragma unroll
+        for (int r = 1; r <= R; r++) {
+            float ax = kA ? hx[r] * ikz_h : 0.f, ay = kA ? hy[r] * ikz_h : 0.f;
+            if (zh && lzh >= 0) {
+                ax += czh * (P.hxz[r] * icH + hx[r]);
+                ay -= czh * (P.hyz[r] * icH - hy[r]);
+                if (kown && jb + r < jbox) {
+                    const long q = ((long)ip * NY + jb + r) * Lzh + lzh;
+                    psi(hout + 0)[q] = bzh * (P.hxz[r] + cH * hx[r]);
+                    psi(hout + 2)[q] = bzh * (P.hyz[r] - cH * hy[r]);
+                }
+            }
+            wx[r] = ax;
+            wy[r] = ay;
+        }
+    };
+
     // H of plane ip for rows 0..R from E^n of plane ip (rows 0..R, ex and ez of
     // row R+1, ex and ey at k+1), ey and ez of plane ip+1, H^{n-1/2} and psi_H
     // of plane ip.  wr: write psi_H of the own cells.
+    // Adjoint: exk, eyk are the z weights of the k+1 neighbours and zx, zy the own ones.
     auto hplane = [&](int ip, const float *ex, const float *ey, const float *ez, float exj, float ezj,
                       const float *exk, const float *eyk, const float *ny, const float *nz, const float (*ho)[R + 1],
-                      const PSet &P, bool wr, float *hx, float *hy, float *hz) {
+                      const PSet &P, bool wr, const float *zx, const float *zy, float *hx, float *hy, float *hz) {
         const bool iA = !XG || ip < NX - 1;
         const float ikx_h = XG ? ik(0, false, ip, NX) : ra.ik0[1][0];
         const int lx = lxh(ip);
-        const int hout = (1 - pp) * 6;
 #pragma unroll
         for (int r = 0; r <= R; r++) {
             const int j = jb + r;
@@ -810,12 +893,18 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
             const float dEy_z = eyk[r] - ey[r], dEz_y = zj - ez[r];
             const float dEz_x = nz[r] - ez[r], dEx_z = exk[r] - ex[r];
             const float dEx_y = xj - ex[r], dEy_x = ny[r] - ey[r];
-            float vx = Chh * ho[0][r] + Che * (dEy_z * ikz_h - dEz_y * iky_h);
-            float vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - dEx_z * ikz_h);
+            float vx, vy;
+            if constexpr (ADJ) {
+                vx = Chh * ho[0][r] + Che * ((eyk[r] - zy[r]) - dEz_y * iky_h);
+                vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - (exk[r] - zx[r]));
+            } else {
+                vx = Chh * ho[0][r] + Che * (dEy_z * ikz_h - dEz_y * iky_h);
+                vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - dEx_z * ikz_h);
+            }
             float vz = Chh * ho[2][r] + Che * (dEx_y * iky_h - dEy_x * ikx_h);
             // per component the psi terms add in the reference's order:
             // hx +z -y, hy -z +x, hz -x +y (main.cu:834-1037)
-            if (zh && iA && jA(r) && lzh >= 0) {
+            if (!ADJ && zh && iA && jA(r) && lzh >= 0) {
                 const long q = ((long)ip * NY + j) * Lzh + lzh;
                 const float a = bzh * P.hxz[r] + czh * dEy_z;
                 const float b = bzh * P.hyz[r] + czh * dEx_z;
@@ -886,8 +975,14 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
         load_psi(i0 - 1, P1);
         e_publish(P.x, P.y, 1);
         lds_sync();
-        e_up(P.x, P.y, 1, pxk, pyk);
-        hplane(i0 - 1, P.x, P.y, P.z, P.xj, P.zj, pxk, pyk, EA.y, EA.z, H1.h, P1, false, hx, hy, hz);
+        float zx[R + 1], zy[R + 1];
+        if constexpr (ADJ) {
+            zweights(i0 - 1, P.x, P.y, P1, false, zx, zy);
+            e_up(zx, zy, 1, pxk, pyk);
+        } else {
+            e_up(P.x, P.y, 1, pxk, pyk);
+        }
+        hplane(i0 - 1, P.x, P.y, P.z, P.xj, P.zj, pxk, pyk, EA.y, EA.z, H1.h, P1, false, zx, zy, hx, hy, hz);
 #pragma unroll
         for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
     }
@@ -909,18 +1004,27 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
         load_psi(i, Pc);
         load_e(more ? i + 2 : i, Fr);
         load_h(more ? i + 1 : i, Hn);
-        float hx[R + 1], hy[R + 1], hz[R + 1], exk[R + 1], eyk[R + 1];
-        e_up(C.x, C.y, buf, exk, eyk);
-        hplane(i, C.x, C.y, C.z, C.xj, C.zj, exk, eyk, N.y, N.z, Hc.h, Pc, true, hx, hy, hz);
+        float hx[R + 1], hy[R + 1], hz[R + 1], exk[R + 1], eyk[R + 1], zx[R + 1], zy[R + 1];
+        if constexpr (ADJ) {
+            zweights(i, C.x, C.y, Pc, true, zx, zy);
+            e_up(zx, zy, buf, exk, eyk);
+        } else {
+            e_up(C.x, C.y, buf, exk, eyk);
+        }
+        hplane(i, C.x, C.y, C.z, C.xj, C.zj, exk, eyk, N.y, N.z, Hc.h, Pc, true, zx, zy, hx, hy, hz);
+        // adjoint: the z differences of the E half are taken of the H-side weights
+        float wx[R + 1], wy[R + 1];
+        if constexpr (ADJ) hweights(i, hx, hy, Pc, wx, wy);
+        const float *sy = ADJ ? wy : hy, *sx = ADJ ? wx : hx;
 #pragma unroll
         for (int r = 1; r <= R; r++) {
             if (kown && jb + r < jbox) { stn<true>(oHx, pl + ro[r], hx[r]); stn<true>(oHy, pl + ro[r], hy[r]); stn<true>(oHz, pl + ro[r], hz[r]); }
-            if (lane == 63) { sK[buf][0][w][r] = hy[r]; sK[buf][1][w][r] = hx[r]; }
+            if (lane == 63) { sK[buf][0][w][r] = sy[r]; sK[buf][1][w][r] = sx[r]; }
         }
         e_publish(N.x, N.y, buf ^ 1);
         float hyk[R + 1], hxk[R + 1];
 #pragma unroll
-        for (int r = 1; r <= R; r++) { hyk[r] = from_lower_lane(hy[r]); hxk[r] = from_lower_lane(hx[r]); }
+        for (int r = 1; r <= R; r++) { hyk[r] = from_lower_lane(sy[r]); hxk[r] = from_lower_lane(sx[r]); }
         lds_sync();
         if (first_lane) {
 #pragma unroll
@@ -937,12 +1041,18 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
             const float dHz_y = hz[r] - hz[r - 1], dHy_z = hy[r] - hyk[r];
             const float dHx_z = hx[r] - hxk[r], dHz_x = hz[r] - hzP[r];
             const float dHy_x = hy[r] - hyP[r], dHx_y = hx[r] - hx[r - 1];
-            float vx = Cee * C.x[r] + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
-            float vy = Cee * C.y[r] + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
+            float vx, vy;
+            if constexpr (ADJ) {
+                vx = Cee * C.x[r] + Ceh * (dHz_y * iky_e - (wy[r] - hyk[r]));
+                vy = Cee * C.y[r] + Ceh * ((wx[r] - hxk[r]) - dHz_x * ikx_e);
+            } else {
+                vx = Cee * C.x[r] + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
+                vy = Cee * C.y[r] + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
+            }
             float vz = Cee * C.z[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
             const bool ux = iA && jC(r) && kC, uy = iC && jA(r) && kC, uz = iC && jC(r) && kA;
             // ex -z +y, ey +z -x, ez -y +x (main.cu:1172-1906)
-            if (ze && iA && lze >= 0) {
+            if (!ADJ && ze && iA && lze >= 0) {
                 const long q = ((long)i * NY + j) * Lze + lze;
                 if (jC(r)) { const float a = bze * Pc.exz[r - 1] + cze * dHy_z; vx -= Ceh * a; if (own) psi(12 + 0)[q] = a; }
                 if (iC && jA(r)) { const float a = bze * Pc.eyz[r - 1] + cze * dHx_z; vy += Ceh * a; if (own) psi(12 + 2)[q] = a; }
@@ -1656,16 +1766,27 @@ __device__ inline float adj_w(const AdjDev &a, int c, int i, int j, int k, float
 // Fused transposed step, the twin of k_fused_step: same tiling, same halo
 // rows/lanes, LDS exchange of the W weights needed at j-1 and k-1, W at i-1
 // carried in registers, V at i+1 computed one plane ahead.
+// Runs the cells of one to four x-y boxes (all z), as k_fused_step.
 template <int TJ>
-__global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk) {
+__global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk, int nkt, Pieces pc) {
     __shared__ float sW[2][4][TJ][65];   // 0 Wyz, 1 Wxz (k-1);  2 Wzy, 3 Wxy (j-1)
     const Geom &g = a.g;
+    int jt, kt, ch, ia, ib, ja, jbx;
+    {
+        int L = blockIdx.x, q = 0;
+        while (q + 1 < pc.n && L >= pc.start[q + 1]) q++;
+        L -= pc.start[q];
+        jt = L % pc.njt[q];
+        kt = (L / pc.njt[q]) % nkt;
+        ch = L / (pc.njt[q] * nkt);
+        ia = pc.ia[q]; ib = pc.ib[q]; ja = pc.ja[q]; jbx = pc.jb[q];
+    }
     const int lane = threadIdx.x, row = threadIdx.y;
-    const int k = blockIdx.x * 63 - 1 + lane;
-    const int j = blockIdx.y * (TJ - 1) - 1 + row;
-    const int i0 = blockIdx.z * ichunk;
-    const int i1 = min(i0 + ichunk, g.NX);
-    const bool own = lane > 0 && row > 0 && k < g.NZ && j < g.NY;
+    const int k = kt * 63 - 1 + lane;
+    const int j = ja + jt * (TJ - 1) - 1 + row;
+    const int i0 = ia + ch * ichunk;
+    const int i1 = min(i0 + ichunk, ib);
+    const bool own = lane > 0 && row > 0 && k < g.NZ && j < jbx;
     const int kc = min(k, g.NZ - 1), jc = min(j, g.NY - 1);
     const int kk = kc == k ? k : g.NZ, jj = jc == j ? j : g.NY;   // beyond the grid: no domain
     const long sz = g.sz, sxy = g.sxy;
@@ -1737,6 +1858,76 @@ __global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk) {
         bx = nbx; by = nby; bz = nbz;
         buf ^= 1;
     }
+}
+
+// Dispersive cells after the adjoint row kernel (variant 3), which applies
+// the vacuum weights everywhere.  At a dispersive E cell the curl weight is
+// Q = Qb instead of Ceh Ehat and the base Eb instead of Cee Ehat; the step is
+// linear, so the outputs take dV = (Qb - Ceh Ehat) 1/(kappa d) through the
+// forward differences of the H half, dH, and then Eb - Cee Ehat plus the
+// backward differences of dW = Che dH 1/(kappa d) in the E half.  One thread
+// per cell of the box grown by one cell on every side; the region must be
+// free of CPML (checked by the host).
+__device__ inline float adj_dq(const AdjDev &a, int c, int i, int j, int k) {
+    const long b = adj_box(a, c, i, j, k);
+    return b < 0 ? 0.f : a.Qb[c][b] - a.c.Ceh * a.Ein[c][gidx(a.g, i, j, k)];
+}
+__device__ inline void adj_dh(const AdjDev &a, int i, int j, int k, float &hx, float &hy, float &hz) {
+    const float ix = a.cp[0].ik_e[i], iy = a.cp[1].ik_e[j], iz = a.cp[2].ik_e[k];
+    // dV at the cell and its +1 neighbours (1/(kappa d) of the E positions)
+    const float yz = adj_dq(a, 1, i, j, k) * iz, yz1 = adj_dq(a, 1, i, j, k + 1) * a.cp[2].ik_e[k + 1];
+    const float zy = adj_dq(a, 2, i, j, k) * iy, zy1 = adj_dq(a, 2, i, j + 1, k) * a.cp[1].ik_e[j + 1];
+    const float xz = adj_dq(a, 0, i, j, k) * iz, xz1 = adj_dq(a, 0, i, j, k + 1) * a.cp[2].ik_e[k + 1];
+    const float zx = adj_dq(a, 2, i, j, k) * ix, zx1 = adj_dq(a, 2, i + 1, j, k) * a.cp[0].ik_e[i + 1];
+    const float xy = adj_dq(a, 0, i, j, k) * iy, xy1 = adj_dq(a, 0, i, j + 1, k) * a.cp[1].ik_e[j + 1];
+    const float yx = adj_dq(a, 1, i, j, k) * ix, yx1 = adj_dq(a, 1, i + 1, j, k) * a.cp[0].ik_e[i + 1];
+    hx = (yz - yz1) - (zy - zy1);
+    hy = -(xz - xz1) + (zx - zx1);
+    hz = (xy - xy1) - (yx - yx1);
+}
+__global__ void k_adj_box_fix(AdjDev a) {
+    const int nx = a.bn[0] + 2, ny = a.bn[1] + 2, nz = a.bn[2] + 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nx * ny * nz) return;
+    const int i = a.blo[0] - 1 + (int)(idx / ((long)ny * nz));
+    const int j = a.blo[1] - 1 + (int)((idx / nz) % ny);
+    const int k = a.blo[2] - 1 + (int)(idx % nz);
+    const long p = gidx(a.g, i, j, k);
+    float hx, hy, hz;
+    adj_dh(a, i, j, k, hx, hy, hz);
+    if (hx != 0.f) a.Hout[0][p] += a.c.Chh * hx;
+    if (hy != 0.f) a.Hout[1][p] += a.c.Chh * hy;
+    if (hz != 0.f) a.Hout[2][p] += a.c.Chh * hz;
+    float xm, ym, zm, t0, t1;
+    const float Che = a.c.Che;
+    const float hx_i = a.cp[0].ik_h[i], hy_i = a.cp[1].ik_h[j], hz_i = a.cp[2].ik_h[k];
+    // E half: dW at the cell and its -1 neighbours (1/(kappa d) of the H positions)
+    float ex = 0.f, ey = 0.f, ez = 0.f;
+    adj_dh(a, i, j, k - 1, xm, ym, t0);          // k-1: dWyz, dWxz
+    const float ikzm = a.cp[2].ik_h[k - 1];
+    ex += Che * (hy * hz_i - ym * ikzm);
+    ey -= Che * (hx * hz_i - xm * ikzm);
+    adj_dh(a, i, j - 1, k, xm, t0, zm);          // j-1: dWzy, dWxy
+    const float ikym = a.cp[1].ik_h[j - 1];
+    ex -= Che * (hz * hy_i - zm * ikym);
+    ez += Che * (hx * hy_i - xm * ikym);
+    adj_dh(a, i - 1, j, k, t0, ym, zm);          // i-1: dWzx, dWyx
+    const float ikxm = a.cp[0].ik_h[i - 1];
+    ey += Che * (hz * hx_i - zm * ikxm);
+    ez -= Che * (hy * hx_i - ym * ikxm);
+    (void)t1;
+    for (int c = 0; c < 3; c++) {
+        const long b = adj_box(a, c, i, j, k);
+        if (b >= 0) {
+            const float db = a.Eb[c][b] - a.c.Cee * a.Ein[c][p];
+            if (c == 0) ex += db;
+            else if (c == 1) ey += db;
+            else ez += db;
+        }
+    }
+    if (ex != 0.f) a.Eout[0][p] += ex;
+    if (ey != 0.f) a.Eout[1][p] += ey;
+    if (ez != 0.f) a.Eout[2][p] += ez;
 }
 
 // Adjoint of the (P, J) dispersive update at every dispersive box cell:
@@ -1937,27 +2128,27 @@ hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int W, bool XG, bool YG>
+template <int W, bool XG, bool YG, bool ADJ>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
     switch (R) {
-        case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG>), grid, dim3(64, W), 0, st, ra); break;
-        case 3: hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG>), grid, dim3(64, W), 0, st, ra); break;
+        case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
+        case 3: hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <bool XG, bool YG>
+template <bool XG, bool YG, bool ADJ = false>
 static hipError_t launch_rows_g(const RowsArgs &ra, int R, int W, dim3 grid, hipStream_t st) {
     switch (W) {
-        case 2: return launch_rows_w<2, XG, YG>(ra, R, grid, st);
-        case 3: return launch_rows_w<3, XG, YG>(ra, R, grid, st);
-        case 4: return launch_rows_w<4, XG, YG>(ra, R, grid, st);
-        case 5: return launch_rows_w<5, XG, YG>(ra, R, grid, st);
-        case 6: return launch_rows_w<6, XG, YG>(ra, R, grid, st);
-        case 8: return launch_rows_w<8, XG, YG>(ra, R, grid, st);
-        case 12: return launch_rows_w<12, XG, YG>(ra, R, grid, st);
-        default: return launch_rows_w<16, XG, YG>(ra, R, grid, st);
+        case 2: return launch_rows_w<2, XG, YG, ADJ>(ra, R, grid, st);
+        case 3: return launch_rows_w<3, XG, YG, ADJ>(ra, R, grid, st);
+        case 4: return launch_rows_w<4, XG, YG, ADJ>(ra, R, grid, st);
+        case 5: return launch_rows_w<5, XG, YG, ADJ>(ra, R, grid, st);
+        case 6: return launch_rows_w<6, XG, YG, ADJ>(ra, R, grid, st);
+        case 8: return launch_rows_w<8, XG, YG, ADJ>(ra, R, grid, st);
+        case 12: return launch_rows_w<12, XG, YG, ADJ>(ra, R, grid, st);
+        default: return launch_rows_w<16, XG, YG, ADJ>(ra, R, grid, st);
     }
 }
 
@@ -1969,7 +2160,7 @@ int rows_waves(int NZ) {
 }
 
 hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
-    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 2)
+    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 3)
         return hipErrorInvalidValue;
     const int W = rows_waves(ra.NZ);
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
@@ -1988,19 +2179,41 @@ hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
     switch (variant) {
         case 0: return launch_rows_g<false, false>(ra, R, W, grid, st);
         case 1: return launch_rows_g<true, false>(ra, R, W, grid, st);
-        default: return launch_rows_g<false, true>(ra, R, W, grid, st);
+        case 2: return launch_rows_g<false, true>(ra, R, W, grid, st);
+        default: return launch_rows_g<false, false, true>(ra, R, W, grid, st);
     }
 }
 
-hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, hipStream_t st) {
+hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st) {
+    if (nbox < 1 || nbox > 4 || ichunk < 1 || (tj != 4 && tj != 8 && tj != 16)) return hipErrorInvalidValue;
     const int nkt = (a.g.NZ + 62) / 63;
-    const int nchunk = (a.g.NX + ichunk - 1) / ichunk;
-    switch (tj) {
-        case 4: hipLaunchKernelGGL(k_adj_step<4>, dim3(nkt, (a.g.NY + 2) / 3, nchunk), dim3(64, 4), 0, st, a, ichunk); break;
-        case 8: hipLaunchKernelGGL(k_adj_step<8>, dim3(nkt, (a.g.NY + 6) / 7, nchunk), dim3(64, 8), 0, st, a, ichunk); break;
-        case 16: hipLaunchKernelGGL(k_adj_step<16>, dim3(nkt, (a.g.NY + 14) / 15, nchunk), dim3(64, 16), 0, st, a, ichunk); break;
-        default: return hipErrorInvalidValue;
+    Pieces pc{};
+    pc.n = nbox;
+    long total = 0;
+    for (int q = 0; q < nbox; q++) {
+        pc.ia[q] = box[q][0]; pc.ib[q] = box[q][1]; pc.ja[q] = box[q][2]; pc.jb[q] = box[q][3];
+        if (pc.ia[q] < 0 || pc.ib[q] > a.g.NX || pc.ia[q] >= pc.ib[q] || pc.ja[q] < 0 || pc.jb[q] > a.g.NY ||
+            pc.ja[q] >= pc.jb[q])
+            return hipErrorInvalidValue;
+        pc.njt[q] = (pc.jb[q] - pc.ja[q] + tj - 2) / (tj - 1);
+        pc.start[q] = (int)total;
+        total += (long)pc.njt[q] * nkt * ((pc.ib[q] - pc.ia[q] + ichunk - 1) / ichunk);
     }
+    if (total > 0x7fffffff) return hipErrorInvalidValue;
+    pc.start[nbox] = (int)total;
+    const dim3 grid((unsigned)total);
+    switch (tj) {
+        case 4: hipLaunchKernelGGL(k_adj_step<4>, grid, dim3(64, 4), 0, st, a, ichunk, nkt, pc); break;
+        case 8: hipLaunchKernelGGL(k_adj_step<8>, grid, dim3(64, 8), 0, st, a, ichunk, nkt, pc); break;
+        default: hipLaunchKernelGGL(k_adj_step<16>, grid, dim3(64, 16), 0, st, a, ichunk, nkt, pc); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_box_fix(const AdjDev &a, hipStream_t st) {
+    const long n = (long)(a.bn[0] + 2) * (a.bn[1] + 2) * (a.bn[2] + 2);
+    if (a.bn[0] <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_box_fix, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

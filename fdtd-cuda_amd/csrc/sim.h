@@ -67,6 +67,8 @@ struct fdtd_sim {
         float *st[3][7] = {};
         float *Qb[3] = {}, *Eb[3] = {};
         double *grad = nullptr;
+        bool rows = false;          // transposed step on the clean box by the row-blocked kernel
+        unsigned *aux_d = nullptr;  // its table image (adjoint psi pointers)
     } adj;
     // misc
     float *dense = nullptr;         // NX*NY*NZ staging for field copies
@@ -83,12 +85,14 @@ struct fdtd_sim {
     int nxbox = 0, nybox = 0, ncorner = 0;
     float ik0[2][2] = {};           // interior 1/(kappa d): [E, H][x, y]
     unsigned *aux_d = nullptr;      // table image of the row-blocked kernel
+    std::vector<uint32_t> aux_img;  // host copy of it
     bool concurrent = true;         // side slabs on a second stream beside the clean box (FDTD_CONCURRENT=0 to serialise)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
     int rows = 3, cchunk = 16;      // own rows per thread and x chunk of the row-blocked kernel (clean box)
     int schunk = 8;                 // x chunk of the side slabs
+    int adj_schunk = 4;             // x chunk of the side slabs in the transposed step (FDTD_ADJ_SCHUNK)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
     bool timing = false;

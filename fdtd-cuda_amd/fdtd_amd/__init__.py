@@ -153,6 +153,8 @@ class Simulation:
     def adjoint(self, nsteps, r0, freqs, checkpoint=64):
         """nsteps forward steps, then the discrete adjoint sweep of the point
         objective J = sum_f sum_c |sum_t E_c^{t+1}(r0) exp(i 2 pi f (t+1) dt)|^2.
+        checkpoint > 0: checkpoints every `checkpoint` steps and segment
+        recompute; 0: the design-box trajectory is kept instead.
         Returns (J, dJ/drho over the design block)."""
         f = np.ascontiguousarray(freqs, dtype=np.float64)
         g = np.empty(int(np.prod(self.design_shape)))

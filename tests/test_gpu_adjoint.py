@@ -3,10 +3,14 @@ oracle adjoint, whose own gradient is pinned by central differences in
 test_oracle_adjoint.py.
 
 FP32 on the GPU against FP64: J within 1e-3 relative, dJ/drho within 2e-3 of
-the largest gradient entry.  Checkpoint spacing must not change a single bit
-of the gradient (every segment is recomputed by the same kernels), and the
-state left behind is the plain forward state after nsteps.
+the largest gradient entry.  Checkpoint spacing, and keeping the trajectory
+instead of recomputing, must not change a single bit of the gradient (every
+segment is recomputed by the same kernels), and the state left behind is the
+plain forward state after nsteps.  The transposed step on the row-blocked
+kernel (clean box) agrees with the general transposed kernel to rounding.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -55,7 +59,7 @@ def test_gpu_adjoint_matches_oracle():
 def test_gpu_adjoint_checkpoint_spacing_is_exact():
     p, rho, lo, r0 = problem(G)
     out = []
-    for ck in (7, 64, STEPS):
+    for ck in (7, 64, STEPS, 0):
         sim = G.Simulation(p)
         sim.set_design(lo, rho)
         out.append(sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=ck))
@@ -99,3 +103,47 @@ def test_gpu_adjoint_autograd_wrapper():
     assert float(J) == J2
     assert np.array_equal(r.grad.numpy(), 3.0 * g2)
 
+
+
+def _adjoint_with_path(path, ck=40, shape=(5, 4, 3)):
+    old = os.environ.get("FDTD_ADJ_PATH")
+    os.environ["FDTD_ADJ_PATH"] = path
+    try:
+        p, rho, lo, r0 = problem(G, shape)
+        sim = G.Simulation(p)
+        sim.set_design(lo, rho)
+        return sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=ck)
+    finally:
+        if old is None:
+            del os.environ["FDTD_ADJ_PATH"]
+        else:
+            os.environ["FDTD_ADJ_PATH"] = old
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_row_kernel_matches_general_kernel():
+    Jr, gr = _adjoint_with_path("rows")
+    Jg, gg = _adjoint_with_path("general")
+    err = np.abs(gr - gg).max() / np.abs(gg).max()
+    print("row vs general transposed step: max |dg| / max |g| = %.2e" % err, flush=True)
+    assert Jr == Jg
+    assert err < 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_larger_block_against_oracle():
+    """A design block wider than a wave's z run and reaching close to the
+    z CPML, trajectory kept (no recompute)."""
+    shape = (6, 7, 7)
+    p, rho, lo, r0 = problem(G, shape, seed=11)
+    sim = G.Simulation(p)
+    sim.set_design(lo, rho)
+    J, g = sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=0)
+    po, rhoo, loo, r0o = problem(O, shape, seed=11)
+    o = O.Oracle(po)
+    o.set_design(loo, rhoo.astype(np.float32).astype(np.float64))
+    o.set_objective(r0o, C0 / LAMS)
+    Jo, go = o.adjoint(STEPS)
+    print("J", J, Jo, "err", np.abs(g - go).max() / np.abs(go).max(), flush=True)
+    assert J == pytest.approx(Jo, rel=1e-3)
+    assert np.abs(g - go).max() < 2e-3 * np.abs(go).max()
