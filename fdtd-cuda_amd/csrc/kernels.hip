@@ -1763,14 +1763,23 @@ __device__ inline float adj_w(const AdjDev &a, int c, int i, int j, int k, float
     return a.c.Chh * ht;
 }
 
-// Fused transposed step, the twin of k_fused_step: same tiling, same halo
-// rows/lanes, LDS exchange of the W weights needed at j-1 and k-1, W at i-1
-// carried in registers, V at i+1 computed one plane ahead.
-// Runs the cells of one to four x-y boxes (all z), as k_fused_step.
+// Fused transposed step over one to four x-y boxes of cells (all z).  Tile:
+// 64 lanes along z (k = kt*62 - 1 + lane; lanes 0 and 63 are halo) by TJ
+// rows along y (j = jt*(TJ-1) - 1 + row; row 0 is halo), marching x.  Each
+// thread forms the V weights of its own cell one plane ahead and the W
+// weights after the H half; V at k+1 and W at k-1 come from the adjacent
+// lane (DPP), W at j-1 and V at j+1 from the adjacent row through LDS (V one
+// plane ahead, so one barrier per plane serves both; the top row computes V
+// at j+1 itself), V at i+1 and W at i-1 stay in registers.  CPML layers are
+// hoisted per lane (z), per row (y) and per plane (x); dispersive-box lookups
+// happen only where the tile meets the box.
+struct AdjV { float xy, xz, yz, yx, zx, zy, bx, by, bz; };
 template <int TJ>
 __global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk, int nkt, Pieces pc) {
-    __shared__ float sW[2][4][TJ][65];   // 0 Wyz, 1 Wxz (k-1);  2 Wzy, 3 Wxy (j-1)
+    __shared__ float sW[2][2][TJ][64];   // Wzy, Wxy (E at j+1 takes them as j-1)
+    __shared__ float sV[2][2][TJ][64];   // Vxy, Vzy one plane ahead (H at j-1 takes them as j+1)
     const Geom &g = a.g;
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
     int jt, kt, ch, ia, ib, ja, jbx;
     {
         int L = blockIdx.x, q = 0;
@@ -1782,81 +1791,194 @@ __global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk, int 
         ia = pc.ia[q]; ib = pc.ib[q]; ja = pc.ja[q]; jbx = pc.jb[q];
     }
     const int lane = threadIdx.x, row = threadIdx.y;
-    const int k = kt * 63 - 1 + lane;
-    const int j = ja + jt * (TJ - 1) - 1 + row;
-    const int i0 = ia + ch * ichunk;
-    const int i1 = min(i0 + ichunk, ib);
-    const bool own = lane > 0 && row > 0 && k < g.NZ && j < jbx;
-    const int kc = min(k, g.NZ - 1), jc = min(j, g.NY - 1);
-    const int kk = kc == k ? k : g.NZ, jj = jc == j ? j : g.NY;   // beyond the grid: no domain
+    const int k = kt * 62 - 1 + lane;
+    const int j = __builtin_amdgcn_readfirstlane(ja + jt * (TJ - 1) - 1 + row);
+    const int i0 = ia + ch * ichunk, i1 = min(i0 + ichunk, ib);
+    const bool own = lane > 0 && lane < 63 && row > 0 && k < NZ && j < jbx;
+    const int kc = min(k, NZ - 1), jc = min(j, NY - 1);
+    const int kk = k < NZ ? k : NZ, jj = j < NY ? j : NY;   // beyond the grid: outside every domain
     const long sz = g.sz, sxy = g.sxy;
     const long cidx = (long)jc * sz + kc;
+    const CpmlDev &cx = a.cp[0], &cy = a.cp[1], &cz = a.cp[2];
+    const float Chh = a.c.Chh, Che = a.c.Che, Cee = a.c.Cee, Ceh = a.c.Ceh;
+
+    // (j,k) parts of the component domains (e_dom, h_dom) and of the psi range
+    const bool kin = kk >= 0, jin = jj >= 0;
+    const bool ejx = jin && kin && jj > 0 && jj < NY - 1 && kk > 0 && kk < NZ - 1;
+    const bool ejy = jin && kin && jj < NY - 1 && kk > 0 && kk < NZ - 1;
+    const bool ejz = jin && kin && jj > 0 && jj < NY - 1 && kk < NZ - 1;
+    const bool hjx = jin && kin && jj < NY - 1 && kk < NZ - 1;
+    const bool hjy = jin && kin && jj < NY && kk < NZ - 1;
+    const bool hjz = jin && kin && jj < NY - 1 && kk < NZ;
+    const bool pjk = jin && kin && jj < NY - 1 && kk < NZ - 1;
+    const int kq = max(kc, 0), jq = max(jc, 0);
+    const float ikz_e = cz.ik_e[kq], ikz_h = cz.ik_h[kq], iky_e = cy.ik_e[jq], iky_h = cy.ik_h[jq];
+    // z slabs per lane, y slabs per row
+    const int lze = pjk ? e_layer(cz, kk) : -1, lzh = pjk ? h_layer(cz, kk) : -1;
+    const bool zE = __ballot(lze >= 0) != 0, zH = __ballot(lzh >= 0) != 0;
+    const float bze = lze >= 0 ? cz.be[lze] : 0.f, cze = lze >= 0 ? cz.ce[lze] : 0.f;
+    const float bzh = lzh >= 0 ? cz.bh[lzh] : 0.f, czh = lzh >= 0 ? cz.ch[lzh] : 0.f;
+    const int Lze = 2 * (cz.Nc + 1), Lzh = 2 * cz.Nc, Lye = 2 * (cy.Nc + 1), Lyh = 2 * cy.Nc;
+    const int lye = pjk ? e_layer(cy, jj) : -1, lyh = pjk ? h_layer(cy, jj) : -1;   // row-uniform
+    const float bye = lye >= 0 ? cy.be[lye] : 0.f, cye = lye >= 0 ? cy.ce[lye] : 0.f;
+    const float byh = lyh >= 0 ? cy.bh[lyh] : 0.f, cyh = lyh >= 0 ? cy.ch[lyh] : 0.f;
+    const int oz = max(kk, 0);
+    // dispersive box along (j,k)
+    const bool bjk = a.bn[0] > 0 && jj >= a.blo[1] && jj < a.blo[1] + a.bn[1] && kk >= a.blo[2] && kk < a.blo[2] + a.bn[2];
+    const bool tbox = __ballot(bjk) != 0;
+    const long bjkoff = bjk ? (long)(jj - a.blo[1]) * a.bn[2] + (kk - a.blo[2]) : 0;
     const float *Ex = a.Ein[0], *Ey = a.Ein[1], *Ez = a.Ein[2];
 
-    // V weights (and bases) of the own cell at plane i, carried one plane ahead
-    float vxy, vxz, vyz, vyx, vzx, vzy, bx, by, bz;
-    float wzxP = 0.f, wyxP = 0.f;   // W at plane i-1
-    auto vplane = [&](int ip, bool wr, float &Vxy, float &Vxz, float &Vyz, float &Vyx, float &Vzx, float &Vzy, float &Bx,
-                      float &By, float &Bz) {
-        const long p = (long)ip * sxy + cidx;
-        adj_v(a, 0, ip, jj, kk, Ex[p], wr, Vxy, Vxz, Bx);
-        adj_v(a, 1, ip, jj, kk, Ey[p], wr, Vyz, Vyx, By);
-        adj_v(a, 2, ip, jj, kk, Ez[p], wr, Vzx, Vzy, Bz);
+    // V weights and bases of the own cell at plane ip from Ehat (ex, ey, ez); wr: psi_E written
+    auto vcell = [&](int ip, float ex, float ey, float ez, bool wr, AdjV &v) {
+        const bool dx = ejx && ip >= 0 && ip < NX - 1, dy = ejy && ip > 0 && ip < NX - 1, dz = ejz && ip > 0 && ip < NX - 1;
+        const float ikx_e = cx.ik_e[min(max(ip, 0), NX - 1)];
+        float Qx = Ceh * ex, Qy = Ceh * ey, Qz = Ceh * ez;
+        float bx = Cee * ex, by = Cee * ey, bz = Cee * ez;
+        if (tbox && ip >= a.blo[0] && ip < a.blo[0] + a.bn[0]) {
+            if (bjk) {
+                const long b = (long)(ip - a.blo[0]) * a.bn[1] * a.bn[2] + bjkoff;
+                if (a.bid[0][b]) { Qx = a.Qb[0][b]; bx = a.Eb[0][b]; }
+                if (a.bid[1][b]) { Qy = a.Qb[1][b]; by = a.Eb[1][b]; }
+                if (a.bid[2][b]) { Qz = a.Qb[2][b]; bz = a.Eb[2][b]; }
+            }
+        }
+        v.xy = Qx * iky_e; v.xz = Qx * ikz_e;
+        v.yz = Qy * ikz_e; v.yx = Qy * ikx_e;
+        v.zx = Qz * ikx_e; v.zy = Qz * iky_e;
+        const bool pin = pjk && ip >= 0 && ip < NX - 1;
+        // CPML (dispersive cells never lie in a slab)
+        if (pin && lye >= 0) {
+            const long q = yslab(g, Lye, ip, lye, oz);
+            if (dx) { const float t = a.psiE_in.p[1][q] + Qx; v.xy += cye * t; if (wr) a.psiE_out.p[1][q] = bye * t; }
+            if (dz) { const float t = a.psiE_in.p[4][q] - Qz; v.zy -= cye * t; if (wr) a.psiE_out.p[4][q] = bye * t; }
+        }
+        if (pin && zE && lze >= 0) {
+            const long q = zslab(g, Lze, ip, jj, lze);
+            if (dx) { const float t = a.psiE_in.p[0][q] - Qx; v.xz -= cze * t; if (wr) a.psiE_out.p[0][q] = bze * t; }
+            if (dy) { const float t = a.psiE_in.p[2][q] + Qy; v.yz += cze * t; if (wr) a.psiE_out.p[2][q] = bze * t; }
+        }
+        const int lxe = pin ? e_layer(cx, ip) : -1;
+        if (lxe >= 0) {
+            const long q = xslab(g, lxe, jj, oz);
+            const float bxe = cx.be[lxe], cxe = cx.ce[lxe];
+            if (dy) { const float t = a.psiE_in.p[3][q] - Qy; v.yx -= cxe * t; if (wr) a.psiE_out.p[3][q] = bxe * t; }
+            if (dz) { const float t = a.psiE_in.p[5][q] + Qz; v.zx += cxe * t; if (wr) a.psiE_out.p[5][q] = bxe * t; }
+        }
+        if (!dx) { v.xy = 0.f; v.xz = 0.f; bx = 0.f; }
+        if (!dy) { v.yz = 0.f; v.yx = 0.f; by = 0.f; }
+        if (!dz) { v.zx = 0.f; v.zy = 0.f; bz = 0.f; }
+        v.bx = bx; v.by = by; v.bz = bz;
     };
-    // adjoint of H^{n+1/2} at plane ip (all lanes), given V at ip and V at ip+1
-    auto htot = [&](int ip, float Vxy, float Vxz, float Vyz, float Vyx, float Vzx, float Vzy, float nVzx, float nVyx,
-                    float &hx, float &hy, float &hz) {
+    // V at j+1 (Vxy of ex, Vzy of ez) for the top row, which has no row above in the tile
+    auto vtop = [&](int ip, float &xy, float &zy) {
+        xy = zy = 0.f;
+        if (row == TJ - 1 && j + 1 < NY && k < NZ) {
+            const long p = (long)ip * sxy + cidx + sz;
+            float t0, t1;
+            adj_v(a, 0, ip, j + 1, kk, Ex[p], false, xy, t0, t1);
+            adj_v(a, 2, ip, j + 1, kk, Ez[p], false, t0, zy, t1);
+        }
+    };
+    // W weights from Hhat_tot at plane ip; returns the H^{n-1/2} adjoints in hx, hy, hz
+    auto wcell = [&](int ip, float &hx, float &hy, float &hz, bool wr, float &wxz, float &wxy, float &wyx, float &wyz,
+                     float &wzy, float &wzx) {
+        const bool dx = hjx && ip >= 0 && ip < NX, dy = hjy && ip >= 0 && ip < NX - 1, dz = hjz && ip >= 0 && ip < NX - 1;
+        const float ikx_h = cx.ik_h[min(max(ip, 0), NX - 1)];
+        const float Gx = Che * hx, Gy = Che * hy, Gz = Che * hz;
+        wxz = Gx * ikz_h; wxy = Gx * iky_h;
+        wyx = Gy * ikx_h; wyz = Gy * ikz_h;
+        wzy = Gz * iky_h; wzx = Gz * ikx_h;
+        const bool pin = pjk && ip >= 0 && ip < NX - 1;
+        if (pin && zH && lzh >= 0) {
+            const long q = zslab(g, Lzh, ip, jj, lzh);
+            if (dx) { const float t = a.psiH_in.p[0][q] + Gx; wxz += czh * t; if (wr) a.psiH_out.p[0][q] = bzh * t; }
+            if (dy) { const float t = a.psiH_in.p[2][q] - Gy; wyz -= czh * t; if (wr) a.psiH_out.p[2][q] = bzh * t; }
+        }
+        if (pin && lyh >= 0) {
+            const long q = yslab(g, Lyh, ip, lyh, oz);
+            if (dx) { const float t = a.psiH_in.p[1][q] - Gx; wxy -= cyh * t; if (wr) a.psiH_out.p[1][q] = byh * t; }
+            if (dz) { const float t = a.psiH_in.p[5][q] + Gz; wzy += cyh * t; if (wr) a.psiH_out.p[5][q] = byh * t; }
+        }
+        const int lxh = pin ? h_layer(cx, ip) : -1;
+        if (lxh >= 0) {
+            const long q = xslab(g, lxh, jj, oz);
+            const float bxh = cx.bh[lxh], cxh = cx.ch[lxh];
+            if (dy) { const float t = a.psiH_in.p[3][q] + Gy; wyx += cxh * t; if (wr) a.psiH_out.p[3][q] = bxh * t; }
+            if (dz) { const float t = a.psiH_in.p[4][q] - Gz; wzx -= cxh * t; if (wr) a.psiH_out.p[4][q] = bxh * t; }
+        }
+        if (!dx) { wxz = 0.f; wxy = 0.f; }
+        if (!dy) { wyx = 0.f; wyz = 0.f; }
+        if (!dz) { wzy = 0.f; wzx = 0.f; }
+        hx = dx ? Chh * hx : 0.f;
+        hy = dy ? Chh * hy : 0.f;
+        hz = dz ? Chh * hz : 0.f;
+    };
+    auto ld3 = [&](int ip, float &x, float &y, float &z) {
         const long p = (long)ip * sxy + cidx;
-        float d0, d1, d2, t;
-        // neighbours at k+1 (ex, ey) and j+1 (ex, ez), recomputed from the read-only adjoint level
-        float vyz1, vxz1, vzy1, vxy1;
-        adj_v(a, 0, ip, jj, kk + 1, Ex[p + 1], false, d0, vxz1, t);
-        adj_v(a, 1, ip, jj, kk + 1, Ey[p + 1], false, vyz1, d1, t);
-        adj_v(a, 0, ip, jj + 1, kk, Ex[p + sz], false, vxy1, d2, t);
-        adj_v(a, 2, ip, jj + 1, kk, Ez[p + sz], false, d0, vzy1, t);
-        hx = a.Hin[0][p] + Vyz - vyz1 - Vzy + vzy1;
-        hy = a.Hin[1][p] - Vxz + vxz1 + Vzx - nVzx;
-        hz = a.Hin[2][p] + Vxy - vxy1 - Vyx + nVyx;
+        x = Ex[p]; y = Ey[p]; z = Ez[p];
     };
 
-    vplane(i0, own, vxy, vxz, vyz, vyx, vzx, vzy, bx, by, bz);
+    AdjV vc, vn;
+    float e0x, e0y, e0z;
+    ld3(i0, e0x, e0y, e0z);
+    vcell(i0, e0x, e0y, e0z, own, vc);
+    float wzxP = 0.f, wyxP = 0.f;
+    sV[0][0][row][lane] = vc.xy;
+    sV[0][1][row][lane] = vc.zy;
     if (i0 > 0) {
-        float pxy, pxz, pyz, pyx, pzx, pzy, t0, t1, t2;
-        vplane(i0 - 1, false, pxy, pxz, pyz, pyx, pzx, pzy, t0, t1, t2);
-        float hx, hy, hz, wa, wb;
-        htot(i0 - 1, pxy, pxz, pyz, pyx, pzx, pzy, vzx, vyx, hx, hy, hz);
-        adj_w(a, 1, i0 - 1, jj, kk, hy, false, wyxP, wb);
-        adj_w(a, 2, i0 - 1, jj, kk, hz, false, wa, wzxP);
+        // Hhat_tot and W of plane i0-1 (values only): its V at j+1 through LDS, at i+1 = vc
+        AdjV vp;
+        float px, py, pz;
+        ld3(i0 - 1, px, py, pz);
+        vcell(i0 - 1, px, py, pz, false, vp);
+        sV[1][0][row][lane] = vp.xy;
+        sV[1][1][row][lane] = vp.zy;
+        __syncthreads();
+        float xyJ, zyJ;
+        if (row < TJ - 1) { xyJ = sV[1][0][row + 1][lane]; zyJ = sV[1][1][row + 1][lane]; }
+        else vtop(i0 - 1, xyJ, zyJ);
+        const float yzK = from_upper_lane(vp.yz), xzK = from_upper_lane(vp.xz);
+        const long p = (long)(i0 - 1) * sxy + cidx;
+        float hx = a.Hin[0][p] + vp.yz - yzK - vp.zy + zyJ;
+        float hy = a.Hin[1][p] - vp.xz + xzK + vp.zx - vc.zx;
+        float hz = a.Hin[2][p] + vp.xy - xyJ - vp.yx + vc.yx;
+        float t0, t1, t2, t3;
+        wcell(i0 - 1, hx, hy, hz, false, t0, t1, wyxP, t2, t3, wzxP);
     }
-    int buf = 0;
+    __syncthreads();
     for (int i = i0; i < i1; i++) {
+        const int rb = (i - i0) & 1;
         const long p = (long)i * sxy + cidx;
-        float nxy, nxz, nyz, nyx, nzx, nzy, nbx, nby, nbz;
-        vplane(i + 1, own && i + 1 < i1, nxy, nxz, nyz, nyx, nzx, nzy, nbx, nby, nbz);
-        float hx, hy, hz;
-        htot(i, vxy, vxz, vyz, vyx, vzx, vzy, nzx, nyx, hx, hy, hz);
+        const float hinx = a.Hin[0][p], hiny = a.Hin[1][p], hinz = a.Hin[2][p];
+        float nx_, ny_, nz_;
+        ld3(i + 1, nx_, ny_, nz_);
+        vcell(i + 1, nx_, ny_, nz_, own && i + 1 < i1, vn);
+        float xyJ, zyJ;
+        if (row < TJ - 1) { xyJ = sV[rb][0][row + 1][lane]; zyJ = sV[rb][1][row + 1][lane]; }
+        else vtop(i, xyJ, zyJ);
+        const float yzK = from_upper_lane(vc.yz), xzK = from_upper_lane(vc.xz);
+        float hx = hinx + vc.yz - yzK - vc.zy + zyJ;
+        float hy = hiny - vc.xz + xzK + vc.zx - vn.zx;
+        float hz = hinz + vc.xy - xyJ - vc.yx + vn.yx;
         float wxz, wxy, wyx, wyz, wzy, wzx;
-        const float hxi = adj_w(a, 0, i, jj, kk, hx, own, wxz, wxy);
-        const float hyi = adj_w(a, 1, i, jj, kk, hy, own, wyx, wyz);
-        const float hzi = adj_w(a, 2, i, jj, kk, hz, own, wzy, wzx);
-        if (own) { a.Hout[0][p] = hxi; a.Hout[1][p] = hyi; a.Hout[2][p] = hzi; }
-        sW[buf][0][row][lane] = wyz;
-        sW[buf][1][row][lane] = wxz;
-        sW[buf][2][row][lane] = wzy;
-        sW[buf][3][row][lane] = wxy;
+        wcell(i, hx, hy, hz, own, wxz, wxy, wyx, wyz, wzy, wzx);
+        if (own) { a.Hout[0][p] = hx; a.Hout[1][p] = hy; a.Hout[2][p] = hz; }
+        sW[rb][0][row][lane] = wzy;
+        sW[rb][1][row][lane] = wxy;
+        sV[rb ^ 1][0][row][lane] = vn.xy;
+        sV[rb ^ 1][1][row][lane] = vn.zy;
+        const float wyzK = from_lower_lane(wyz), wxzK = from_lower_lane(wxz);
         __syncthreads();
         if (own) {
-            const float wyz_k = sW[buf][0][row][lane - 1], wxz_k = sW[buf][1][row][lane - 1];
-            const float wzy_j = sW[buf][2][row - 1][lane], wxy_j = sW[buf][3][row - 1][lane];
-            const bool dx = e_dom(g, 0, i, j, k), dy = e_dom(g, 1, i, j, k), dz = e_dom(g, 2, i, j, k);
-            a.Eout[0][p] = dx ? bx - wyz_k + wyz + wzy_j - wzy : 0.f;
-            a.Eout[1][p] = dy ? by + wxz_k - wxz - wzxP + wzx : 0.f;
-            a.Eout[2][p] = dz ? bz - wxy_j + wxy + wyxP - wyx : 0.f;
+            const float wzyJ = sW[rb][0][row - 1][lane], wxyJ = sW[rb][1][row - 1][lane];
+            const bool dx = ejx && i < NX - 1, dy = ejy && i > 0 && i < NX - 1, dz = ejz && i > 0 && i < NX - 1;
+            a.Eout[0][p] = dx ? vc.bx - wyzK + wyz + wzyJ - wzy : 0.f;
+            a.Eout[1][p] = dy ? vc.by + wxzK - wxz - wzxP + wzx : 0.f;
+            a.Eout[2][p] = dz ? vc.bz - wxyJ + wxy + wyxP - wyx : 0.f;
         }
         wzxP = wzx; wyxP = wyx;
-        vxy = nxy; vxz = nxz; vyz = nyz; vyx = nyx; vzx = nzx; vzy = nzy;
-        bx = nbx; by = nby; bz = nbz;
-        buf ^= 1;
+        vc = vn;
     }
 }
 
@@ -2186,7 +2308,7 @@ hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
 
 hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st) {
     if (nbox < 1 || nbox > 4 || ichunk < 1 || (tj != 4 && tj != 8 && tj != 16)) return hipErrorInvalidValue;
-    const int nkt = (a.g.NZ + 62) / 63;
+    const int nkt = (a.g.NZ + 61) / 62;
     Pieces pc{};
     pc.n = nbox;
     long total = 0;
