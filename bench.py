@@ -287,7 +287,7 @@ def sweep_bench(steps, seed=20240601):
         "one_by_one_mcells_per_s": round(cells * steps / t_alone / 1e6, 1),
         "speedup": round(t_alone / t_batch, 3),
         "ms_per_step_batched": round(t_batch * 1e3 / steps, 3),
-        "tiles_tj_ichunk": [list(t) for t in tiles],
+        "tiles_path_a_c": [list(t) for t in tiles],   # path 1: rows per thread, x chunk; 0: block rows, x chunk
         "autotune_s": round(t_tune, 2),
         "create_s": round(t_create, 2),
     }

@@ -3,18 +3,20 @@
 // the reference runs one simulation per process (main.cu:2724-2934); each
 // batched step is that loop body for every simulation at once.
 //
-// Per step every kernel runs ONCE for all simulations: the 1D lines (one
-// block each), the general fused field update with TFSF inside (the blocks of
-// every simulation in one grid, one launch per tile height), the dispersive
-// boxes and, on sampling steps, the monitor samples.  Per-simulation
-// arguments live in device memory, one array per time parity, and each block
-// finds its simulation in a block map.  The only arguments that change from
-// one sampling period to the next (the step index of the source pulse, the
-// ring row of the sample) are read from a two-int device counter that the
-// last launch of the period advances; so the launches of one period are
-// captured once in a HIP graph and replayed.  The K-sample DFT contraction
-// runs between replays (every dft_block samples) with the simulation's own
-// launches.
+// Per step every kernel runs ONCE for all simulations that use it: the 1D
+// lines (one block each); the field update of fdtd_step's fast path (the
+// row-blocked kernel on clean boxes and side slabs, one launch per variant,
+// block height and rows per thread; the general kernel on corner columns, or
+// on whole grids where the row path does not apply, one launch per tile
+// height); the TFSF fix-up; the dispersive boxes and, on sampling steps, the
+// monitor samples.  Per-simulation arguments live in device memory, one array
+// per time parity, and each block finds its simulation in a block map.  The
+// only arguments that change from one sampling period to the next (the step
+// index of the source pulse, the ring row of the sample) are read from a
+// two-int device counter that the last launch of the period advances; so the
+// launches of one period are captured once in a HIP graph and replayed.  The
+// K-sample DFT contraction runs between replays (every dft_block samples)
+// with the simulation's own launches.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,22 +29,36 @@
 
 using namespace fdtd_host;
 
+namespace {
+
+// One batched launch of a step: its kernel, shape and block map.
+struct Launch {
+    enum Type { Rows = 0, Fused = 1, Tfsf = 2, Disp = 3, Sample = 4 };
+    int type = 0;
+    int a = 0, b = 0, c = 0;             // Rows: W, R, variant; Fused: tile rows
+    std::vector<unsigned> map;
+    unsigned *d = nullptr;
+    bool same(const Launch &o) const { return type == o.type && a == o.a && b == o.b && c == o.c && map == o.map; }
+};
+
+}  // namespace
+
 struct fdtd_batch {
     std::vector<fdtd_sim *> sims;
     int flags = 0;
     std::string err = "no error";
     hipStream_t stream = nullptr;
-    std::vector<int> tj, ichunk;            // tile per simulation
+    // tile per simulation: row path (rows per thread, x chunk of the clean box)
+    // or general kernel (tile rows, x chunk)
+    std::vector<int> rowpath, ta, tb;
     // per-simulation arguments, [t & 1]
     FusedItem *fused[2] = {};
+    RowsArgs *rows[2][3] = {};
     LineDev *line[2] = {};
     DispDev *disp[2] = {};
     SampleItem *samp[2] = {};
     int *dyn = nullptr;                     // {t of the next step, ring row of the next sample}
-    // block maps: fused per tile height (4, 8, 16), dispersive, samples
-    unsigned *fmap[3] = {}, *dmap = nullptr, *smap = nullptr;
-    std::vector<unsigned> h_fmap[3], h_dmap, h_smap;
-    size_t cap_fmap[3] = {}, cap_dmap = 0, cap_smap = 0;
+    std::vector<Launch> launches;           // field update, TFSF fix-up, dispersive, samples (in order)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
@@ -52,7 +68,6 @@ struct fdtd_batch {
 namespace {
 
 thread_local std::string g_batch_err = "no error";
-const int kTileRows[3] = {4, 8, 16};
 
 int berr(fdtd_batch *b, int code, const std::string &m) {
     if (b) b->err = m;
@@ -76,14 +91,76 @@ int balloc(fdtd_batch *b, T **p, size_t n) {
     return FDTD_OK;
 }
 
-int tile_index(int tj) { return tj == 4 ? 0 : (tj == 8 ? 1 : (tj == 16 ? 2 : -1)); }
+// Runs f with the handle's parities set as for a step whose index has parity par.
+template <typename F>
+auto at_parity(fdtd_sim *s, int par, F f) {
+    const int flip = (s->t & 1) ^ par;
+    s->p ^= flip; s->pp ^= flip; s->le ^= flip;
+    auto r = f();
+    s->p ^= flip; s->pp ^= flip; s->le ^= flip;
+    return r;
+}
+
+// General-kernel arguments at the handle's current parity (fdtd_step's StepDev).
+StepDev fused_step(fdtd_sim *s, bool tfsf_inside) {
+    StepDev d = make_step(s, true);
+    d.tfsf_on = tfsf_inside ? 1 : 0;
+    const int q = s->p, pp = s->pp;
+    for (int c = 0; c < 3; c++) {
+        d.Ein[c] = s->E[q][c]; d.Hin[c] = s->H[q][c];
+        d.Eout[c] = s->E[1 - q][c]; d.Hout[c] = s->H[1 - q][c];
+    }
+    for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
+    return d;
+}
+
+// General kernel over the whole grid (TFSF inside), tile rows tj and x chunk ic.
+FusedItem fused_full(fdtd_sim *s, int tj, int ic) {
+    FusedItem it{};
+    it.s = fused_step(s, true);
+    it.ichunk = ic;
+    fused_batch_blocks(s->g, tj, ic, &it.pc, &it.nkt);
+    return it;
+}
+
+// Row path: the corner columns on the general kernel (TFSF left to the fix-up).
+FusedItem fused_corners(fdtd_sim *s) {
+    FusedItem it{};
+    it.s = fused_step(s, false);
+    it.ichunk = 2;
+    if (s->ncorner) fused_pieces(s->g, s->tj, 2, s->cbox4, s->ncorner, &it.pc, &it.nkt);
+    return it;
+}
+
+// Row path: the clean box (variant 0, R rows, x chunk ic) or the x / y side slabs (variants 1, 2).
+RowsArgs rows_item(fdtd_sim *s, int variant, int R, int ic) {
+    RowsArgs r = make_rows(s);
+    r.xcd = 0;
+    if (variant == 0) {
+        r.ichunk = ic;
+        r.npiece = 1;
+        std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
+        rows_prepare(r, R, 0);
+    } else {
+        const int n = variant == 1 ? s->nxbox : s->nybox;
+        r.ichunk = s->schunk;
+        r.npiece = n;
+        if (n) {
+            std::memcpy(r.box, variant == 1 ? s->xbox : s->ybox, sizeof(r.box));
+            rows_prepare(r, 2, variant);
+        }
+    }
+    return r;
+}
+
+long rows_blocks(const RowsArgs &r) { return r.npiece ? r.start[r.npiece] : 0; }
 
 // ---------------------------------------------------------------- autotuner
 
-// Tile per grid size, remembered for the process and optionally in the text
-// file named by FDTD_TUNE_FILE (lines "NX NY NZ tj ichunk").
+// Tile per grid size and path, remembered for the process and optionally in
+// the text file named by FDTD_TUNE_FILE (lines "NX NY NZ path a b").
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tune;
+std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_tune;
 bool g_tune_loaded = false;
 
 void tune_load() {
@@ -92,9 +169,9 @@ void tune_load() {
     const char *f = std::getenv("FDTD_TUNE_FILE");
     if (!f) return;
     std::ifstream in(f);
-    int nx, ny, nz, tj, ic;
-    while (in >> nx >> ny >> nz >> tj >> ic)
-        if (tile_index(tj) >= 0 && ic > 0) g_tune[{nx, ny, nz}] = {tj, ic};
+    int nx, ny, nz, path, ta, tb;
+    while (in >> nx >> ny >> nz >> path >> ta >> tb)
+        if (ta > 0 && tb > 0) g_tune[{nx, ny, nz, path}] = {ta, tb};
 }
 
 void tune_save() {
@@ -103,42 +180,24 @@ void tune_save() {
     std::ofstream out(f);
     for (auto &kv : g_tune)
         out << std::get<0>(kv.first) << ' ' << std::get<1>(kv.first) << ' ' << std::get<2>(kv.first) << ' '
-            << kv.second.first << ' ' << kv.second.second << '\n';
+            << std::get<3>(kv.first) << ' ' << kv.second.first << ' ' << kv.second.second << '\n';
 }
 
-// Fused-kernel arguments of simulation s for the step whose index has parity
-// par (the handle's parities flip with every fused step).
-FusedItem fused_item(fdtd_sim *s, int par, int tj, int ichunk) {
-    const int flip = (s->t & 1) ^ par;
-    const int q = s->p ^ flip, pp = s->pp ^ flip;
-    s->le ^= flip;
-    StepDev d = make_step(s, true);
-    s->le ^= flip;
-    for (int c = 0; c < 3; c++) {
-        d.Ein[c] = s->E[q][c]; d.Hin[c] = s->H[q][c];
-        d.Eout[c] = s->E[1 - q][c]; d.Hout[c] = s->H[1 - q][c];
-    }
-    for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
-    FusedItem it{};
-    it.s = d;
-    it.ichunk = ichunk;
-    fused_batch_blocks(s->g, tj, ichunk, &it.pc, &it.nkt);
-    return it;
-}
-
-// Times every candidate tile on simulation s and keeps the fastest per cell.
-// The kernel writes only the handle's spare time level and psi_E (in place),
-// so psi_E is saved first and restored after; the live state is untouched.
-// Each candidate runs as a batched launch of `rep` copies of the simulation's
-// blocks (identical results written by each copy), so that small grids are
-// timed on a filled device as they run in a batch.
-int autotune(fdtd_batch *b, fdtd_sim *s, int *tj_out, int *ic_out) {
-    const auto key = std::make_tuple(s->g.NX, s->g.NY, s->g.NZ);
+// Times every candidate tile of simulation s's dominant launch and keeps the
+// fastest per cell: on the row path the clean box (rows per thread 2 or 3 by
+// x chunk 8, 16 or 32), else the general kernel over the grid (tile rows 4, 8
+// or 16 by x chunk 8 to 64).  The kernels write only the handle's spare time
+// level and psi_E (in place), so psi_E is saved first and restored after; the
+// live state is untouched.  Each candidate runs as a batched launch of `rep`
+// copies of the simulation's blocks (identical results written by each copy)
+// so that small grids are timed on a filled device, as they run in a batch.
+int autotune(fdtd_batch *b, fdtd_sim *s, bool rowpath, int *a_out, int *b_out) {
+    const auto key = std::make_tuple(s->g.NX, s->g.NY, s->g.NZ, rowpath ? 1 : 0);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         tune_load();
         auto it = g_tune.find(key);
-        if (it != g_tune.end()) { *tj_out = it->second.first; *ic_out = it->second.second; return FDTD_OK; }
+        if (it != g_tune.end()) { *a_out = it->second.first; *b_out = it->second.second; return FDTD_OK; }
     }
     BCHK(b, hipStreamSynchronize(s->stream));
     std::vector<void *> tmp;
@@ -154,46 +213,63 @@ int autotune(fdtd_batch *b, fdtd_sim *s, int *tj_out, int *ic_out) {
             return berr(b, FDTD_ERR_DEVICE, "autotune: psi copy failed");
         }
     }
-    FusedItem *item_d = nullptr;
+    void *item_d = nullptr;
     unsigned *map_d = nullptr;
     const size_t map_cap = 1u << 20;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = FDTD_OK;
-    if (hipMalloc((void **)&item_d, sizeof(FusedItem)) != hipSuccess) rc = FDTD_ERR_ALLOC;
+    if (hipMalloc(&item_d, std::max(sizeof(FusedItem), sizeof(RowsArgs))) != hipSuccess) rc = FDTD_ERR_ALLOC;
     else tmp.push_back(item_d);
     if (!rc && hipMalloc((void **)&map_d, map_cap * sizeof(unsigned)) != hipSuccess) rc = FDTD_ERR_ALLOC;
     else if (!rc) tmp.push_back(map_d);
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = FDTD_ERR_DEVICE;
     double best = 1e30;
-    int btj = 8, bic = 32;
-    const int chunks[4] = {8, 16, 32, 64};
-    for (int ti = 0; ti < 3 && !rc; ti++)
-        for (int ci = 0; ci < 4 && !rc; ci++) {
-            const int tj = kTileRows[ti], ic = chunks[ci];
-            const FusedItem it = fused_item(s, s->t & 1, tj, ic);
-            const long nb = it.pc.start[1];
-            if (nb <= 0 || nb >= (1L << kBatchLocalBits)) continue;
-            const int rep = (int)std::max(1L, std::min(16L, (4096 + nb - 1) / nb));
-            std::vector<unsigned> map;
-            for (int r = 0; r < rep; r++)
-                for (long L = 0; L < nb; L++) map.push_back((unsigned)L);
-            if (map.size() > map_cap) continue;
-            if (hipStreamSynchronize(b->stream) != hipSuccess ||
-                hipMemcpy(item_d, &it, sizeof(it), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(map_d, map.data(), map.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
-                rc = FDTD_ERR_DEVICE;
-                break;
-            }
-            const int nblk = (int)map.size();
-            bool ok = launch_fused_batch(item_d, map_d, nblk, tj, b->stream) == hipSuccess &&
-                      hipEventRecord(e0, b->stream) == hipSuccess;
-            for (int r = 0; r < 3 && ok; r++) ok = launch_fused_batch(item_d, map_d, nblk, tj, b->stream) == hipSuccess;
-            ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-            float ms = 0;
-            if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = FDTD_ERR_DEVICE; break; }
-            const double per = ms / (3.0 * rep);
-            if (per < best) { best = per; btj = tj; bic = ic; }
+    int ba = rowpath ? s->rows : 8, bb = rowpath ? s->cchunk : 32;
+    std::vector<std::pair<int, int>> cands;
+    if (rowpath) {
+        for (int R : {2, 3})
+            for (int ic : {8, 16, 32}) cands.push_back({R, ic});
+    } else {
+        for (int tj : {4, 8, 16})
+            for (int ic : {8, 16, 32, 64}) cands.push_back({tj, ic});
+    }
+    const int W = rows_waves(s->g.NZ);
+    for (size_t ci = 0; ci < cands.size() && !rc; ci++) {
+        const int ca = cands[ci].first, cb = cands[ci].second;
+        long nb;
+        hipError_t he = hipSuccess;
+        if (rowpath) {
+            const RowsArgs r = rows_item(s, 0, ca, cb);
+            nb = rows_blocks(r);
+            if (hipStreamSynchronize(b->stream) != hipSuccess || hipMemcpy(item_d, &r, sizeof(r), hipMemcpyHostToDevice) != hipSuccess)
+                he = hipErrorUnknown;
+        } else {
+            const FusedItem it = fused_full(s, ca, cb);
+            nb = it.pc.start[1];
+            if (hipStreamSynchronize(b->stream) != hipSuccess || hipMemcpy(item_d, &it, sizeof(it), hipMemcpyHostToDevice) != hipSuccess)
+                he = hipErrorUnknown;
         }
+        if (he != hipSuccess) { rc = FDTD_ERR_DEVICE; break; }
+        if (nb <= 0 || nb >= (1L << kBatchLocalBits)) continue;
+        const int rep = (int)std::max(1L, std::min(16L, (4096 + nb - 1) / nb));
+        std::vector<unsigned> map;
+        for (int r = 0; r < rep; r++)
+            for (long L = 0; L < nb; L++) map.push_back((unsigned)L);
+        if (map.size() > map_cap) continue;
+        if (hipMemcpy(map_d, map.data(), map.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) { rc = FDTD_ERR_DEVICE; break; }
+        const int nblk = (int)map.size();
+        auto launch = [&]() {
+            return rowpath ? launch_rows_batch((const RowsArgs *)item_d, map_d, nblk, W, ca, 0, b->stream)
+                           : launch_fused_batch((const FusedItem *)item_d, map_d, nblk, ca, b->stream);
+        };
+        bool ok = launch() == hipSuccess && hipEventRecord(e0, b->stream) == hipSuccess;
+        for (int r = 0; r < 3 && ok; r++) ok = launch() == hipSuccess;
+        ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float ms = 0;
+        if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = FDTD_ERR_DEVICE; break; }
+        const double per = ms / (3.0 * rep);
+        if (per < best) { best = per; ba = ca; bb = cb; }
+    }
     // restore psi_E whatever happened above
     for (int a = 0; a < 6; a++)
         if (hipMemcpyAsync(s->psiE[a], saved[a], s->esz[a] * sizeof(float), hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
@@ -203,34 +279,50 @@ int autotune(fdtd_batch *b, fdtd_sim *s, int *tj_out, int *ic_out) {
     if (e1) (void)hipEventDestroy(e1);
     cleanup();
     if (rc) return berr(b, rc, "autotune failed");
-    *tj_out = btj;
-    *ic_out = bic;
+    *a_out = ba;
+    *b_out = bb;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {btj, bic};
+    g_tune[key] = {ba, bb};
     tune_save();
     return FDTD_OK;
 }
 
 // ---------------------------------------------------------------- launches
 
+Launch &group(std::vector<Launch> &ls, int type, int a, int b, int c) {
+    for (Launch &l : ls)
+        if (l.type == type && l.a == a && l.b == b && l.c == c) return l;
+    ls.push_back(Launch{});
+    Launch &l = ls.back();
+    l.type = type; l.a = a; l.b = b; l.c = c;
+    return l;
+}
+
+void add_blocks(Launch &l, int sim, long n) {
+    for (long L = 0; L < n; L++) l.map.push_back(((unsigned)sim << kBatchLocalBits) | (unsigned)L);
+}
+
 // Uploads every per-simulation argument for both parities and rebuilds the
-// block maps; a changed map invalidates the captured graph.
+// launch list; a changed list invalidates the captured graph.
 int upload_items(fdtd_batch *b) {
     const int n = (int)b->sims.size();
-    std::vector<unsigned> fm[3], dm, sm;
+    std::vector<Launch> ls;
     for (int par = 0; par < 2; par++) {
         std::vector<FusedItem> fi(n);
+        std::vector<RowsArgs> ri[3] = {std::vector<RowsArgs>(n), std::vector<RowsArgs>(n), std::vector<RowsArgs>(n)};
         std::vector<LineDev> li(n);
         std::vector<DispDev> di(n);
         std::vector<SampleItem> si(n);
         for (int i = 0; i < n; i++) {
             fdtd_sim *s = b->sims[i];
-            const int flip = (s->t & 1) ^ par;
-            fi[i] = fused_item(s, par, b->tj[i], b->ichunk[i]);
-            const int q = s->p ^ flip;
-            s->le ^= flip;
-            li[i] = make_line(s);
-            s->le ^= flip;
+            if (b->rowpath[i]) {
+                fi[i] = at_parity(s, par, [&] { return fused_corners(s); });
+                for (int v = 0; v < 3; v++) ri[v][i] = at_parity(s, par, [&] { return rows_item(s, v, b->ta[i], b->tb[i]); });
+            } else {
+                fi[i] = at_parity(s, par, [&] { return fused_full(s, b->ta[i], b->tb[i]); });
+            }
+            li[i] = at_parity(s, par, [&] { return make_line(s); });
+            const int q = at_parity(s, par, [&] { return s->p; });
             di[i] = make_disp(s, 1 - q);
             SampleItem &S = si[i];
             S = SampleItem{};
@@ -239,37 +331,42 @@ int upload_items(fdtd_batch *b) {
             for (int c = 0; c < 3; c++) { S.E[c] = s->E[1 - q][c]; S.H[c] = s->H[1 - q][c]; }
             S.X = s->X;
             S.mtot = s->mtot;
-            if (par == 0) {
-                const long nb = fi[i].pc.start[1];
-                const int g = tile_index(b->tj[i]);
-                for (long L = 0; L < nb; L++) fm[g].push_back(((unsigned)i << kBatchLocalBits) | (unsigned)L);
-                const long nd = (s->nbox + 255) / 256;
-                for (long L = 0; L < nd; L++) dm.push_back(((unsigned)i << kBatchLocalBits) | (unsigned)L);
-                const long ns = (s->fd.mE + 255) / 256;
-                for (long L = 0; L < ns; L++) sm.push_back(((unsigned)i << kBatchLocalBits) | (unsigned)L);
+        }
+        if (par == 0) {
+            // field update: row-blocked launches, then the general kernel; TFSF fix-up; dispersive; samples
+            for (int i = 0; i < n; i++) {
+                if (!b->rowpath[i]) continue;
+                const int W = rows_waves(b->sims[i]->g.NZ);
+                add_blocks(group(ls, Launch::Rows, W, b->ta[i], 0), i, rows_blocks(ri[0][i]));
+                if (rows_blocks(ri[1][i])) add_blocks(group(ls, Launch::Rows, W, 2, 1), i, rows_blocks(ri[1][i]));
+                if (rows_blocks(ri[2][i])) add_blocks(group(ls, Launch::Rows, W, 2, 2), i, rows_blocks(ri[2][i]));
             }
+            for (int i = 0; i < n; i++) {
+                const int tj = b->rowpath[i] ? b->sims[i]->tj : b->ta[i];
+                const long nb = b->rowpath[i] && !b->sims[i]->ncorner ? 0 : fi[i].pc.start[fi[i].pc.n];
+                if (nb) add_blocks(group(ls, Launch::Fused, tj, 0, 0), i, nb);
+            }
+            for (int i = 0; i < n; i++)
+                if (b->rowpath[i]) add_blocks(group(ls, Launch::Tfsf, 0, 0, 0), i, tfsf_fix_blocks(fi[i].s));
+            for (int i = 0; i < n; i++) add_blocks(group(ls, Launch::Disp, 0, 0, 0), i, (b->sims[i]->nbox + 255) / 256);
+            for (int i = 0; i < n; i++) add_blocks(group(ls, Launch::Sample, 0, 0, 0), i, (b->sims[i]->fd.mE + 255) / 256);
         }
         BCHK(b, hipMemcpy(b->fused[par], fi.data(), n * sizeof(FusedItem), hipMemcpyHostToDevice));
+        for (int v = 0; v < 3; v++) BCHK(b, hipMemcpy(b->rows[par][v], ri[v].data(), n * sizeof(RowsArgs), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->line[par], li.data(), n * sizeof(LineDev), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->disp[par], di.data(), n * sizeof(DispDev), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->samp[par], si.data(), n * sizeof(SampleItem), hipMemcpyHostToDevice));
     }
-    auto put = [&](std::vector<unsigned> &h, unsigned *&d, size_t &cap, std::vector<unsigned> &v) -> int {
-        if (v == h && d) return FDTD_OK;
-        b->graph_valid = false;
-        if (v.size() > cap || !d) {
-            // maps are re-allocated only when they grow; old buffers stay owned by the batch
-            if (balloc(b, &d, v.size())) return FDTD_ERR_ALLOC;
-            cap = std::max<size_t>(v.size(), 1);
-        }
-        if (!v.empty()) BCHK(b, hipMemcpy(d, v.data(), v.size() * sizeof(unsigned), hipMemcpyHostToDevice));
-        h = v;
-        return FDTD_OK;
-    };
-    for (int g = 0; g < 3; g++)
-        if (int rc = put(b->h_fmap[g], b->fmap[g], b->cap_fmap[g], fm[g])) return rc;
-    if (int rc = put(b->h_dmap, b->dmap, b->cap_dmap, dm)) return rc;
-    if (int rc = put(b->h_smap, b->smap, b->cap_smap, sm)) return rc;
+    bool same = ls.size() == b->launches.size();
+    for (size_t q = 0; same && q < ls.size(); q++) same = ls[q].same(b->launches[q]);
+    if (same) return FDTD_OK;
+    b->graph_valid = false;
+    for (Launch &l : ls) {
+        if (l.map.empty()) continue;
+        if (int rc = balloc(b, &l.d, l.map.size())) return rc;
+        BCHK(b, hipMemcpy(l.d, l.map.data(), l.map.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
+    b->launches = std::move(ls);
     return FDTD_OK;
 }
 
@@ -278,10 +375,18 @@ int upload_items(fdtd_batch *b) {
 int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     const int n = (int)b->sims.size();
     BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, b->stream));
-    for (int g = 0; g < 3; g++)
-        BCHK(b, launch_fused_batch(b->fused[par], b->fmap[g], (int)b->h_fmap[g].size(), kTileRows[g], b->stream));
-    BCHK(b, launch_disp_batch(b->disp[par], b->dmap, (int)b->h_dmap.size(), b->stream));
-    if (sidx >= 0) BCHK(b, launch_sample_batch(b->samp[par], b->smap, (int)b->h_smap.size(), b->dyn, sidx, b->stream));
+    for (const Launch &l : b->launches) {
+        const int nb = (int)l.map.size();
+        switch (l.type) {
+            case Launch::Rows: BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, b->stream)); break;
+            case Launch::Fused: BCHK(b, launch_fused_batch(b->fused[par], l.d, nb, l.a, b->stream)); break;
+            case Launch::Tfsf: BCHK(b, launch_tfsf_fix_batch(b->fused[par], l.d, nb, b->stream)); break;
+            case Launch::Disp: BCHK(b, launch_disp_batch(b->disp[par], l.d, nb, b->stream)); break;
+            default:
+                if (sidx >= 0) BCHK(b, launch_sample_batch(b->samp[par], l.d, nb, b->dyn, sidx, b->stream));
+                break;
+        }
+    }
     return FDTD_OK;
 }
 
@@ -383,21 +488,34 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(berr(b, FDTD_ERR_DEVICE, "hipStreamCreate failed"));
     int rc;
-    for (int par = 0; par < 2; par++)
+    for (int par = 0; par < 2; par++) {
         if ((rc = balloc(b, &b->fused[par], n)) || (rc = balloc(b, &b->line[par], n)) || (rc = balloc(b, &b->disp[par], n)) ||
             (rc = balloc(b, &b->samp[par], n)))
             return fail(rc);
+        for (int v = 0; v < 3; v++)
+            if ((rc = balloc(b, &b->rows[par][v], n))) return fail(rc);
+    }
     if ((rc = balloc(b, &b->dyn, 2))) return fail(rc);
-    b->tj.assign(n, 8);
-    b->ichunk.assign(n, 32);
-    if (!(flags & FDTD_BATCH_NO_TUNE))
-        for (int i = 0; i < n; i++)
-            if ((rc = autotune(b, b->sims[i], &b->tj[i], &b->ichunk[i]))) return fail(rc);
+    b->rowpath.assign(n, 0);
+    b->ta.assign(n, 8);
+    b->tb.assign(n, 32);
     for (int i = 0; i < n; i++) {
-        const long nb = fused_batch_blocks(b->sims[i]->g, b->tj[i], b->ichunk[i], nullptr, nullptr);
-        if (nb <= 0 || nb >= (1L << kBatchLocalBits) || (b->sims[i]->nbox + 255) / 256 >= (1L << kBatchLocalBits) ||
-            (b->sims[i]->fd.mE + 255) / 256 >= (1L << kBatchLocalBits))
-            return fail(berr(b, FDTD_ERR_UNSUPPORTED, "batch: grid too large for the block map"));
+        fdtd_sim *s = b->sims[i];
+        b->rowpath[i] = s->use_rows && !(flags & FDTD_BATCH_GENERAL) ? 1 : 0;
+        if (b->rowpath[i]) { b->ta[i] = s->rows; b->tb[i] = s->cchunk; }
+        if (!(flags & FDTD_BATCH_NO_TUNE))
+            if ((rc = autotune(b, s, b->rowpath[i] != 0, &b->ta[i], &b->tb[i]))) return fail(rc);
+        long nb[4] = {0, 0, 0, 0};
+        if (b->rowpath[i]) {
+            for (int v = 0; v < 3; v++) nb[v] = rows_blocks(rows_item(s, v, b->ta[i], b->tb[i]));
+            if (nb[0] <= 0) return fail(berr(b, FDTD_ERR_UNSUPPORTED, "batch: row-blocked tile does not fit"));
+        } else {
+            nb[0] = fused_batch_blocks(s->g, b->ta[i], b->tb[i], nullptr, nullptr);
+            if (nb[0] <= 0) return fail(berr(b, FDTD_ERR_UNSUPPORTED, "batch: general tile does not fit"));
+        }
+        nb[3] = std::max((s->nbox + 255) / 256, (s->fd.mE + 255) / 256);
+        for (long v : nb)
+            if (v >= (1L << kBatchLocalBits)) return fail(berr(b, FDTD_ERR_UNSUPPORTED, "batch: grid too large for the block map"));
     }
     *out = b;
     return FDTD_OK;
@@ -447,10 +565,11 @@ int fdtd_batch_run(fdtd_batch *b, int nsteps) {
     return FDTD_OK;
 }
 
-int fdtd_batch_tile(const fdtd_batch *b, int i, int *tj, int *ichunk) {
+int fdtd_batch_tile(const fdtd_batch *b, int i, int *path, int *a, int *c) {
     if (!b || i < 0 || i >= (int)b->sims.size()) return FDTD_ERR_INVALID;
-    if (tj) *tj = b->tj[i];
-    if (ichunk) *ichunk = b->ichunk[i];
+    if (path) *path = b->rowpath[i];
+    if (a) *a = b->ta[i];
+    if (c) *c = b->tb[i];
     return FDTD_OK;
 }
 

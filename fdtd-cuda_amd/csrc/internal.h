@@ -267,8 +267,19 @@ hipError_t launch_disp_batch(const DispDev *items, const unsigned *map, int nblo
 hipError_t launch_sample_batch(const SampleItem *items, const unsigned *map, int nblocks, const int *dyn, int sidx,
                                hipStream_t st);
 hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st);
-// Blocks of one simulation in a batched launch of the general kernel (tile tj x ichunk over the whole grid).
+// Row-blocked kernel over many simulations: items are RowsArgs prepared by
+// rows_prepare (same W = rows_waves(NZ), R and variant 0-2 for all of them).
+hipError_t launch_rows_batch(const RowsArgs *items, const unsigned *map, int nblocks, int W, int R, int variant,
+                             hipStream_t st);
+// TFSF fix-up over many simulations (item.s as for launch_tfsf_fix).
+hipError_t launch_tfsf_fix_batch(const FusedItem *items, const unsigned *map, int nblocks, hipStream_t st);
+long tfsf_fix_blocks(const StepDev &s);
+// Fills nseg and start of a RowsArgs for R and variant; returns its block count or -1.
+long rows_prepare(RowsArgs &ra, int R, int variant);
+// Blocks of one simulation in a batched launch of the general kernel (tile tj x ichunk over the whole grid,
+// or over one to four x-y boxes); -1 when the arguments do not fit.
 long fused_batch_blocks(const Geom &g, int tj, int ichunk, Pieces *pc, int *nkt);
+long fused_pieces(const Geom &g, int tj, int ichunk, const int (*box)[4], int nbox, Pieces *pc, int *nkt);
 
 // Adjoint: inject dJ/dE^{n+1}(r0), transposed dispersive update, fused transposed step.
 hipError_t launch_adj_inject(float *const E[3], long p0, float ax, float ay, float az, hipStream_t st);

@@ -652,8 +652,11 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // cells) and the z differences are taken of those.  Both adjoint psi sets are
 // double buffered (pp: psi_H set read, pe: psi_E set read).  The lanes at
 // wave edges must lie outside the z slabs (checked by the host).
-template <int W, int R, bool XG, bool YG, bool ADJ = false>
-__global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
+//
+// The body takes the block's index within its simulation's cell set (L), so
+// batched launches run it over many simulations.
+template <int W, int R, bool XG, bool YG, bool ADJ>
+__device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     static_assert(!ADJ || (!XG && !YG), "the adjoint variant runs clean boxes only");
     __shared__ float sK[2][2][W][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
     __shared__ float sE[2][2][W][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
@@ -662,7 +665,7 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     const int NX = ra.NX, NY = ra.NY, NZ = ra.NZ;
     for (int q = threadIdx.y * 64 + threadIdx.x; q < kTab; q += 64 * W) sT[q] = ra.aux[q];
     const float *sTf = (const float *)sT;
-    int L = ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : blockIdx.x, pc = 0;
+    int pc = 0;
     while (pc + 1 < ra.npiece && L >= ra.start[pc + 1]) pc++;
     L -= ra.start[pc];
     const int ia = ra.box[pc][0], ib = ra.box[pc][1], ja = ra.box[pc][2], jbox = ra.box[pc][3];
@@ -1089,6 +1092,11 @@ __global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
     }
 }
 
+template <int W, int R, bool XG, bool YG, bool ADJ = false>
+__global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
+    rows_body<W, R, XG, YG, ADJ>(ra, ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : (int)blockIdx.x);
+}
+
 // ----------------------------------------------------------------- split path kernels
 
 __global__ void k_split_h(StepDev s) {
@@ -1153,12 +1161,12 @@ __global__ void k_tfsf_split(StepDev s) {
 // Cells: per normal axis a the planes t0-1, t0, t1, t1+1 spanning [t0-1, t1+1]
 // along the other axes; a cell lying on planes of several axes is taken by
 // the lowest one.  Requires the region free of CPML and inside every domain.
-__global__ void k_tfsf_fix(StepDev s) {
+__device__ __forceinline__ void tfsf_fix_body(const StepDev &s, int bx, int by, int bz) {
     const TfsfDev &t = s.tf;
-    const int a = blockIdx.z >> 2, q = blockIdx.z & 3;
+    const int a = bz >> 2, q = bz & 3;
     const int b0 = a == 0 ? 1 : 0, b1 = a == 2 ? 1 : 2;   // other axes, b1 fastest in memory
-    const int u = pick3(b0, t.t0[0], t.t0[1], t.t0[2]) - 1 + (int)blockIdx.y;
-    const int v = pick3(b1, t.t0[0], t.t0[1], t.t0[2]) - 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int u = pick3(b0, t.t0[0], t.t0[1], t.t0[2]) - 1 + by;
+    const int v = pick3(b1, t.t0[0], t.t0[1], t.t0[2]) - 1 + bx * 64 + (int)threadIdx.x;
     if (u > pick3(b0, t.t1[0], t.t1[1], t.t1[2]) + 1 || v > pick3(b1, t.t1[0], t.t1[1], t.t1[2]) + 1) return;
     const int t0a = pick3(a, t.t0[0], t.t0[1], t.t0[2]), t1a = pick3(a, t.t1[0], t.t1[1], t.t1[2]);
     const int xa = q == 0 ? t0a - 1 : (q == 1 ? t0a : (q == 2 ? t1a : t1a + 1));
@@ -1188,6 +1196,8 @@ __global__ void k_tfsf_fix(StepDev s) {
     if (fy != 0.f) s.Eout[1][p] += fy;
     if (fz != 0.f) s.Eout[2][p] += fz;
 }
+
+__global__ __launch_bounds__(64) void k_tfsf_fix(StepDev s) { tfsf_fix_body(s, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // One thread per sample column of the E half (H taken at the same cell).
 __device__ __forceinline__ void sample_body(const Geom &g, const FaceDev &f, const float *const *E, const float *const *H,
@@ -2173,6 +2183,24 @@ __global__ __launch_bounds__(64 * TJ) void k_fused_batch(const FusedItem *items,
     fused_body<TJ>(it.s, it.ichunk, it.nkt, it.pc, (int)(m & ((1u << kBatchLocalBits) - 1)));
 }
 
+template <int W, int R, bool XG, bool YG>
+__global__ __launch_bounds__(64 * W) void k_rows_batch(const RowsArgs *items, const unsigned *map) {
+    const unsigned m = batch_map(map);
+    rows_body<W, R, XG, YG, false>(batch_item(items, m >> kBatchLocalBits), (int)(m & ((1u << kBatchLocalBits) - 1)));
+}
+
+// Local block L of a simulation's TFSF fix-up grid ((span + 63) / 64, span, 12) in x-fastest order.
+__global__ __launch_bounds__(64) void k_tfsf_fix_batch(const FusedItem *items, const unsigned *map) {
+    const unsigned m = batch_map(map);
+    const FusedItem &it = batch_item(items, m >> kBatchLocalBits);
+    const TfsfDev &t = it.s.tf;
+    int span = 0;
+    for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
+    const int nbx = (span + 63) / 64;
+    const int L = (int)(m & ((1u << kBatchLocalBits) - 1));
+    tfsf_fix_body(it.s, L % nbx, (L / nbx) % span, L / (nbx * span));
+}
+
 __global__ __launch_bounds__(1024) void k_line_batch(const LineDev *items, const int *dyn, int step_off) {
     const CONST_AS int *d = (const CONST_AS int *)dyn;
     line_body(batch_item(items, blockIdx.x), d[0] + step_off);
@@ -2199,20 +2227,30 @@ __global__ void k_batch_advance(int *dyn, int dt, int dslot) {
     }
 }
 
-long fused_batch_blocks(const Geom &g, int tj, int ichunk, Pieces *pc, int *nkt) {
-    if (tj != 4 && tj != 8 && tj != 16) return -1;
-    if (ichunk < 1) return -1;
+long fused_pieces(const Geom &g, int tj, int ichunk, const int (*box)[4], int nbox, Pieces *pc, int *nkt) {
+    if ((tj != 4 && tj != 8 && tj != 16) || ichunk < 1 || nbox < 1 || nbox > 4) return -1;
     Pieces q{};
-    q.n = 1;
-    q.ia[0] = 0; q.ib[0] = g.NX; q.ja[0] = 0; q.jb[0] = g.NY;
-    q.njt[0] = (g.NY + tj - 2) / (tj - 1);
-    q.start[0] = 0;
+    q.n = nbox;
     const int nk = (g.NZ + 62) / 63;
-    const long total = (long)q.njt[0] * nk * ((g.NX + ichunk - 1) / ichunk);
-    q.start[1] = (int)total;
+    long total = 0;
+    for (int b = 0; b < nbox; b++) {
+        q.ia[b] = box[b][0]; q.ib[b] = box[b][1]; q.ja[b] = box[b][2]; q.jb[b] = box[b][3];
+        if (q.ia[b] < 0 || q.ib[b] > g.NX || q.ia[b] >= q.ib[b] || q.ja[b] < 0 || q.jb[b] > g.NY || q.ja[b] >= q.jb[b])
+            return -1;
+        q.njt[b] = (q.jb[b] - q.ja[b] + tj - 2) / (tj - 1);
+        q.start[b] = (int)total;
+        total += (long)q.njt[b] * nk * ((q.ib[b] - q.ia[b] + ichunk - 1) / ichunk);
+    }
+    if (total > 0x7fffffff) return -1;
+    q.start[nbox] = (int)total;
     if (pc) *pc = q;
     if (nkt) *nkt = nk;
     return total;
+}
+
+long fused_batch_blocks(const Geom &g, int tj, int ichunk, Pieces *pc, int *nkt) {
+    const int full[1][4] = {{0, g.NX, 0, g.NY}};
+    return fused_pieces(g, tj, ichunk, full, 1, pc, nkt);
 }
 
 hipError_t launch_fused_batch(const FusedItem *items, const unsigned *map, int nblocks, int tj, hipStream_t st) {
@@ -2224,6 +2262,46 @@ hipError_t launch_fused_batch(const FusedItem *items, const unsigned *map, int n
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int W>
+static hipError_t launch_rows_batch_w(const RowsArgs *items, const unsigned *map, int nblocks, int R, int variant,
+                                      hipStream_t st) {
+    const dim3 grid(nblocks), block(64, W);
+    if (R == 3 && variant == 0) hipLaunchKernelGGL((k_rows_batch<W, 3, false, false>), grid, block, 0, st, items, map);
+    else if (R == 2 && variant == 0) hipLaunchKernelGGL((k_rows_batch<W, 2, false, false>), grid, block, 0, st, items, map);
+    else if (R == 2 && variant == 1) hipLaunchKernelGGL((k_rows_batch<W, 2, true, false>), grid, block, 0, st, items, map);
+    else if (R == 2 && variant == 2) hipLaunchKernelGGL((k_rows_batch<W, 2, false, true>), grid, block, 0, st, items, map);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_batch(const RowsArgs *items, const unsigned *map, int nblocks, int W, int R, int variant,
+                             hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    switch (W) {
+        case 2: return launch_rows_batch_w<2>(items, map, nblocks, R, variant, st);
+        case 3: return launch_rows_batch_w<3>(items, map, nblocks, R, variant, st);
+        case 4: return launch_rows_batch_w<4>(items, map, nblocks, R, variant, st);
+        case 5: return launch_rows_batch_w<5>(items, map, nblocks, R, variant, st);
+        case 6: return launch_rows_batch_w<6>(items, map, nblocks, R, variant, st);
+        case 8: return launch_rows_batch_w<8>(items, map, nblocks, R, variant, st);
+        case 12: return launch_rows_batch_w<12>(items, map, nblocks, R, variant, st);
+        case 16: return launch_rows_batch_w<16>(items, map, nblocks, R, variant, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tfsf_fix_batch(const FusedItem *items, const unsigned *map, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tfsf_fix_batch, dim3(nblocks), dim3(64), 0, st, items, map);
+    return hipGetLastError();
+}
+
+long tfsf_fix_blocks(const StepDev &s) {
+    int span = 0;
+    for (int a = 0; a < 3; a++) span = max(span, s.tf.t1[a] - s.tf.t0[a] + 3);
+    return (long)((span + 63) / 64) * span * 12;
 }
 
 hipError_t launch_line_batch(const LineDev *items, int n, const int *dyn, int step_off, hipStream_t st) {
@@ -2281,22 +2359,28 @@ int rows_waves(int NZ) {
     return 16;
 }
 
-hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
-    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 3)
-        return hipErrorInvalidValue;
+long rows_prepare(RowsArgs &ra, int R, int variant) {
+    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 3) return -1;
     const int W = rows_waves(ra.NZ);
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
     long total = 0;
     for (int q = 0; q < ra.npiece; q++) {
         const int *b = ra.box[q];
-        if (b[0] < 0 || b[1] > ra.NX || b[0] >= b[1] || b[2] < 0 || b[3] > ra.NY || b[2] >= b[3]) return hipErrorInvalidValue;
-        if (variant != 1 && (b[0] < 1 || b[1] > ra.NX - 1)) return hipErrorInvalidValue;   // x must be clean
-        if (variant != 2 && (b[2] < 1 || b[3] > ra.NY - 1)) return hipErrorInvalidValue;   // y must be clean
+        if (b[0] < 0 || b[1] > ra.NX || b[0] >= b[1] || b[2] < 0 || b[3] > ra.NY || b[2] >= b[3]) return -1;
+        if (variant != 1 && (b[0] < 1 || b[1] > ra.NX - 1)) return -1;   // x must be clean
+        if (variant != 2 && (b[2] < 1 || b[3] > ra.NY - 1)) return -1;   // y must be clean
         ra.start[q] = (int)total;
         total += (long)((b[3] - b[2] + R - 1) / R) * ra.nseg * ((b[1] - b[0] + ra.ichunk - 1) / ra.ichunk);
     }
-    if (total > 0x7fffffff) return hipErrorInvalidValue;
+    if (total > 0x7fffffff) return -1;
     ra.start[ra.npiece] = (int)total;
+    return total;
+}
+
+hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
+    const long total = rows_prepare(ra, R, variant);
+    if (total < 0) return hipErrorInvalidValue;
+    const int W = rows_waves(ra.NZ);
     const dim3 grid((unsigned)total);
     switch (variant) {
         case 0: return launch_rows_g<false, false>(ra, R, W, grid, st);
@@ -2380,7 +2464,7 @@ hipError_t launch_tfsf_fix(const StepDev &s, hipStream_t st) {
     const TfsfDev &t = s.tf;
     int span = 0;
     for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
-    dim3 grid((span + 63) / 64, span, 12), block(64);
+    dim3 grid((span + 63) / 64, span, 12), block(64);   // the body assumes 64-thread blocks
     hipLaunchKernelGGL(k_tfsf_fix, grid, block, 0, st, s);
     return hipGetLastError();
 }

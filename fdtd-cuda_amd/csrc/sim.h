@@ -142,6 +142,8 @@ int upload(fdtd_sim *s, T **ptr, const std::vector<T> &v) {
 StepDev make_step(fdtd_sim *s, bool fused);
 LineDev make_line(fdtd_sim *s);
 DispDev make_disp(fdtd_sim *s, int parity);
+// Row-blocked kernel arguments at the handle's parity (clean-box chunk; boxes and R set by the caller).
+RowsArgs make_rows(fdtd_sim *s);
 // DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.
 bool sampling_step(const fdtd_sim *s);
 int after_sample(fdtd_sim *s);

@@ -25,6 +25,7 @@ EXPORTS = [
 
 BATCH_NO_GRAPH = 1
 BATCH_NO_TUNE = 2
+BATCH_GENERAL = 4
 
 _lib = None
 
@@ -65,7 +66,7 @@ def load_library(path=LIB_PATH):
         "fdtd_add_rod": (ip, [vp, C.POINTER(C.c_double * 3), ip, C.c_double, C.c_double]),
         "fdtd_batch_create": (ip, [C.POINTER(vp), ip, ip, C.POINTER(vp)]),
         "fdtd_batch_run": (ip, [vp, ip]),
-        "fdtd_batch_tile": (ip, [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "fdtd_batch_tile": (ip, [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "fdtd_batch_last_error": (C.c_char_p, [vp]),
         "fdtd_batch_destroy": (ip, [vp]),
     }
@@ -255,9 +256,10 @@ class Batch:
         self._check(self.lib.fdtd_batch_run(self.h, int(nsteps)))
 
     def tile(self, i):
-        tj, ic = C.c_int(), C.c_int()
-        self._check(self.lib.fdtd_batch_tile(self.h, int(i), C.byref(tj), C.byref(ic)))
-        return tj.value, ic.value
+        """(path, a, c): path 1 row-blocked kernel (rows per thread, x chunk), 0 general (block rows, x chunk)."""
+        path, a, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.fdtd_batch_tile(self.h, int(i), C.byref(path), C.byref(a), C.byref(c)))
+        return path.value, a.value, c.value
 
     def close(self):
         if self.h:

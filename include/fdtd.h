@@ -216,10 +216,12 @@ int  fdtd_kernel_timing(fdtd_sim *sim, int enable, double *total_ms, long *launc
  * independent simulations of any grid sizes together: each kernel of a step
  * runs once for all of them (the blocks of every simulation in one grid), and
  * the launches of one DFT sampling period are captured in a HIP graph and
- * replayed.  The field update of a batch is the general fused kernel (TFSF
- * inside) with a tile per grid size picked by the autotuner (timed on the
- * device at fdtd_batch_create, remembered per process and, when the
- * environment variable FDTD_TUNE_FILE names a file, across processes).
+ * replayed.  The field update is fdtd_step's: the row-blocked kernel with the
+ * TFSF fix-up where the simulation's grid allows it, else the general fused
+ * kernel; the tile of each grid size's dominant launch is picked by the
+ * autotuner (timed on the device at fdtd_batch_create, remembered per process
+ * and, when the environment variable FDTD_TUNE_FILE names a file, across
+ * processes).
  *
  * The batch does not own the simulations, which must outlive it.  They must
  * be at the same step t, share dft_stride and dft_block, and have no
@@ -228,12 +230,15 @@ int  fdtd_kernel_timing(fdtd_sim *sim, int enable, double *total_ms, long *launc
  */
 typedef struct fdtd_batch fdtd_batch;
 #define FDTD_BATCH_NO_GRAPH 1   /* launch every step directly instead of replaying a graph */
-#define FDTD_BATCH_NO_TUNE  2   /* default tile (8 rows, x chunk 32) for every grid size   */
+#define FDTD_BATCH_NO_TUNE  2   /* the simulations' own tiles (those fdtd_step uses)       */
+#define FDTD_BATCH_GENERAL  4   /* general fused kernel for every simulation               */
 int  fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out);
 /* Advances every simulation by nsteps full steps (as fdtd_step does). */
 int  fdtd_batch_run(fdtd_batch *b, int nsteps);
-/* Tile used for simulation i: block rows (tj - 1 owned) and x chunk. */
-int  fdtd_batch_tile(const fdtd_batch *b, int i, int *tj, int *ichunk);
+/* Field-update path and tile of simulation i: path 1 = row-blocked kernel
+ * (a = rows per thread, c = x chunk of the clean box), 0 = general kernel
+ * (a = block rows, tj - 1 owned, c = x chunk). */
+int  fdtd_batch_tile(const fdtd_batch *b, int i, int *path, int *a, int *c);
 const char *fdtd_batch_last_error(const fdtd_batch *b);
 int  fdtd_batch_destroy(fdtd_batch *b);
 

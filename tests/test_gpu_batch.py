@@ -3,10 +3,11 @@
 * Every simulation of an uneven-size batch against the FP64 oracle (fields
   2e-4 of the largest E or H value, spectra 1e-4 of their maximum, as the
   single-simulation parity tests).
-* The batch is the same arithmetic as the single-simulation general kernel:
-  with the default tile, graph replay, direct launches and each simulation
-  run alone (FDTD_PATH=general) agree bit for bit in the fields, including
-  runs split by individual steps between batch runs.
+* The batch is the same arithmetic as fdtd_step: with the simulations' own
+  tiles, graph replay, direct launches and each simulation run alone agree
+  bit for bit in the fields, on the row-blocked path and on the general
+  kernel (FDTD_PATH=general), including runs split by individual steps
+  between batch runs.
 * The autotuned tiles change only rounding (group-normalised 1e-5).
 """
 import os
@@ -105,10 +106,11 @@ def test_rod_masks():
     assert s.get_info().n_dispersive[0] > 0
 
 
-def test_batch_matches_oracle():
+@pytest.mark.parametrize("flags", [0, G.BATCH_GENERAL])
+def test_batch_matches_oracle(flags):
     steps = 301      # not a multiple of the sampling period: direct steps after the graph replays
     sims = sims_for(CASES)
-    b = G.Batch(sims)
+    b = G.Batch(sims, flags=flags)
     b.run(steps)
     for case, s in zip(CASES, sims):
         o = oracle_for(case, s, steps)
@@ -124,13 +126,15 @@ def test_batch_matches_oracle():
             assert ext_err < 1e-4
 
 
-def test_batch_graph_direct_and_alone_agree_bitwise():
+@pytest.mark.parametrize("general", [False, True])
+def test_batch_graph_direct_and_alone_agree_bitwise(general):
     steps = 203
-    a = sims_for(CASES, env_general=True)
-    b = sims_for(CASES, env_general=True)
-    c = sims_for(CASES, env_general=True)
+    a = sims_for(CASES, env_general=general)
+    b = sims_for(CASES, env_general=general)
+    c = sims_for(CASES, env_general=general)
     ba = G.Batch(a, flags=G.BATCH_NO_TUNE)
     bb = G.Batch(b, flags=G.BATCH_NO_TUNE | G.BATCH_NO_GRAPH)
+    assert all(ba.tile(i)[0] == (0 if general else 1) for i in range(len(CASES)))
     ba.run(steps)
     bb.run(steps)
     for s in c:
@@ -146,11 +150,12 @@ def test_batch_graph_direct_and_alone_agree_bitwise():
         assert a[i].get_info().sample_count == c[i].get_info().sample_count
 
 
-def test_batch_runs_interleaved_with_single_steps():
+@pytest.mark.parametrize("general", [False, True])
+def test_batch_runs_interleaved_with_single_steps(general):
     """Batch runs split by fdtd_step calls on every simulation (odd counts,
     so parities and the sampling phase move) equal one uninterrupted run."""
-    a = sims_for(CASES, env_general=True)
-    c = sims_for(CASES, env_general=True)
+    a = sims_for(CASES, env_general=general)
+    c = sims_for(CASES, env_general=general)
     ba = G.Batch(a, flags=G.BATCH_NO_TUNE)
     ba.run(97)
     for s in a:
@@ -168,12 +173,13 @@ def test_batch_runs_interleaved_with_single_steps():
         assert np.max(np.abs(ra - rc)) <= 1e-5 * np.max(np.abs(rc))
 
 
-def test_autotuned_tiles_change_only_rounding():
+@pytest.mark.parametrize("flags", [0, G.BATCH_GENERAL])
+def test_autotuned_tiles_change_only_rounding(flags):
     steps = 160
     a = sims_for(CASES)
     b = sims_for(CASES)
-    ta = G.Batch(a)
-    tb = G.Batch(b, flags=G.BATCH_NO_TUNE)
+    ta = G.Batch(a, flags=flags)
+    tb = G.Batch(b, flags=flags | G.BATCH_NO_TUNE)
     ta.run(steps)
     tb.run(steps)
     for i in range(len(CASES)):
