@@ -1052,6 +1052,10 @@ int fdtd_update_e(fdtd_sim *s) {
 int fdtd_dft_accumulate(fdtd_sim *s) {
     if (!s) return FDTD_ERR_INVALID;
     int rc = FDTD_OK;
+    if (s->mon && s->t - s->mon_t0 < s->mon_cap) {
+        const float *E1[3] = {s->E[s->p][0], s->E[s->p][1], s->E[s->p][2]};
+        HIPCHK(s, launch_probe(E1, s->mon_p, s->mon + 3 * (size_t)(s->t - s->mon_t0), s->stream));
+    }
     if (sampling_step(s)) {
         StepDev d = make_step(s, false);
         for (int c = 0; c < 3; c++) { d.Ein[c] = s->E[s->p][c]; d.Hin[c] = s->H[s->p][c]; }
@@ -1105,6 +1109,10 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
             const float *E1[3] = {s->E[1 - q][0], s->E[1 - q][1], s->E[1 - q][2]};
             HIPCHK(s, launch_probe(E1, s->probe_p, s->probe + 3 * (size_t)(s->t - s->probe_t0), s->stream));
         }
+        if (s->mon && s->t - s->mon_t0 < s->mon_cap) {
+            const float *E1[3] = {s->E[1 - q][0], s->E[1 - q][1], s->E[1 - q][2]};
+            HIPCHK(s, launch_probe(E1, s->mon_p, s->mon + 3 * (size_t)(s->t - s->mon_t0), s->stream));
+        }
         s->p ^= 1;
         s->pp ^= 1;
         s->le ^= 1;
@@ -1118,6 +1126,43 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
             if (rc) return rc;
         }
         s->t++;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_set_point_monitor(fdtd_sim *s, const int r[3], int capacity) {
+    if (!s || !r || capacity < 1) return FDTD_ERR_INVALID;
+    for (int a = 0; a < 3; a++)
+        if (r[a] < 0 || r[a] >= s->S.N[a]) return set_err(s, FDTD_ERR_INVALID, "point monitor outside the grid");
+    if (capacity > s->mon_cap_alloc) {
+        float *m = nullptr;
+        if (int rc = dalloc(s, &m, (size_t)3 * capacity)) return rc;
+        s->mon = m;
+        s->mon_cap_alloc = capacity;
+    }
+    s->mon_cap = capacity;
+    s->mon_p = gidx(s->g, r[0], r[1], r[2]);
+    s->mon_t0 = s->t;
+    return FDTD_OK;
+}
+
+int fdtd_point_spectrum(fdtd_sim *s, int nf, const double *freq, double *e2, double *amp) {
+    if (!s || nf < 1 || !freq || (!e2 && !amp)) return FDTD_ERR_INVALID;
+    if (!s->mon) return set_err(s, FDTD_ERR_INVALID, "no point monitor set");
+    const int n = std::min(s->t - s->mon_t0, s->mon_cap);
+    std::vector<float> e((size_t)3 * std::max(n, 1));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (n > 0) HIPCHK(s, hipMemcpy(e.data(), s->mon, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost));
+    const double dt = s->S.dt;
+    for (int f = 0; f < nf; f++) {
+        std::complex<double> g[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < n; q++) {
+            const std::complex<double> w = std::exp(std::complex<double>(0, 2 * fdtd::kPI * freq[f] * (s->mon_t0 + q + 1.0) * dt));
+            for (int c = 0; c < 3; c++) g[c] += (double)e[(size_t)3 * q + c] * w;
+        }
+        if (e2) e2[f] = std::norm(g[0]) + std::norm(g[1]) + std::norm(g[2]);
+        if (amp)
+            for (int c = 0; c < 3; c++) { amp[(f * 3 + c) * 2] = g[c].real(); amp[(f * 3 + c) * 2 + 1] = g[c].imag(); }
     }
     return FDTD_OK;
 }

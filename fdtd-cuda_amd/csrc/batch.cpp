@@ -462,6 +462,7 @@ int check_sims(fdtd_batch *b) {
             return berr(b, FDTD_ERR_INVALID, "batch: simulations must share dft_stride, dft_block and StaticField");
         if (s->S.hydro) return berr(b, FDTD_ERR_UNSUPPORTED, "batch: hydrodynamic simulations are not batched");
         if (s->probe || !s->sampling) return berr(b, FDTD_ERR_INVALID, "batch: simulation is inside an adjoint run");
+        if (s->mon) return berr(b, FDTD_ERR_UNSUPPORTED, "batch: point monitors are recorded by fdtd_step only");
     }
     return FDTD_OK;
 }

@@ -56,6 +56,9 @@ struct fdtd_sim {
     float *rho_box = nullptr;       // density per dispersive-box cell (1 outside the design block)
     bool sampling = true;           // DFT sampling on (off while the adjoint recomputes a segment)
     float *probe = nullptr;         // E(r0) per step while the adjoint's forward runs
+    float *mon = nullptr;           // point monitor: E at mon_p for every step since mon_t0 (mon_cap steps)
+    long mon_p = 0;
+    int mon_t0 = 0, mon_cap = 0, mon_cap_alloc = 0;
     long probe_p = 0;
     int probe_t0 = 0;               // step whose E^{n+1} lands in probe[0..2]
     float *traj_step[3] = {};       // dispersive kernel writes E^{n+1} here when set

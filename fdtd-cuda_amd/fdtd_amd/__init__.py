@@ -19,7 +19,7 @@ EXPORTS = [
     "fdtd_set_materials", "fdtd_add_sphere", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
     "fdtd_dft_accumulate", "fdtd_step", "fdtd_sync", "fdtd_get_field", "fdtd_set_field",
     "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
-    "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod",
+    "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod", "fdtd_set_point_monitor", "fdtd_point_spectrum",
     "fdtd_batch_create", "fdtd_batch_run", "fdtd_batch_tile", "fdtd_batch_last_error", "fdtd_batch_destroy",
 ]
 
@@ -64,6 +64,8 @@ def load_library(path=LIB_PATH):
         "fdtd_set_design": (ip, [vp, C.POINTER(C.c_int * 3), C.POINTER(C.c_int * 3), fp]),
         "fdtd_adjoint": (ip, [vp, ip, ip, C.POINTER(C.c_int * 3), ip, dp, dp, dp]),
         "fdtd_add_rod": (ip, [vp, C.POINTER(C.c_double * 3), ip, C.c_double, C.c_double]),
+        "fdtd_set_point_monitor": (ip, [vp, C.POINTER(C.c_int * 3), ip]),
+        "fdtd_point_spectrum": (ip, [vp, ip, dp, dp, dp]),
         "fdtd_batch_create": (ip, [C.POINTER(vp), ip, ip, C.POINTER(vp)]),
         "fdtd_batch_run": (ip, [vp, ip]),
         "fdtd_batch_tile": (ip, [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -142,6 +144,20 @@ class Simulation:
         c = (C.c_double * 3)(*[float(x) for x in centre])
         self._check(self.lib.fdtd_add_rod(self.h, C.byref(c), int(axis), float(length), float(radius)))
         self.info = self.get_info()
+
+    def set_point_monitor(self, r, capacity):
+        """Records E at cell r after each of the next `capacity` steps."""
+        a = (C.c_int * 3)(*[int(x) for x in r])
+        self._check(self.lib.fdtd_set_point_monitor(self.h, C.byref(a), int(capacity)))
+
+    def point_spectrum(self, freqs):
+        """(|E(f)|^2 per frequency, complex amplitudes [nf, 3]) of the point monitor."""
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        e2 = np.empty(len(f))
+        amp = np.empty(len(f) * 6)
+        self._check(self.lib.fdtd_point_spectrum(self.h, len(f), _ptr(f, C.c_double), _ptr(e2, C.c_double),
+                                                 _ptr(amp, C.c_double)))
+        return e2, (amp[0::2] + 1j * amp[1::2]).reshape(len(f), 3)
 
     def set_design(self, lo, rho):
         """Design block of densities rho (shape n, i-major, k fastest) at cell lo."""

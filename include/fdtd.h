@@ -177,6 +177,15 @@ int  fdtd_dft_accumulate(fdtd_sim *sim);
 /* Fused fast path: nsteps full steps (H + TFSF + E + DFT). */
 int  fdtd_step(fdtd_sim *sim, int nsteps);
 
+/* Point monitor (no reference counterpart; the gap field of dimer
+ * workloads): records E at cell r after every following step (fdtd_step or
+ * the split path), up to `capacity` steps.  fdtd_point_spectrum returns, per
+ * frequency (Hz), |E(f)|^2 = sum_c |sum_t E_c^{t+1}(r) e^{i 2 pi f (t+1) dt}|^2
+ * in FP64 (the DFT convention of the monitors), and when amp is not NULL the
+ * three complex amplitudes (re, im pairs, nf*3 of them). */
+int  fdtd_set_point_monitor(fdtd_sim *sim, const int r[3], int capacity);
+int  fdtd_point_spectrum(fdtd_sim *sim, int nf, const double *freq, double *e2, double *amp);
+
 /* Blocks until queued work finished; reports device errors. */
 int  fdtd_sync(fdtd_sim *sim);
 

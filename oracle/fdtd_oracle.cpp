@@ -1407,6 +1407,13 @@ int orc_set_objective(void *h, const int *r0, int nf, const double *freq) {
 
 double orc_objective(void *h) { return objective(*(Oracle *)h); }
 
+// The complex sums of the point objective, (re, im) per frequency and component.
+int orc_objective_terms(void *h, double *out) {
+    const Oracle &o = *(const Oracle *)h;
+    for (size_t q = 0; q < o.g.size(); q++) { out[2 * q] = o.g[q].real(); out[2 * q + 1] = o.g[q].imag(); }
+    return (int)o.g.size();
+}
+
 int orc_adjoint(void *h, int nsteps, double *J, double *grad) { return forward_adjoint(*(Oracle *)h, nsteps, J, grad); }
 
 }  // extern "C"

@@ -49,6 +49,7 @@ def load():
         "orc_set_design": (ip, [vp, C.POINTER(C.c_int * 3), C.POINTER(C.c_int * 3), dp]),
         "orc_set_objective": (ip, [vp, C.POINTER(C.c_int * 3), ip, dp]),
         "orc_objective": (C.c_double, [vp]),
+        "orc_objective_terms": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "orc_adjoint": (ip, [vp, ip, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -183,11 +184,19 @@ class Oracle:
 
     def set_objective(self, r0, freqs):
         f = np.ascontiguousarray(freqs, dtype=np.float64)
+        self._obj_freq = f
         a = (C.c_int * 3)(*r0)
         self.lib.orc_set_objective(self.h, C.byref(a), len(f), _ptr(f))
 
     def objective(self):
         return self.lib.orc_objective(self.h)
+
+    def objective_terms(self):
+        """Complex sums sum_t E_c^{t+1}(r0) e^{i 2 pi f (t+1) dt}, shape [nf, 3]."""
+        n = len(self._obj_freq)
+        out = np.empty(n * 6)
+        self.lib.orc_objective_terms(self.h, _ptr(out))
+        return (out[0::2] + 1j * out[1::2]).reshape(n, 3)
 
     def adjoint(self, nsteps):
         """Runs nsteps forward then the reverse sweep; returns (J, dJ/drho)."""

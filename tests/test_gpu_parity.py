@@ -186,3 +186,31 @@ def test_hydrodynamic_sphere(fused):
     errs = field_errors(s, o)
     print(f"hydro fused={fused} errors={['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) < TOL, errs
+
+
+def test_point_monitor_gap_field_of_a_dimer():
+    """|E(f)|^2 at the gap centre of a small gold dimer (the gap-field metric)
+    against the oracle's FP64 sums; normalised by the largest value, 1e-4."""
+    p = small_params(O, n=30, ncpml=6, sphere=0, steps=500, phi=90.0)
+    s = fdtd_amd.Simulation(p)
+    T = s.shape[0]
+    c = T // 2
+    # 3-cell spheres ((int)(6.5 nm / 2 nm)) centred 8 cells apart: cells c-1..c+1 of the gap are vacuum
+    s.add_sphere((c - 4, c, c), 6.5e-9)
+    s.add_sphere((c + 4, c, c), 6.5e-9)
+    o = O.Oracle(p)
+    o.set_materials(s.get_materials(1), s.get_materials(2), s.get_materials(3))
+    r0 = (c, c, c)
+    freqs = np.array([3.8e14, 4.6e14, 5.5e14, 6.7e14])
+    s.set_point_monitor(r0, 600)
+    o.set_objective(r0, freqs)
+    s.step(500)
+    o.step(500)
+    e2, amp = s.point_spectrum(freqs)
+    ref = o.objective_terms()
+    e2o = np.sum(np.abs(ref) ** 2, axis=1)
+    err = float(np.max(np.abs(e2 - e2o)) / np.max(e2o))
+    aerr = float(np.max(np.abs(amp - ref)) / np.max(np.abs(ref)))
+    print("gap |E|^2", e2, "oracle", e2o, "err %.2e amp err %.2e" % (err, aerr), flush=True)
+    assert s.get_info().n_dispersive[0] > 0
+    assert err < 1e-4 and aerr < 1e-4

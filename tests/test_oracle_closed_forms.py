@@ -255,3 +255,28 @@ def test_hydro_local_limit_is_drude():
     ref = drude_cp_eps(w) + (1.3202e16 ** 2 - wp2) / (w ** 2 - 1j * 1.0805e14 * w)
     err = np.abs(eps_h - ref) / np.abs(ref)
     assert np.all(err < 5 * w * dt), (err, w * dt)
+
+
+def test_point_objective_terms_are_the_dft_of_the_recorded_field():
+    """The oracle's point-objective sums equal a numpy DFT, with phase
+    (t+1) dt, of E at r0 read back after every step."""
+    p = small_params(O, n=18, ncpml=4, steps=60, sphere=0, phi=90.0)
+    p.BandWidth = 8e16
+    o = O.Oracle(p)
+    info = o.get_info()
+    r0 = (info.NX // 2, info.NY // 2 + 1, info.NZ // 2 - 2)
+    freqs = np.array([3.5e14, 4.7e14, 6.1e14])
+    o.set_objective(r0, freqs)
+    dt = info.dt
+    rec = []
+    for t in range(60):
+        o.step(1)
+        rec.append([o.get_field(c)[r0] for c in range(3)])
+    rec = np.array(rec)
+    assert np.abs(rec).max() > 0
+    t = np.arange(60)
+    w = np.exp(1j * 2 * np.pi * freqs[:, None] * (t[None, :] + 1.0) * dt)
+    ref = w @ rec
+    got = o.objective_terms()
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    assert o.objective() == pytest.approx(float(np.sum(np.abs(ref) ** 2)), rel=1e-12)
