@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2048)
     ap.add_argument("--warmup", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=128,
                     help="steps of the 32-rod sweep window, batched and one by one (0: skip)")
@@ -182,7 +182,7 @@ def main():
             "grid": [info.NX, info.NY, info.NZ],
             "dispersive_cells": list(info.n_dispersive[:]),
             "tile": {"tj": int(os.environ.get("FDTD_TJ", 8)), "ichunk": int(os.environ.get("FDTD_ICHUNK", 32)),
-                     "rows": int(os.environ.get("FDTD_ROWS", 3)), "cchunk": int(os.environ.get("FDTD_CCHUNK", 16)),
+                     "rows": int(os.environ.get("FDTD_ROWS", 2)), "cchunk": int(os.environ.get("FDTD_CCHUNK", 16)),
                      "schunk": int(os.environ.get("FDTD_SCHUNK", 8)),
                      "side_stream": int(os.environ.get("FDTD_CONCURRENT", 1)), "xcd_order": int(os.environ.get("FDTD_XCD", 0))},
         },

@@ -1092,8 +1092,10 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     }
 }
 
+// Two rows per thread fit in 128 VGPRs (four waves per SIMD); the cap costs
+// the x-general variant a few spilled registers and gains it a wave per SIMD.
 template <int W, int R, bool XG, bool YG, bool ADJ = false>
-__global__ __launch_bounds__(64 * W) void k_step_rows(RowsArgs ra) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_step_rows(RowsArgs ra) {
     rows_body<W, R, XG, YG, ADJ>(ra, ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : (int)blockIdx.x);
 }
 
@@ -2184,7 +2186,7 @@ __global__ __launch_bounds__(64 * TJ) void k_fused_batch(const FusedItem *items,
 }
 
 template <int W, int R, bool XG, bool YG>
-__global__ __launch_bounds__(64 * W) void k_rows_batch(const RowsArgs *items, const unsigned *map) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_rows_batch(const RowsArgs *items, const unsigned *map) {
     const unsigned m = batch_map(map);
     rows_body<W, R, XG, YG, false>(batch_item(items, m >> kBatchLocalBits), (int)(m & ((1u << kBatchLocalBits) - 1)));
 }
