@@ -102,6 +102,8 @@ def main():
                     help="steps of the 32-rod sweep window, batched and one by one (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the adjoint/forward timing on the inverse-design grid (0: skip)")
+    ap.add_argument("--hydro-steps", type=int, default=200,
+                    help="steps of the hydrodynamic 384^3 timing window (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB copy kernel 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -158,6 +160,7 @@ def main():
         copy_calibration()
     sweep = sweep_bench(args.sweep_steps) if args.sweep_steps > 0 else None
     adjoint = adjoint_bench(args.adjoint_steps) if args.adjoint_steps > 0 else None
+    hydro = hydro_bench(args.hydro_steps) if args.hydro_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -208,6 +211,7 @@ def main():
         },
         "sweep": sweep,
         "adjoint": adjoint,
+        "hydro": hydro,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -349,6 +353,49 @@ def adjoint_bench(steps, ckpt=None, seed=7):
         "stored_trajectory_ratio": round(t_sto / t_fwd, 3),
         "J": res["recompute"][1],
         "grad_norm": res["recompute"][2],
+    }
+
+
+def hydro_bench(steps):
+    """Hydrodynamic nonlocal gold: 384^3 interior at dx = 0.25 nm (409^3 cells),
+    12-cell CPML, one 10 nm sphere (40 cells), beta = 1.27e6 m/s with the
+    Drude + 2 CP poles, 48 wavelengths 400-900 nm; Mcell-updates/s over a
+    window of `steps` full steps (velocity, charge and E-current kernels on the
+    metal box after the field update)."""
+    import fdtd_amd
+    p = fdtd_amd.default_params()
+    p.NCELLX = p.NCELLY = p.NCELLZ = 384
+    p.NcpmlX = p.NcpmlY = p.NcpmlZ = 12
+    p.dx = p.dy = p.dz = 0.25e-9
+    p.TimeFactor = 3 ** 0.5
+    p.inc_phi = 90.0
+    p.nano_sphere = 1
+    p.nano_sphere_radius = 10e-9
+    p.material = 3
+    p.NONLOCAL = 3
+    p.hydro_beta = 1.27e6
+    p.tfsf_inset = 10
+    p.NUM_freq = 48
+    p.lam_min, p.lam_max = 400e-9, 900e-9
+    p.dft_block = 64
+    s = fdtd_amd.Simulation(p)
+    info = s.get_info()
+    cells = info.NX * info.NY * info.NZ
+    s.step(16)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.close()
+    return {
+        "workload": "hydrodynamic gold sphere r=10 nm, 384^3 interior + 12-cell CPML (409^3), dx=0.25 nm, "
+                    "beta=1.27e6 m/s, 48 wavelengths",
+        "cells": cells,
+        "metal_cells": int(info.n_dispersive[0]),
+        "steps": steps,
+        "ms_per_step": round(wall * 1e3 / steps, 4),
+        "mcells_per_s": round(cells * steps / wall / 1e6, 1),
     }
 
 
