@@ -347,10 +347,10 @@ def adjoint_bench(steps, ckpt=None, seed=7):
         "steps": steps,
         "checkpoint": ckpt,
         "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
-        "adjoint_ms_per_step": round(t_rec * 1e3 / steps, 4),
-        "ratio": round(t_rec / t_fwd, 3),
         "stored_trajectory_ms_per_step": round(t_sto * 1e3 / steps, 4),
-        "stored_trajectory_ratio": round(t_sto / t_fwd, 3),
+        "ratio_stored_trajectory": round(t_sto / t_fwd, 3),
+        "checkpoint_recompute_ms_per_step": round(t_rec * 1e3 / steps, 4),
+        "ratio_checkpoint_recompute": round(t_rec / t_fwd, 3),
         "J": res["recompute"][1],
         "grad_norm": res["recompute"][2],
     }
