@@ -685,17 +685,24 @@ int ckpt_copy(fdtd_sim *s, Checkpoint &ck, bool save) {
     items.push_back({s->line_e[s->le], s->S.L * sizeof(double)});
     items.push_back({s->line_h, s->S.L * sizeof(double)});
     items.push_back({s->mur, 2 * sizeof(double)});
+    // one allocation per checkpoint, items at 256-byte aligned offsets
+    std::vector<size_t> off(items.size());
+    size_t total = 0;
+    for (size_t n = 0; n < items.size(); n++) {
+        off[n] = total;
+        total += (items[n].second + 255) / 256 * 256;
+    }
     if (save) {
-        for (auto &it : items) {
-            void *b = nullptr;
-            if (hipMalloc(&b, it.second) != hipSuccess) return set_err(s, FDTD_ERR_ALLOC, "checkpoint allocation failed");
-            ck.bufs.push_back(b);
-            HIPCHK(s, hipMemcpyAsync(b, it.first, it.second, hipMemcpyDeviceToDevice, s->stream));
-        }
+        void *b = nullptr;
+        if (hipMalloc(&b, total) != hipSuccess) return set_err(s, FDTD_ERR_ALLOC, "checkpoint allocation failed");
+        ck.bufs.push_back(b);
+        for (size_t n = 0; n < items.size(); n++)
+            HIPCHK(s, hipMemcpyAsync((char *)b + off[n], items[n].first, items[n].second, hipMemcpyDeviceToDevice, s->stream));
         ck.t = s->t; ck.p = s->p; ck.pp = s->pp; ck.le = s->le;
     } else {
+        char *b = (char *)ck.bufs.at(0);
         for (size_t n = 0; n < items.size(); n++)
-            HIPCHK(s, hipMemcpyAsync(items[n].first, ck.bufs[n], items[n].second, hipMemcpyDeviceToDevice, s->stream));
+            HIPCHK(s, hipMemcpyAsync(items[n].first, b + off[n], items[n].second, hipMemcpyDeviceToDevice, s->stream));
     }
     return FDTD_OK;
 }
@@ -730,6 +737,10 @@ int adj_alloc(fdtd_sim *s) {
     // cells inside the clean box and free of z CPML (launch_adj_box_fix)
     s->adj.rows = false;
     if (const char *v = std::getenv("FDTD_ADJ_SCHUNK")) s->adj_schunk = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_ADJ_STJ")) {
+        const int tj = std::atoi(v);
+        if (tj == 4 || tj == 8 || tj == 16) s->adj_stj = tj;
+    }
     const char *ap = std::getenv("FDTD_ADJ_PATH");
     if (s->use_rows && !(ap && std::string(ap) == "general") && s->S.Chh == 1.0) {
         const Setup &S = s->S;
@@ -855,7 +866,7 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
     int slabs[4][4], ns = 0;
     for (int q = 0; q < s->nxbox; q++) std::memcpy(slabs[ns++], s->xbox[q], sizeof(slabs[0]));
     for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
-    if (ns) HIPCHK(s, launch_adj_step(av, 8, s->adj_schunk, slabs, ns, side));
+    if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
     if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, side));
     if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     RowsArgs r = make_rows(s);
