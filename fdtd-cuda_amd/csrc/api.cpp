@@ -375,10 +375,13 @@ int plan_rows(fdtd_sim *s) {
     int rc = upload(s, &s->aux_d, img);
     if (rc) return rc;
     if (const char *v = std::getenv("FDTD_CONCURRENT")) s->concurrent = std::atoi(v) != 0;
+    if (const char *v = std::getenv("FDTD_EARLY")) s->early = std::atoi(v) != 0;
+    if (const char *v = std::getenv("FDTD_TILES")) s->tiles = std::atoi(v) == 3 ? 3 : 1;
     if (s->concurrent && s->has_clean) {
         if (hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_line, hipEventDisableTiming) != hipSuccess)
             return set_err(s, FDTD_ERR_DEVICE, "second stream or events could not be created");
     }
     s->use_rows = s->has_clean;
@@ -407,18 +410,60 @@ RowsArgs make_rows(fdtd_sim *s) {
     return r;
 }
 
+// Whether the TFSF fix may run right after the clean-box kernel, before the
+// side pieces finish: its cells (planes t0-1, t0, t1, t1+1 of each axis over
+// [t0-1, t1+1] of the others) lie in the clean box in x and y, so no side
+// piece reads or writes them, and the fix reads no field values.
+bool tfsf_in_clean(const fdtd_sim *s) {
+    const Setup &S = s->S;
+    for (int a = 0; a < 2; a++)
+        if (S.tf0[a] - 1 < s->cbox[2 * a] || S.tf1[a] + 1 >= s->cbox[2 * a + 1]) return false;
+    return true;
+}
+
+// Whether the dispersive (or hydrodynamic) box may run right after the
+// clean-box kernel and ahead of the TFSF fix: the box with a 2-cell margin
+// (its kernels read fields one cell beyond their own) lies in the clean box in
+// x and y, and either strictly inside the TFSF planes on every axis or clear
+// of [t0-1, t1+1] on one, so it shares no cell with the fix in either order.
+bool disp_in_clean(const fdtd_sim *s) {
+    if (s->nbox <= 0) return false;
+    const Setup &S = s->S;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) { lo[a] = s->box_lo[a] - 2; hi[a] = s->box_lo[a] + s->box_n[a] + 2; }   // [lo, hi)
+    for (int a = 0; a < 2; a++)
+        if (lo[a] < s->cbox[2 * a] || hi[a] > s->cbox[2 * a + 1]) return false;
+    bool inside = true, clear = false;
+    for (int a = 0; a < 3; a++) {
+        if (lo[a] < S.tf0[a] + 1 || hi[a] - 1 > S.tf1[a] - 1) inside = false;
+        if (hi[a] - 1 < S.tf0[a] - 1 || lo[a] > S.tf1[a] + 1) clear = true;
+    }
+    return inside || clear;
+}
+
 // The row-blocked field update over independent cell sets: the clean
 // variant on the clean box, the x- and y-general variants on the slabs beside
-// it, and the general kernel (TFSF off) on the four corner columns.  With
-// FDTD_CONCURRENT=1 the side pieces run on a second stream.
-int launch_rows_update(fdtd_sim *s, const StepDev &d, hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
+// it, and the general kernel (TFSF off) on the four corner columns, then the
+// TFSF fix.  With a second stream (FDTD_CONCURRENT=1) the 1D line and the
+// side pieces run there beside the clean box, and the TFSF fix and the
+// dispersive box (dd, when given) follow the clean box on the main stream
+// without waiting for the side pieces wherever tfsf_in_clean and
+// disp_in_clean allow.  Returns with *disp_done set when the dispersive box
+// was launched here.
+int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const DispDev *dd, bool *disp_done,
+                       hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
     RowsArgs r = make_rows(s);
     const bool fork = s->stream2 != nullptr;
     hipStream_t side = fork ? s->stream2 : s->stream;
+    const bool tf_early = fork && s->early && tfsf_in_clean(s);
+    const bool disp_early = fork && s->early && dd && disp_in_clean(s);
+    *disp_done = false;
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
     }
+    HIPCHK(s, launch_line(l, side));
+    if (tf_early) HIPCHK(s, hipEventRecord(s->ev_line, side));
     if (s->nxbox) {
         RowsArgs rx = r;
         rx.ichunk = s->schunk;
@@ -437,10 +482,20 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, hipEvent_t ev_clean0, hipE
     if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
+    r.tiles = s->tiles == 3 && 3 * rows_waves(s->g.NZ) <= 16 ? 3 : 1;
     if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
     HIPCHK(s, launch_step_rows(r, s->rows, 0, s->stream));
     if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));
+    if (disp_early) {
+        HIPCHK(s, s->S.hydro ? launch_hydro(*dd, s->stream) : launch_dispersive(*dd, s->stream));
+        *disp_done = true;
+    }
+    if (tf_early) {
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
+        HIPCHK(s, launch_tfsf_fix(d, s->stream));
+    }
     if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    if (!tf_early) HIPCHK(s, launch_tfsf_fix(d, s->stream));
     return FDTD_OK;
 }
 
@@ -558,6 +613,7 @@ int fdtd_destroy(fdtd_sim *s) {
     if (s->stream2) (void)hipStreamSynchronize(s->stream2);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->ev_line) (void)hipEventDestroy(s->ev_line);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->stream2) (void)hipStreamDestroy(s->stream2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -588,6 +644,8 @@ int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
     g->sample_count = s->samples;
     g->fused_path = s->use_rows ? 1 : 0;
     for (int a = 0; a < 4; a++) g->clean[a] = s->use_rows && s->has_clean ? s->cbox[a] : -1;
+    const bool two = s->use_rows && s->stream2 && s->early;
+    g->schedule = (two && tfsf_in_clean(s) ? 1 : 0) | (two && disp_in_clean(s) ? 2 : 0);
     return FDTD_OK;
 }
 
@@ -1082,7 +1140,6 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
     if (!s || nsteps < 0) return FDTD_ERR_INVALID;
     for (int n = 0; n < nsteps; n++) {
         LineDev l = make_line(s);
-        HIPCHK(s, launch_line(l, s->stream));
         StepDev d = make_step(s, true);
         const int q = s->p, pp = s->pp;
         for (int c = 0; c < 3; c++) {
@@ -1098,24 +1155,25 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
             if (!ev0 || !ev1) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
         }
         const bool only_clean = s->timing_what == 2 && s->use_rows;
+        DispDev dd{};
+        if (s->nbox) dd = make_disp(s, 1 - q);
+        bool disp_done = false;
         if (s->use_rows) {
             d.tfsf_on = 0;
             if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev0, s->stream));
-            int rc2 = launch_rows_update(s, d, only_clean ? ev0 : nullptr, only_clean ? ev1 : nullptr);
+            int rc2 = launch_rows_update(s, d, l, s->nbox ? &dd : nullptr, &disp_done, only_clean ? ev0 : nullptr,
+                                         only_clean ? ev1 : nullptr);
             if (rc2) return rc2;
-            HIPCHK(s, launch_tfsf_fix(d, s->stream));
             if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev1, s->stream));
         } else {
+            HIPCHK(s, launch_line(l, s->stream));
             const int full[1][4] = {{0, s->g.NX, 0, s->g.NY}};
             if (s->timing) HIPCHK(s, hipEventRecord(ev0, s->stream));
             HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, full, 1, s->stream));
             if (s->timing) HIPCHK(s, hipEventRecord(ev1, s->stream));
         }
         if (s->timing) s->ev_used++;
-        if (s->nbox) {
-            DispDev dd = make_disp(s, 1 - q);
-            HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
-        }
+        if (s->nbox && !disp_done) HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
         if (s->probe) {
             const float *E1[3] = {s->E[1 - q][0], s->E[1 - q][1], s->E[1 - q][2]};
             HIPCHK(s, launch_probe(E1, s->probe_p, s->probe + 3 * (size_t)(s->t - s->probe_t0), s->stream));

@@ -655,29 +655,44 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 //
 // The body takes the block's index within its simulation's cell set (L), so
 // batched launches run it over many simulations.
-template <int W, int R, bool XG, bool YG, bool ADJ>
+//
+// T row tiles per block: waves T*W; tile wg / W takes row tile jt = T*jb + tile
+// and the tiles march the same x chunk in step (one barrier per x step for
+// the whole block).  A workgroup is placed only where every SIMD has
+// ceil(waves / 4) free wave slots, so at four waves per SIMD a 5-wave block
+// holds two per CU (10 waves) while one 15-wave block (T = 3) holds 15.
+// Tiles past the last row tile of the box repeat its loads and store nothing.
+template <int W, int R, bool XG, bool YG, bool ADJ, int T = 1>
 __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     static_assert(!ADJ || (!XG && !YG), "the adjoint variant runs clean boxes only");
-    __shared__ float sK[2][2][W][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
-    __shared__ float sE[2][2][W][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
+    static_assert(T >= 1 && W * T <= 16, "at most 16 waves per block");
+    __shared__ float sK[2][2][W * T][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
+    __shared__ float sE[2][2][W * T][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
     constexpr int kTab = kAuxPsi + (ADJ ? 48 : 36);
     __shared__ __align__(16) unsigned sT[kTab];
     const int NX = ra.NX, NY = ra.NY, NZ = ra.NZ;
-    for (int q = threadIdx.y * 64 + threadIdx.x; q < kTab; q += 64 * W) sT[q] = ra.aux[q];
+    for (int q = threadIdx.y * 64 + threadIdx.x; q < kTab; q += 64 * W * T) sT[q] = ra.aux[q];
     const float *sTf = (const float *)sT;
     int pc = 0;
     while (pc + 1 < ra.npiece && L >= ra.start[pc + 1]) pc++;
     L -= ra.start[pc];
     const int ia = ra.box[pc][0], ib = ra.box[pc][1], ja = ra.box[pc][2], jbox = ra.box[pc][3];
     const int njt = (jbox - ja + R - 1) / R;
-    const int jt = L % njt;
-    L /= njt;
+    const int njb = (njt + T - 1) / T;
+    const int jtb = L % njb;
+    L /= njb;
     const int seg = L % ra.nseg;
     const int ch = L / ra.nseg;
-    const int lane = threadIdx.x, w = threadIdx.y, t = w * 64 + lane;
+    // the wave index is uniform across a wave (blocks are 64 wide): keep it,
+    // and the row tile and row offsets derived from it, in scalar registers
+    const int lane = threadIdx.x, wg = __builtin_amdgcn_readfirstlane((int)threadIdx.y), w = T == 1 ? wg : wg % W;
+    const int t = w * 64 + lane;
+    const int jt0 = T == 1 ? jtb : jtb * T + wg / W;
+    const bool live = jt0 < njt;
+    const int jt = live ? jt0 : njt - 1;
     const int k = seg * (64 * W - 2) - 1 + t;
     const int kc = min(k, NZ - 1);
-    const bool kown = t > 0 && t < 64 * W - 1 && k < NZ;
+    const bool kown = live && t > 0 && t < 64 * W - 1 && k < NZ;
     const int i0 = ia + ch * ra.ichunk, i1 = min(i0 + ra.ichunk, ib);
     const int pp = ra.pp;
     auto hlay = [&](int a, int x) { return x < ra.Nc[a] ? x : (x >= ra.F[a] ? ra.Nc[a] + x - ra.F[a] : -1); };
@@ -735,7 +750,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
         for (int r = 0; r <= R; r++) { xk[r] = from_upper_lane(x[r]); yk[r] = from_upper_lane(y[r]); }
         if (last_lane) {
 #pragma unroll
-            for (int r = 0; r <= R; r++) { xk[r] = sE[b][0][w + 1][r]; yk[r] = sE[b][1][w + 1][r]; }
+            for (int r = 0; r <= R; r++) { xk[r] = sE[b][0][wg + 1][r]; yk[r] = sE[b][1][wg + 1][r]; }
         }
     };
     // adjoint: the first lane (outside the z slabs) publishes its z weights
@@ -743,8 +758,8 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
         if (lane == 0) {
 #pragma unroll
             for (int r = 0; r <= R; r++) {
-                sE[b][0][w][r] = ADJ ? (kC ? x[r] * ikz_e : 0.f) : x[r];
-                sE[b][1][w][r] = ADJ ? (kC ? y[r] * ikz_e : 0.f) : y[r];
+                sE[b][0][wg][r] = ADJ ? (kC ? x[r] * ikz_e : 0.f) : x[r];
+                sE[b][1][wg][r] = ADJ ? (kC ? y[r] * ikz_e : 0.f) : y[r];
             }
         }
     };
@@ -1022,7 +1037,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
 #pragma unroll
         for (int r = 1; r <= R; r++) {
             if (kown && jb + r < jbox) { stn<true>(oHx, pl + ro[r], hx[r]); stn<true>(oHy, pl + ro[r], hy[r]); stn<true>(oHz, pl + ro[r], hz[r]); }
-            if (lane == 63) { sK[buf][0][w][r] = sy[r]; sK[buf][1][w][r] = sx[r]; }
+            if (lane == 63) { sK[buf][0][wg][r] = sy[r]; sK[buf][1][wg][r] = sx[r]; }
         }
         e_publish(N.x, N.y, buf ^ 1);
         float hyk[R + 1], hxk[R + 1];
@@ -1031,7 +1046,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
         lds_sync();
         if (first_lane) {
 #pragma unroll
-            for (int r = 1; r <= R; r++) { hyk[r] = sK[buf][0][w - 1][r]; hxk[r] = sK[buf][1][w - 1][r]; }
+            for (int r = 1; r <= R; r++) { hyk[r] = sK[buf][0][wg - 1][r]; hxk[r] = sK[buf][1][wg - 1][r]; }
         }
         const bool iA = !XG || i < NX - 1, iC = !XG || (i > 0 && i < NX - 1);
         const float ikx_e = XG ? ik(0, true, i, NX) : ra.ik0[0][0];
@@ -1094,9 +1109,9 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
 
 // Two rows per thread fit in 128 VGPRs (four waves per SIMD); the cap costs
 // the x-general variant a few spilled registers and gains it a wave per SIMD.
-template <int W, int R, bool XG, bool YG, bool ADJ = false>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_step_rows(RowsArgs ra) {
-    rows_body<W, R, XG, YG, ADJ>(ra, ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : (int)blockIdx.x);
+template <int W, int R, bool XG, bool YG, bool ADJ = false, int T = 1>
+__global__ __launch_bounds__(64 * W * T) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_step_rows(RowsArgs ra) {
+    rows_body<W, R, XG, YG, ADJ, T>(ra, ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : (int)blockIdx.x);
 }
 
 // ----------------------------------------------------------------- split path kernels
@@ -2332,6 +2347,17 @@ hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st) {
 
 template <int W, bool XG, bool YG, bool ADJ>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
+    if (ra.tiles == 3) {
+        if constexpr (W * 3 <= 16 && !XG && !YG && !ADJ) {
+            switch (R) {
+                case 2: hipLaunchKernelGGL((k_step_rows<W, 2, false, false, false, 3>), grid, dim3(64, 3 * W), 0, st, ra); break;
+                case 3: hipLaunchKernelGGL((k_step_rows<W, 3, false, false, false, 3>), grid, dim3(64, 3 * W), 0, st, ra); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
     switch (R) {
         case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
         case 3: hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
@@ -2364,6 +2390,9 @@ int rows_waves(int NZ) {
 long rows_prepare(RowsArgs &ra, int R, int variant) {
     if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 3) return -1;
     const int W = rows_waves(ra.NZ);
+    if (ra.tiles < 1) ra.tiles = 1;
+    if (ra.tiles != 1 && (ra.tiles != 3 || 3 * W > 16 || variant != 0)) return -1;
+    const int T = ra.tiles;
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
     long total = 0;
     for (int q = 0; q < ra.npiece; q++) {
@@ -2372,7 +2401,7 @@ long rows_prepare(RowsArgs &ra, int R, int variant) {
         if (variant != 1 && (b[0] < 1 || b[1] > ra.NX - 1)) return -1;   // x must be clean
         if (variant != 2 && (b[2] < 1 || b[3] > ra.NY - 1)) return -1;   // y must be clean
         ra.start[q] = (int)total;
-        total += (long)((b[3] - b[2] + R - 1) / R) * ra.nseg * ((b[1] - b[0] + ra.ichunk - 1) / ra.ichunk);
+        total += (long)(((b[3] - b[2] + R - 1) / R + T - 1) / T) * ra.nseg * ((b[1] - b[0] + ra.ichunk - 1) / ra.ichunk);
     }
     if (total > 0x7fffffff) return -1;
     ra.start[ra.npiece] = (int)total;

@@ -72,6 +72,7 @@ class GridInfo(C.Structure):
         ("sample_count", C.c_int),
         ("fused_path", C.c_int),
         ("clean", C.c_int * 4),
+        ("schedule", C.c_int),
     ]
 
 

@@ -214,3 +214,55 @@ def test_point_monitor_gap_field_of_a_dimer():
     print("gap |E|^2", e2, "oracle", e2o, "err %.2e amp err %.2e" % (err, aerr), flush=True)
     assert s.get_info().n_dispersive[0] > 0
     assert err < 1e-4 and aerr < 1e-4
+
+
+@pytest.mark.parametrize("hydro", [0, 1])
+def test_step_schedules_bit_identical(hydro, monkeypatch):
+    """fdtd_step on two streams, with the TFSF fix-up and the dispersive box
+    following the clean box ahead of the side pieces (schedule 3), with them
+    after the join (FDTD_EARLY=0) and on one stream (FDTD_CONCURRENT=0): the
+    cells each kernel touches are disjoint, so every schedule gives the same
+    bits, DFT samples included."""
+    p = small_params(O, steps=300, block=8)
+    if hydro:
+        p.NONLOCAL = 3
+        p.hydro_beta = 1.27e6
+    runs = {}
+    for name, env in (("early", {}), ("late", {"FDTD_EARLY": "0"}), ("serial", {"FDTD_CONCURRENT": "0"})):
+        for k in ("FDTD_EARLY", "FDTD_CONCURRENT"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        s = fdtd_amd.Simulation(p)
+        info = s.get_info()
+        assert info.hydro == hydro and info.n_dispersive[0] > 0
+        s.step(300)
+        runs[name] = (info.schedule, [s.get_field(c) for c in range(6)], [s.get_dft(f, 0) for f in range(12)])
+    assert runs["early"][0] == 3, "the small sphere grid should take the early schedule"
+    assert runs["late"][0] == 0 and runs["serial"][0] == 0
+    _, ref, ref_dft = runs["serial"]
+    for name in ("early", "late"):
+        _, fields, dft = runs[name]
+        for c in range(6):
+            assert np.array_equal(fields[c], ref[c]), (name, c)
+        for f in range(12):
+            assert np.array_equal(dft[f], ref_dft[f]), (name, f)
+
+
+@pytest.mark.parametrize("ny,rows", [(24, 2), (26, 2), (28, 2), (26, 3)])
+def test_row_tiles_per_block_bit_identical(ny, rows, monkeypatch):
+    """Three row tiles per block on the clean box (FDTD_TILES=3) against one:
+    the same cells with the same arithmetic, so the same bits, also when the
+    row-tile count is not a multiple of three (idle tiles store nothing)."""
+    p = small_params(O, steps=300)
+    p.NCELLY = ny
+    monkeypatch.setenv("FDTD_ROWS", str(rows))
+    out = {}
+    for tiles in ("1", "3"):
+        monkeypatch.setenv("FDTD_TILES", tiles)
+        s = fdtd_amd.Simulation(p)
+        s.step(300)
+        out[tiles] = [s.get_field(c) for c in range(6)]
+    assert np.abs(out["3"][0]).max() > 0
+    for c in range(6):
+        assert np.array_equal(out["1"][c], out["3"][c]), c
