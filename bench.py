@@ -186,7 +186,7 @@ def main():
             "dispersive_cells": list(info.n_dispersive[:]),
             "tile": {"tj": int(os.environ.get("FDTD_TJ", 8)), "ichunk": int(os.environ.get("FDTD_ICHUNK", 32)),
                      "rows": int(os.environ.get("FDTD_ROWS", 2)), "cchunk": int(os.environ.get("FDTD_CCHUNK", 16)),
-                     "schunk": int(os.environ.get("FDTD_SCHUNK", 8)),
+                     "schunk": int(os.environ.get("FDTD_SCHUNK", 16)),
                      "side_stream": int(os.environ.get("FDTD_CONCURRENT", 1)), "xcd_order": int(os.environ.get("FDTD_XCD", 0)),
                      "schedule": info.schedule},
         },
@@ -204,11 +204,14 @@ def main():
             "launches": k_n,
         },
         "field_update": {
-            # the window runs from the fork to the join of the two streams: with
-            # schedule bit 1 it also holds the 1D line and the dispersive box,
-            # whose bytes are not counted (the figure is a lower bound)
-            "kernels": ("k_line + k_step_rows + k_fused_step (corners) + k_tfsf_fix"
-                        + (" + k_dispersive" if info.schedule & 2 else "")) if rows else "k_fused_step",
+            # the window runs from the fork to the join of the two streams: it
+            # also holds the 1D line and, with schedule bits 1 and 2, the
+            # dispersive box and the DFT sample (every fourth step), whose bytes
+            # are not counted (the figure is a lower bound)
+            "kernels": ("k_line + k_step_rows + k_fused_step (corners) + "
+                        + ("k_post (TFSF fix-up" + (" + dispersive box" if info.schedule & 2 else "")
+                           + (" + DFT sample" if info.schedule & 4 else "") + ")"
+                           if info.schedule & 6 else "k_tfsf_fix")) if rows else "k_fused_step",
             "avg_ms": round(u_avg_s * 1e3, 5),
             "bytes_per_step": abytes,
             "achieved_GBs": round(abytes / u_avg_s / 1e9, 1),

@@ -441,6 +441,51 @@ bool disp_in_clean(const fdtd_sim *s) {
     return inside || clear;
 }
 
+// Whether the box [lo, hi) meets a cell of the TFSF fix-up: planes t0-1, t0,
+// t1, t1+1 of each axis over [t0-1, t1+1] of the other two.
+static bool meets_tfsf(const Setup &S, const int lo[3], const int hi[3]) {
+    for (int a = 0; a < 3; a++) {
+        const int xs[4] = {S.tf0[a] - 1, S.tf0[a], S.tf1[a], S.tf1[a] + 1};
+        for (int x : xs) {
+            if (x < lo[a] || x >= hi[a]) continue;
+            bool all = true;
+            for (int b = 0; b < 3; b++)
+                if (b != a && (hi[b] <= S.tf0[b] - 1 || lo[b] >= S.tf1[b] + 2)) all = false;
+            if (all) return true;
+        }
+    }
+    return false;
+}
+
+// Whether the DFT sample may run with the post-update work, before the side
+// pieces finish: every monitor face lies in the clean box in x and y (no
+// side piece writes it) and shares no cell with the TFSF fix-up or with the
+// dispersive box and its 2-cell margin, so it reads the final fields.
+bool sample_in_clean(const fdtd_sim *s) {
+    const FaceDev &f = s->fd;
+    for (int q = 0; q < 12; q++) {
+        const int axis = (q >> 1) % 3;
+        int lo[3], hi[3];
+        const int ra[2] = {f.lo_a[q], f.lo_a[q] + f.na[q]}, rb[2] = {f.lo_b[q], f.lo_b[q] + f.nb[q]};
+        // in-plane axes: x (y, z), y (x, z), z (x, y), as the sample kernel reads them
+        const int oa = axis == 0 ? 1 : 0, ob = axis == 2 ? 1 : 2;
+        lo[axis] = f.plane[q]; hi[axis] = f.plane[q] + 1;
+        lo[oa] = ra[0]; hi[oa] = ra[1];
+        lo[ob] = rb[0]; hi[ob] = rb[1];
+        if (f.na[q] <= 0 || f.nb[q] <= 0) continue;
+        for (int a = 0; a < 2; a++)
+            if (lo[a] < s->cbox[2 * a] || hi[a] > s->cbox[2 * a + 1]) return false;
+        if (meets_tfsf(s->S, lo, hi)) return false;
+        if (s->nbox > 0) {
+            bool apart = false;
+            for (int a = 0; a < 3; a++)
+                if (hi[a] <= s->box_lo[a] - 2 || lo[a] >= s->box_lo[a] + s->box_n[a] + 2) apart = true;
+            if (!apart) return false;
+        }
+    }
+    return true;
+}
+
 // The row-blocked field update over independent cell sets: the clean
 // variant on the clean box, the x- and y-general variants on the slabs beside
 // it, and the general kernel (TFSF off) on the four corner columns, then the
@@ -448,16 +493,19 @@ bool disp_in_clean(const fdtd_sim *s) {
 // side pieces run there beside the clean box, and the TFSF fix and the
 // dispersive box (dd, when given) follow the clean box on the main stream
 // without waiting for the side pieces wherever tfsf_in_clean and
-// disp_in_clean allow.  Returns with *disp_done set when the dispersive box
-// was launched here.
-int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const DispDev *dd, bool *disp_done,
-                       hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
+// disp_in_clean allow, in one launch together with the DFT sample of the
+// step (samp; d.sample_row set) where sample_in_clean allows.  Returns with
+// *disp_done and *samp_done set when the dispersive box and the sample were
+// launched here.
+int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const DispDev *dd, bool samp, bool *disp_done,
+                       bool *samp_done, hipEvent_t ev_clean0, hipEvent_t ev_clean1) {
     RowsArgs r = make_rows(s);
     const bool fork = s->stream2 != nullptr;
     hipStream_t side = fork ? s->stream2 : s->stream;
     const bool tf_early = fork && s->early && tfsf_in_clean(s);
     const bool disp_early = fork && s->early && dd && disp_in_clean(s);
     *disp_done = false;
+    *samp_done = false;
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
@@ -486,13 +534,18 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
     HIPCHK(s, launch_step_rows(r, s->rows, 0, s->stream));
     if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));
-    if (disp_early) {
+    if (tf_early) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
+    const bool merge_disp = disp_early && tf_early && !s->S.hydro;
+    if (disp_early && !merge_disp) {
         HIPCHK(s, s->S.hydro ? launch_hydro(*dd, s->stream) : launch_dispersive(*dd, s->stream));
         *disp_done = true;
     }
+    const bool merge_samp = tf_early && samp && d.sample_row && (s->nbox <= 0 || disp_early) && sample_in_clean(s);
     if (tf_early) {
-        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
-        HIPCHK(s, launch_tfsf_fix(d, s->stream));
+        if (merge_disp || merge_samp) HIPCHK(s, launch_post(d, merge_disp ? dd : nullptr, merge_samp, s->stream));
+        else HIPCHK(s, launch_tfsf_fix(d, s->stream));
+        *disp_done = *disp_done || merge_disp;
+        *samp_done = merge_samp;
     }
     if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     if (!tf_early) HIPCHK(s, launch_tfsf_fix(d, s->stream));
@@ -645,7 +698,8 @@ int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
     g->fused_path = s->use_rows ? 1 : 0;
     for (int a = 0; a < 4; a++) g->clean[a] = s->use_rows && s->has_clean ? s->cbox[a] : -1;
     const bool two = s->use_rows && s->stream2 && s->early;
-    g->schedule = (two && tfsf_in_clean(s) ? 1 : 0) | (two && disp_in_clean(s) ? 2 : 0);
+    const bool tf = two && tfsf_in_clean(s), dp = two && disp_in_clean(s);
+    g->schedule = (tf ? 1 : 0) | (dp ? 2 : 0) | (tf && (s->nbox <= 0 || dp) && sample_in_clean(s) ? 4 : 0);
     return FDTD_OK;
 }
 
@@ -1157,12 +1211,13 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         const bool only_clean = s->timing_what == 2 && s->use_rows;
         DispDev dd{};
         if (s->nbox) dd = make_disp(s, 1 - q);
-        bool disp_done = false;
+        bool disp_done = false, samp_done = false;
         if (s->use_rows) {
             d.tfsf_on = 0;
+            if (samp) d.sample_row = s->X + (size_t)s->slot * s->mtot;
             if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev0, s->stream));
-            int rc2 = launch_rows_update(s, d, l, s->nbox ? &dd : nullptr, &disp_done, only_clean ? ev0 : nullptr,
-                                         only_clean ? ev1 : nullptr);
+            int rc2 = launch_rows_update(s, d, l, s->nbox ? &dd : nullptr, samp, &disp_done, &samp_done,
+                                         only_clean ? ev0 : nullptr, only_clean ? ev1 : nullptr);
             if (rc2) return rc2;
             if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev1, s->stream));
         } else {
@@ -1186,11 +1241,14 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         s->pp ^= 1;
         s->le ^= 1;
         if (samp) {
-            // tangential E^{n+1} and H^{n+1/2} on the monitor faces (ScattAbs, main.cu:2920)
-            StepDev ds = make_step(s, false);
-            for (int c = 0; c < 3; c++) { ds.Ein[c] = s->E[s->p][c]; ds.Hin[c] = s->H[s->p][c]; }
-            ds.sample_row = s->X + (size_t)s->slot * s->mtot;
-            HIPCHK(s, launch_dft_sample_split(ds, s->stream));
+            // tangential E^{n+1} and H^{n+1/2} on the monitor faces (ScattAbs, main.cu:2920), unless the
+            // post-update launch took them
+            if (!samp_done) {
+                StepDev ds = make_step(s, false);
+                for (int c = 0; c < 3; c++) { ds.Ein[c] = s->E[s->p][c]; ds.Hin[c] = s->H[s->p][c]; }
+                ds.sample_row = s->X + (size_t)s->slot * s->mtot;
+                HIPCHK(s, launch_dft_sample_split(ds, s->stream));
+            }
             int rc = after_sample(s);
             if (rc) return rc;
         }

@@ -224,6 +224,10 @@ hipError_t launch_tfsf_split(const StepDev &s, hipStream_t st);
 hipError_t launch_tfsf_fix(const StepDev &s, hipStream_t st);
 hipError_t launch_line(const LineDev &l, hipStream_t st);
 hipError_t launch_dispersive(const DispDev &d, hipStream_t st);
+// launch_tfsf_fix, launch_dispersive (Drude + CP, when d is given) and, with
+// sample, the DFT sample of s.Eout / s.Hout into s.sample_row, in one launch;
+// only for cell sets that are pairwise disjoint.
+hipError_t launch_post(const StepDev &s, const DispDev *d, bool sample, hipStream_t st);
 hipError_t launch_dft_sample_split(const StepDev &s, hipStream_t st);
 hipError_t launch_twiddle(double *W, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
                           double dt, hipStream_t st);

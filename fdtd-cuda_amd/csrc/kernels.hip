@@ -1178,12 +1178,12 @@ __global__ void k_tfsf_split(StepDev s) {
 // Cells: per normal axis a the planes t0-1, t0, t1, t1+1 spanning [t0-1, t1+1]
 // along the other axes; a cell lying on planes of several axes is taken by
 // the lowest one.  Requires the region free of CPML and inside every domain.
-__device__ __forceinline__ void tfsf_fix_body(const StepDev &s, int bx, int by, int bz) {
+__device__ __forceinline__ void tfsf_fix_body(const StepDev &s, int bx, int by, int bz, int lane) {
     const TfsfDev &t = s.tf;
     const int a = bz >> 2, q = bz & 3;
     const int b0 = a == 0 ? 1 : 0, b1 = a == 2 ? 1 : 2;   // other axes, b1 fastest in memory
     const int u = pick3(b0, t.t0[0], t.t0[1], t.t0[2]) - 1 + by;
-    const int v = pick3(b1, t.t0[0], t.t0[1], t.t0[2]) - 1 + bx * 64 + (int)threadIdx.x;
+    const int v = pick3(b1, t.t0[0], t.t0[1], t.t0[2]) - 1 + bx * 64 + lane;
     if (u > pick3(b0, t.t1[0], t.t1[1], t.t1[2]) + 1 || v > pick3(b1, t.t1[0], t.t1[1], t.t1[2]) + 1) return;
     const int t0a = pick3(a, t.t0[0], t.t0[1], t.t0[2]), t1a = pick3(a, t.t1[0], t.t1[1], t.t1[2]);
     const int xa = q == 0 ? t0a - 1 : (q == 1 ? t0a : (q == 2 ? t1a : t1a + 1));
@@ -1214,7 +1214,7 @@ __device__ __forceinline__ void tfsf_fix_body(const StepDev &s, int bx, int by, 
     if (fz != 0.f) s.Eout[2][p] += fz;
 }
 
-__global__ __launch_bounds__(64) void k_tfsf_fix(StepDev s) { tfsf_fix_body(s, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(64) void k_tfsf_fix(StepDev s) { tfsf_fix_body(s, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x); }
 
 // One thread per sample column of the E half (H taken at the same cell).
 __device__ __forceinline__ void sample_body(const Geom &g, const FaceDev &f, const float *const *E, const float *const *H,
@@ -1304,6 +1304,29 @@ __device__ __forceinline__ void disp_body(const DispDev &d, long idx) {
 }
 
 __global__ void k_dispersive(DispDev d) { disp_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x); }
+
+// The work that follows the field update of a step in one launch, for cell
+// sets that are pairwise disjoint (the TFSF fix-up adds to its own cells only
+// and reads no field; the dispersive box reads H beside its cells and
+// rewrites E in them; the DFT sample reads E and H on the monitor faces):
+// blocks [0, nd) run the dispersive box, 256 cells each, the next nt the
+// fix-up, one 64-lane row of its planes per wave, the last ns the sample of
+// the new fields (s.Eout, s.Hout) into s.sample_row, 256 columns each.
+__global__ __launch_bounds__(256) void k_post(StepDev s, DispDev d, int nd, int nbx, int span, int nt, int ns) {
+    const int b = (int)blockIdx.x;
+    if (b < nd) {
+        disp_body(d, (long)b * 256 + threadIdx.x);
+        return;
+    }
+    if (b < nd + nt) {
+        const int L = (b - nd) * 4 + (int)(threadIdx.x >> 6);
+        if (L >= nbx * span * 12) return;
+        tfsf_fix_body(s, L % nbx, (L / nbx) % span, L / (nbx * span), threadIdx.x & 63);
+        return;
+    }
+    if (b < nd + nt + ns)
+        sample_body(s.g, s.fd, s.Eout, s.Hout, s.sample_row, (long)(b - nd - nt) * 256 + threadIdx.x);
+}
 
 // ----------------------------------------------------------------- hydrodynamic model (main.cu:1916-2276)
 
@@ -2215,7 +2238,7 @@ __global__ __launch_bounds__(64) void k_tfsf_fix_batch(const FusedItem *items, c
     for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
     const int nbx = (span + 63) / 64;
     const int L = (int)(m & ((1u << kBatchLocalBits) - 1));
-    tfsf_fix_body(it.s, L % nbx, (L / nbx) % span, L / (nbx * span));
+    tfsf_fix_body(it.s, L % nbx, (L / nbx) % span, L / (nbx * span), threadIdx.x);
 }
 
 __global__ __launch_bounds__(1024) void k_line_batch(const LineDev *items, const int *dyn, int step_off) {
@@ -2497,6 +2520,21 @@ hipError_t launch_tfsf_fix(const StepDev &s, hipStream_t st) {
     for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
     dim3 grid((span + 63) / 64, span, 12), block(64);   // the body assumes 64-thread blocks
     hipLaunchKernelGGL(k_tfsf_fix, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_post(const StepDev &s, const DispDev *d, bool sample, hipStream_t st) {
+    static_assert(sizeof(StepDev) + sizeof(DispDev) + 32 <= 4096, "kernel arguments");
+    const TfsfDev &t = s.tf;
+    int span = 0;
+    for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
+    const int nbx = (span + 63) / 64;
+    const long nb = d ? (long)d->n[0] * d->n[1] * d->n[2] : 0;
+    const long nd = (nb + 255) / 256, nt = ((long)nbx * span * 12 + 3) / 4, ns = sample ? (s.fd.mE + 255) / 256 : 0;
+    if (nd + nt + ns > 0x7fffffff) return hipErrorInvalidValue;
+    const DispDev none{};
+    hipLaunchKernelGGL(k_post, dim3((unsigned)(nd + nt + ns)), dim3(256), 0, st, s, d ? *d : none, (int)nd, nbx, span,
+                       (int)nt, (int)ns);
     return hipGetLastError();
 }
 

@@ -96,7 +96,7 @@ struct fdtd_sim {
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
     int rows = 2, cchunk = 16;      // own rows per thread and x chunk of the row-blocked kernel (clean box)
     int tiles = 1;                  // row tiles per block of the clean-box launch (FDTD_TILES: 1 or 3; 3 needs 3 W <= 16)
-    int schunk = 8;                 // x chunk of the side slabs
+    int schunk = 16;                // x chunk of the side slabs
     int adj_schunk = 8;             // x chunk of the side slabs in the transposed step (FDTD_ADJ_SCHUNK)
     int adj_stj = 8;                // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)

@@ -108,8 +108,8 @@ typedef struct fdtd_grid_info {
     int    clean[4];           /* x [clean0, clean1) by y [clean2, clean3), all z: cells run by the
                                   CPML-free variant of the row-blocked kernel; -1 when none       */
     int    schedule;           /* fdtd_step on two streams: bit 0 the TFSF fix-up, bit 1 the
-                                  dispersive box follows the clean box without waiting for the
-                                  side pieces (no reference counterpart)                          */
+                                  dispersive box, bit 2 the DFT sample follow the clean box without
+                                  waiting for the side pieces (no reference counterpart)          */
 } fdtd_grid_info;
 
 typedef struct fdtd_sim fdtd_sim;

@@ -216,14 +216,20 @@ def test_point_monitor_gap_field_of_a_dimer():
     assert err < 1e-4 and aerr < 1e-4
 
 
-@pytest.mark.parametrize("hydro", [0, 1])
-def test_step_schedules_bit_identical(hydro, monkeypatch):
+@pytest.mark.parametrize("hydro,radius,offsets,schedule", [(0, 8e-9, (2, 2), 3), (0, 4e-9, (3, 1), 7),
+                                                           (1, 8e-9, (2, 2), 3), (1, 4e-9, (3, 1), 7)])
+def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypatch):
     """fdtd_step on two streams, with the TFSF fix-up and the dispersive box
-    following the clean box ahead of the side pieces (schedule 3), with them
-    after the join (FDTD_EARLY=0) and on one stream (FDTD_CONCURRENT=0): the
-    cells each kernel touches are disjoint, so every schedule gives the same
-    bits, DFT samples included."""
-    p = small_params(O, steps=300, block=8)
+    (schedule bits 0, 1) and the DFT sample (bit 2) following the clean box
+    ahead of the side pieces, in one launch with the Drude metal, against
+    them after the join (FDTD_EARLY=0) and on one stream (FDTD_CONCURRENT=0):
+    the cells each kernel touches are disjoint, so every schedule gives the
+    same bits, DFT spectra included.  The sample joins only when its faces
+    clear the TFSF planes (scattering offset 2 puts the far face on plane
+    t1+1; 3 clears it) and the dispersive box with its margin (the 4 nm
+    sphere's box clears absorption faces at offset 1)."""
+    p = small_params(O, steps=300, block=8, radius=radius)
+    p.sca_offset, p.abs_offset = offsets
     if hydro:
         p.NONLOCAL = 3
         p.hydro_beta = 1.27e6
@@ -238,7 +244,7 @@ def test_step_schedules_bit_identical(hydro, monkeypatch):
         assert info.hydro == hydro and info.n_dispersive[0] > 0
         s.step(300)
         runs[name] = (info.schedule, [s.get_field(c) for c in range(6)], [s.get_dft(f, 0) for f in range(12)])
-    assert runs["early"][0] == 3, "the small sphere grid should take the early schedule"
+    assert runs["early"][0] == schedule, runs["early"][0]
     assert runs["late"][0] == 0 and runs["serial"][0] == 0
     _, ref, ref_dft = runs["serial"]
     for name in ("early", "late"):
