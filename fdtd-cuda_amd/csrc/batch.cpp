@@ -63,6 +63,11 @@ struct fdtd_batch {
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
     std::vector<void *> allocs;
+    // second stream for the lines, side slabs and corner columns beside the
+    // clean boxes (FDTD_CONCURRENT=0: one stream); the fork and join are graph
+    // edges in a captured period
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -371,20 +376,41 @@ int upload_items(fdtd_batch *b) {
 }
 
 // One step for every simulation: par = parity of its index, off = its offset
-// from dyn's step, sidx = ring row offset of its sample or -1.
+// from dyn's step, sidx = ring row offset of its sample or -1.  The lines,
+// the side slabs and the general-kernel launches (corner columns, or whole
+// grids off the row path) run on the second stream beside the clean boxes:
+// each writes only its own cells and reads the previous step's fields.  The
+// TFSF fix-ups, dispersive boxes and samples follow the join.
 int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     const int n = (int)b->sims.size();
-    BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, b->stream));
+    const bool fork = b->stream2 != nullptr;
+    hipStream_t side = fork ? b->stream2 : b->stream;
+    if (fork) {
+        BCHK(b, hipEventRecord(b->ev_fork, b->stream));
+        BCHK(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+    }
+    BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, side));
+    for (const Launch &l : b->launches) {
+        const int nb = (int)l.map.size();
+        if (l.type == Launch::Rows && l.c != 0) BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, side));
+        if (l.type == Launch::Fused) BCHK(b, launch_fused_batch(b->fused[par], l.d, nb, l.a, side));
+    }
+    for (const Launch &l : b->launches)
+        if (l.type == Launch::Rows && l.c == 0)
+            BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, (int)l.map.size(), l.a, l.b, l.c, b->stream));
+    if (fork) {
+        BCHK(b, hipEventRecord(b->ev_join, b->stream2));
+        BCHK(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    }
     for (const Launch &l : b->launches) {
         const int nb = (int)l.map.size();
         switch (l.type) {
-            case Launch::Rows: BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, b->stream)); break;
-            case Launch::Fused: BCHK(b, launch_fused_batch(b->fused[par], l.d, nb, l.a, b->stream)); break;
             case Launch::Tfsf: BCHK(b, launch_tfsf_fix_batch(b->fused[par], l.d, nb, b->stream)); break;
             case Launch::Disp: BCHK(b, launch_disp_batch(b->disp[par], l.d, nb, b->stream)); break;
-            default:
+            case Launch::Sample:
                 if (sidx >= 0) BCHK(b, launch_sample_batch(b->samp[par], l.d, nb, b->dyn, sidx, b->stream));
                 break;
+            default: break;
         }
     }
     return FDTD_OK;
@@ -488,6 +514,13 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
     if (int rc = check_sims(b)) return fail(rc);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(berr(b, FDTD_ERR_DEVICE, "hipStreamCreate failed"));
+    const char *conc = std::getenv("FDTD_CONCURRENT");
+    if (!conc || std::atoi(conc) != 0) {
+        if (hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+            return fail(berr(b, FDTD_ERR_DEVICE, "second stream or events could not be created"));
+    }
     int rc;
     for (int par = 0; par < 2; par++) {
         if ((rc = balloc(b, &b->fused[par], n)) || (rc = balloc(b, &b->line[par], n)) || (rc = balloc(b, &b->disp[par], n)) ||
@@ -579,9 +612,13 @@ const char *fdtd_batch_last_error(const fdtd_batch *b) { return b ? b->err.c_str
 int fdtd_batch_destroy(fdtd_batch *b) {
     if (!b) return FDTD_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : b->allocs) (void)hipFree(p);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return FDTD_OK;

@@ -278,6 +278,7 @@ hipError_t launch_rows_batch(const RowsArgs *items, const unsigned *map, int nbl
                              hipStream_t st);
 // TFSF fix-up over many simulations (item.s as for launch_tfsf_fix).
 hipError_t launch_tfsf_fix_batch(const FusedItem *items, const unsigned *map, int nblocks, hipStream_t st);
+// Blocks of launch_tfsf_fix_batch for one simulation (four 64-lane rows each).
 long tfsf_fix_blocks(const StepDev &s);
 // Fills nseg and start of a RowsArgs for R and variant; returns its block count or -1.
 long rows_prepare(RowsArgs &ra, int R, int variant);

@@ -2230,15 +2230,17 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(R == 2 ?
 }
 
 // Local block L of a simulation's TFSF fix-up grid ((span + 63) / 64, span, 12) in x-fastest order.
-__global__ __launch_bounds__(64) void k_tfsf_fix_batch(const FusedItem *items, const unsigned *map) {
+// Four 64-lane rows of the fix-up's planes per block (one per wave).
+__global__ __launch_bounds__(256) void k_tfsf_fix_batch(const FusedItem *items, const unsigned *map) {
     const unsigned m = batch_map(map);
     const FusedItem &it = batch_item(items, m >> kBatchLocalBits);
     const TfsfDev &t = it.s.tf;
     int span = 0;
     for (int a = 0; a < 3; a++) span = max(span, t.t1[a] - t.t0[a] + 3);
     const int nbx = (span + 63) / 64;
-    const int L = (int)(m & ((1u << kBatchLocalBits) - 1));
-    tfsf_fix_body(it.s, L % nbx, (L / nbx) % span, L / (nbx * span), threadIdx.x);
+    const int L = (int)(m & ((1u << kBatchLocalBits) - 1)) * 4 + (int)(threadIdx.x >> 6);
+    if (L >= nbx * span * 12) return;
+    tfsf_fix_body(it.s, L % nbx, (L / nbx) % span, L / (nbx * span), threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(1024) void k_line_batch(const LineDev *items, const int *dyn, int step_off) {
@@ -2334,14 +2336,14 @@ hipError_t launch_rows_batch(const RowsArgs *items, const unsigned *map, int nbl
 
 hipError_t launch_tfsf_fix_batch(const FusedItem *items, const unsigned *map, int nblocks, hipStream_t st) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tfsf_fix_batch, dim3(nblocks), dim3(64), 0, st, items, map);
+    hipLaunchKernelGGL(k_tfsf_fix_batch, dim3(nblocks), dim3(256), 0, st, items, map);
     return hipGetLastError();
 }
 
 long tfsf_fix_blocks(const StepDev &s) {
     int span = 0;
     for (int a = 0; a < 3; a++) span = max(span, s.tf.t1[a] - s.tf.t0[a] + 3);
-    return (long)((span + 63) / 64) * span * 12;
+    return ((long)((span + 63) / 64) * span * 12 + 3) / 4;
 }
 
 hipError_t launch_line_batch(const LineDev *items, int n, const int *dyn, int step_off, hipStream_t st) {
