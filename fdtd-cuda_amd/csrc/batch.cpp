@@ -67,7 +67,11 @@ struct fdtd_batch {
     // clean boxes (FDTD_CONCURRENT=0: one stream); the fork and join are graph
     // edges in a captured period
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_line = nullptr;
+    // TFSF fix-ups (every row-path simulation's planes in its clean box) and
+    // dispersive boxes (every simulation on the row path, every box in its
+    // clean box) follow the clean boxes without waiting for the side pieces
+    bool tf_early = false, dp_early = false;
 };
 
 namespace {
@@ -362,6 +366,12 @@ int upload_items(fdtd_batch *b) {
         BCHK(b, hipMemcpy(b->disp[par], di.data(), n * sizeof(DispDev), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->samp[par], si.data(), n * sizeof(SampleItem), hipMemcpyHostToDevice));
     }
+    b->tf_early = b->dp_early = true;
+    for (int i = 0; i < n; i++) {
+        fdtd_sim *s = b->sims[i];
+        if (b->rowpath[i] && !tfsf_in_clean(s)) b->tf_early = false;
+        if (!b->rowpath[i] || (s->nbox > 0 && !disp_in_clean(s))) b->dp_early = false;
+    }
     bool same = ls.size() == b->launches.size();
     for (size_t q = 0; same && q < ls.size(); q++) same = ls[q].same(b->launches[q]);
     if (same) return FDTD_OK;
@@ -380,16 +390,20 @@ int upload_items(fdtd_batch *b) {
 // the side slabs and the general-kernel launches (corner columns, or whole
 // grids off the row path) run on the second stream beside the clean boxes:
 // each writes only its own cells and reads the previous step's fields.  The
-// TFSF fix-ups, dispersive boxes and samples follow the join.
+// TFSF fix-ups and the dispersive boxes follow the clean boxes on the main
+// stream, ahead of the join where tf_early and dp_early allow (the same
+// order as after it); the samples follow the join.
 int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     const int n = (int)b->sims.size();
     const bool fork = b->stream2 != nullptr;
     hipStream_t side = fork ? b->stream2 : b->stream;
+    const bool tf_early = fork && b->tf_early, dp_early = tf_early && b->dp_early;
     if (fork) {
         BCHK(b, hipEventRecord(b->ev_fork, b->stream));
         BCHK(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
     }
     BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, side));
+    if (tf_early) BCHK(b, hipEventRecord(b->ev_line, side));
     for (const Launch &l : b->launches) {
         const int nb = (int)l.map.size();
         if (l.type == Launch::Rows && l.c != 0) BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, side));
@@ -398,22 +412,25 @@ int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     for (const Launch &l : b->launches)
         if (l.type == Launch::Rows && l.c == 0)
             BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, (int)l.map.size(), l.a, l.b, l.c, b->stream));
+    auto post = [&](bool early) -> int {
+        for (const Launch &l : b->launches) {
+            const int nb = (int)l.map.size();
+            if (l.type == Launch::Tfsf && early == tf_early) BCHK(b, launch_tfsf_fix_batch(b->fused[par], l.d, nb, b->stream));
+            if (l.type == Launch::Disp && early == dp_early) BCHK(b, launch_disp_batch(b->disp[par], l.d, nb, b->stream));
+            if (l.type == Launch::Sample && !early && sidx >= 0)
+                BCHK(b, launch_sample_batch(b->samp[par], l.d, nb, b->dyn, sidx, b->stream));
+        }
+        return FDTD_OK;
+    };
+    if (tf_early) {
+        BCHK(b, hipStreamWaitEvent(b->stream, b->ev_line, 0));
+        if (int rc = post(true)) return rc;
+    }
     if (fork) {
         BCHK(b, hipEventRecord(b->ev_join, b->stream2));
         BCHK(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
     }
-    for (const Launch &l : b->launches) {
-        const int nb = (int)l.map.size();
-        switch (l.type) {
-            case Launch::Tfsf: BCHK(b, launch_tfsf_fix_batch(b->fused[par], l.d, nb, b->stream)); break;
-            case Launch::Disp: BCHK(b, launch_disp_batch(b->disp[par], l.d, nb, b->stream)); break;
-            case Launch::Sample:
-                if (sidx >= 0) BCHK(b, launch_sample_batch(b->samp[par], l.d, nb, b->dyn, sidx, b->stream));
-                break;
-            default: break;
-        }
-    }
-    return FDTD_OK;
+    return post(false);
 }
 
 // Host bookkeeping of one step of every simulation (what fdtd_step does after its launches).
@@ -518,7 +535,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
     if (!conc || std::atoi(conc) != 0) {
         if (hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_line, hipEventDisableTiming) != hipSuccess)
             return fail(berr(b, FDTD_ERR_DEVICE, "second stream or events could not be created"));
     }
     int rc;
@@ -618,6 +636,7 @@ int fdtd_batch_destroy(fdtd_batch *b) {
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev_line) (void)hipEventDestroy(b->ev_line);
     if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -150,6 +150,10 @@ LineDev make_line(fdtd_sim *s);
 DispDev make_disp(fdtd_sim *s, int parity);
 // Row-blocked kernel arguments at the handle's parity (clean-box chunk; boxes and R set by the caller).
 RowsArgs make_rows(fdtd_sim *s);
+// Cell-set tests of the two-stream schedule (the clean box and the TFSF,
+// dispersive and monitor cells; see api.cpp).
+bool tfsf_in_clean(const fdtd_sim *s);
+bool disp_in_clean(const fdtd_sim *s);
 // DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.
 bool sampling_step(const fdtd_sim *s);
 int after_sample(fdtd_sim *s);
