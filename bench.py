@@ -156,6 +156,10 @@ def main():
     u_avg_s = (u_ms / 1e3) / max(u_n, 1)
 
     sim.close()
+    # the same kernel with the side pieces on the main stream after it (one
+    # stream), so its duration is not shared with them: reported beside the
+    # roofline figure, which is the production two-stream schedule
+    serial = serial_kernel_ms(p, max(64, args.steps // 8)) if cbytes else None
     if args.calibrate_copy:
         copy_calibration()
     sweep = sweep_bench(args.sweep_steps) if args.sweep_steps > 0 else None
@@ -202,6 +206,8 @@ def main():
             "bytes_per_launch": kbytes,
             "kernel_avg_ms": round(k_avg_s * 1e3, 5),
             "launches": k_n,
+            "one_stream_kernel_avg_ms": round(serial, 5) if serial else None,
+            "one_stream_frac": round(kbytes / (serial / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if serial else None,
         },
         "field_update": {
             # the window runs from the fork to the join of the two streams: it
@@ -223,6 +229,29 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def serial_kernel_ms(p, steps):
+    """Average HIP-event duration of the clean-box launch with every kernel of
+    the step on one stream (FDTD_CONCURRENT=0), on the same workload."""
+    import fdtd_amd
+    old = os.environ.get("FDTD_CONCURRENT")
+    os.environ["FDTD_CONCURRENT"] = "0"
+    try:
+        sim = fdtd_amd.Simulation(p)
+    finally:
+        if old is None:
+            del os.environ["FDTD_CONCURRENT"]
+        else:
+            os.environ["FDTD_CONCURRENT"] = old
+    for c in dimer_centres(sim.info.NX):
+        sim.add_sphere(c, DIMER_RADIUS)
+    sim.step(16)
+    sim.kernel_timing(2)
+    sim.step(steps)
+    ms, n = sim.kernel_timing(1)
+    sim.close()
+    return ms / max(n, 1)
 
 
 def traffic_per_launch():
