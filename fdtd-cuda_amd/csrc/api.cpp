@@ -1042,9 +1042,19 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
         unrecord();
     } else {
         for (int seg = 0; seg < nseg; seg++) {
-            if ((rc = ckpt_copy(s, cks[seg], true))) return rc;
             const int len = std::min(ckpt, nsteps - seg * ckpt);
-            if ((rc = fdtd_step(s, len))) return rc;
+            if (seg + 1 < nseg) {
+                if ((rc = ckpt_copy(s, cks[seg], true))) return rc;
+                if ((rc = fdtd_step(s, len))) return rc;
+            } else {
+                // the last segment records its box trajectory on the way, so the
+                // reverse sweep starts on it without a checkpoint or a recompute
+                for (int st = 0; st < len; st++) {
+                    record(st);
+                    if ((rc = fdtd_step(s, 1))) return rc;
+                }
+                unrecord();
+            }
         }
     }
     s->probe = nullptr;
@@ -1071,16 +1081,19 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
     for (int seg = nseg - 1; seg >= 0; seg--) {
         int seg_t0 = t0, len = nsteps;
         if (!store) {
-            Checkpoint &ck = cks[seg];
-            s->t = ck.t; s->p = ck.p; s->pp = ck.pp; s->le = ck.le;
-            if ((rc = ckpt_copy(s, ck, false))) return rc;
-            seg_t0 = ck.t;
+            seg_t0 = t0 + seg * ckpt;
             len = std::min(ckpt, nsteps - seg * ckpt);
-            for (int st = 0; st < len; st++) {
-                record(st);
-                if ((rc = fdtd_step(s, 1))) return rc;
+            if (seg + 1 < nseg) {
+                // restore the segment's checkpoint and recompute its box trajectory
+                Checkpoint &ck = cks[seg];
+                s->t = ck.t; s->p = ck.p; s->pp = ck.pp; s->le = ck.le;
+                if ((rc = ckpt_copy(s, ck, false))) return rc;
+                for (int st = 0; st < len; st++) {
+                    record(st);
+                    if ((rc = fdtd_step(s, 1))) return rc;
+                }
+                unrecord();
             }
-            unrecord();
         }
         for (int st = len - 1; st >= 0; st--) {
             const int t = seg_t0 + st;

@@ -137,6 +137,56 @@ __device__ inline float tfsf_e(const TfsfDev &t, int c, int i, int j, int k) {
     return tfsf_corr(t, true, c, i, j, k);
 }
 
+// tfsf_term, tfsf_h and tfsf_e with the component and face axis fixed at
+// compile time (the same expressions and summation order, so the same
+// values): the index picks fold away, which the fix-up kernel, evaluating
+// twelve of these per cell, needs; the fused kernel keeps the runtime form
+// for code size.
+template <bool EF, int c, int a>
+__device__ __forceinline__ float tfsf_term_ct(const TfsfDev &t, int i, int j, int k) {
+    constexpr int b = 3 - a - c;
+    const int x[3] = {i, j, k};
+    const int xa = x[a], xb = x[b], xc = x[c];
+    const int t0a = t.t0[a], t1a = t.t1[a], t0b = t.t0[b], t1b = t.t1[b], t0c = t.t0[c], t1c = t.t1[c];
+    const int near_plane = EF ? t0a : t0a - 1;
+    const bool nr = xa == near_plane, fr = xa == t1a;
+    if (!nr && !fr) return 0.f;
+    const bool c_ok = EF ? (xc >= t0c && xc < t1c) : (xc >= t0c && xc <= t1c);
+    const bool b_ok = EF ? (xb >= t0b && xb <= t1b) : (xb >= t0b && xb < t1b);
+    if (!c_ok || !b_ok) return 0.f;
+    const float ddc = (float)(xc - t0c) + (EF ? 0.5f : 0.f);
+    const float ddb = (float)(xb - t0b) + (EF ? 0.f : 0.5f);
+    const float span = (float)(t1a - t0a);
+    const float dda = EF ? (nr ? -0.5f : span + 0.5f) : (nr ? 0.f : span);
+    const float d0 = a == 0 ? dda : (b == 0 ? ddb : ddc);
+    const float d1 = a == 1 ? dda : (b == 1 ? ddb : ddc);
+    const float d2 = a == 2 ? dda : (b == 2 ? ddb : ddc);
+    constexpr bool cyc = a == (c + 1) % 3;
+    float sg = (cyc != EF) ? 1.f : -1.f;
+    if (fr) sg = -sg;
+    const float pol = EF ? t.hf[b] : t.ef[b];
+    const float v = inc_interp(t, EF, d0, d1, d2) * pol;
+    return sg * (EF ? t.cE : t.cH) * v * t.inv_d[a];
+}
+template <bool EF, int c>
+__device__ __forceinline__ float tfsf_corr_ct(const TfsfDev &t, int i, int j, int k) {
+    float r = 0.f;
+    if constexpr (c != 0) r += tfsf_term_ct<EF, c, 0>(t, i, j, k);
+    if constexpr (c != 1) r += tfsf_term_ct<EF, c, 1>(t, i, j, k);
+    if constexpr (c != 2) r += tfsf_term_ct<EF, c, 2>(t, i, j, k);
+    return r;
+}
+template <int c>
+__device__ __forceinline__ float tfsf_h_ct(const TfsfDev &t, int i, int j, int k) {
+    if (i < t.t0[0] - 1 || i > t.t1[0] || j < t.t0[1] - 1 || j > t.t1[1] || k < t.t0[2] - 1 || k > t.t1[2]) return 0.f;
+    return tfsf_corr_ct<false, c>(t, i, j, k);
+}
+template <int c>
+__device__ __forceinline__ float tfsf_e_ct(const TfsfDev &t, int i, int j, int k) {
+    if (i < t.t0[0] || i > t.t1[0] || j < t.t0[1] || j > t.t1[1] || k < t.t0[2] || k > t.t1[2]) return 0.f;
+    return tfsf_corr_ct<true, c>(t, i, j, k);
+}
+
 // ----------------------------------------------------------------- per-cell H update (main.cu:762-1044)
 
 // Neighbour values of E^n needed by the H update at (i,j,k).
@@ -1197,18 +1247,18 @@ __device__ __forceinline__ void tfsf_fix_body(const StepDev &s, int bx, int by, 
     if (a >= 1 && on(0, i)) return;
     if (a >= 2 && on(1, j)) return;
     const long p = gidx(s.g, i, j, k);
-    const float dhx = tfsf_h(t, 0, i, j, k), dhy = tfsf_h(t, 1, i, j, k), dhz = tfsf_h(t, 2, i, j, k);
+    const float dhx = tfsf_h_ct<0>(t, i, j, k), dhy = tfsf_h_ct<1>(t, i, j, k), dhz = tfsf_h_ct<2>(t, i, j, k);
     if (dhx != 0.f) s.Hout[0][p] += dhx;
     if (dhy != 0.f) s.Hout[1][p] += dhy;
     if (dhz != 0.f) s.Hout[2][p] += dhz;
     const float Cee = s.c.Cee, Ceh = s.c.Ceh;
     const float ikx = s.cp[0].ik_e[i], iky = s.cp[1].ik_e[j], ikz = s.cp[2].ik_e[k];
-    const float dz_jm = tfsf_h(t, 2, i, j - 1, k), dx_jm = tfsf_h(t, 0, i, j - 1, k);
-    const float dy_km = tfsf_h(t, 1, i, j, k - 1), dx_km = tfsf_h(t, 0, i, j, k - 1);
-    const float dz_im = tfsf_h(t, 2, i - 1, j, k), dy_im = tfsf_h(t, 1, i - 1, j, k);
-    const float fx = Cee * tfsf_e(t, 0, i, j, k) + Ceh * ((dhz - dz_jm) * iky - (dhy - dy_km) * ikz);
-    const float fy = Cee * tfsf_e(t, 1, i, j, k) + Ceh * ((dhx - dx_km) * ikz - (dhz - dz_im) * ikx);
-    const float fz = Cee * tfsf_e(t, 2, i, j, k) + Ceh * ((dhy - dy_im) * ikx - (dhx - dx_jm) * iky);
+    const float dz_jm = tfsf_h_ct<2>(t, i, j - 1, k), dx_jm = tfsf_h_ct<0>(t, i, j - 1, k);
+    const float dy_km = tfsf_h_ct<1>(t, i, j, k - 1), dx_km = tfsf_h_ct<0>(t, i, j, k - 1);
+    const float dz_im = tfsf_h_ct<2>(t, i - 1, j, k), dy_im = tfsf_h_ct<1>(t, i - 1, j, k);
+    const float fx = Cee * tfsf_e_ct<0>(t, i, j, k) + Ceh * ((dhz - dz_jm) * iky - (dhy - dy_km) * ikz);
+    const float fy = Cee * tfsf_e_ct<1>(t, i, j, k) + Ceh * ((dhx - dx_km) * ikz - (dhz - dz_im) * ikx);
+    const float fz = Cee * tfsf_e_ct<2>(t, i, j, k) + Ceh * ((dhy - dy_im) * ikx - (dhx - dx_jm) * iky);
     if (fx != 0.f) s.Eout[0][p] += fx;
     if (fy != 0.f) s.Eout[1][p] += fy;
     if (fz != 0.f) s.Eout[2][p] += fz;

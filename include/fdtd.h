@@ -158,7 +158,8 @@ int  fdtd_set_design(fdtd_sim *sim, const int lo[3], const int n[3], const float
 /* Discrete adjoint (no reference counterpart).  From the current state (t = 0)
  * runs nsteps forward with a full-state checkpoint every ckpt steps, then the
  * transposed step in reverse time, recomputing each segment from its
- * checkpoint.  ckpt <= 0 keeps E on the dispersive box for every step instead
+ * checkpoint (the last segment keeps its box trajectory from the forward pass
+ * and is not recomputed).  ckpt <= 0 keeps E on the dispersive box for every step instead
  * (3 floats per box cell per step of device memory) and needs no recompute;
  * both give the same bits.  Objective J = sum_f sum_c |sum_t E_c^{t+1}(r0) e^{i 2 pi f_j (t+1) dt}|^2
  * over the nf frequencies freq (Hz) at cell r0.  Writes J and dJ/drho for the
