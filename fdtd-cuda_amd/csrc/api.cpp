@@ -376,7 +376,10 @@ int plan_rows(fdtd_sim *s) {
     if (rc) return rc;
     if (const char *v = std::getenv("FDTD_CONCURRENT")) s->concurrent = std::atoi(v) != 0;
     if (const char *v = std::getenv("FDTD_EARLY")) s->early = std::atoi(v) != 0;
-    if (const char *v = std::getenv("FDTD_TILES")) s->tiles = std::atoi(v) == 3 ? 3 : 1;
+    if (const char *v = std::getenv("FDTD_TILES")) s->tiles = std::min(3, std::max(1, std::atoi(v)));
+    if (const char *v = std::getenv("FDTD_STILES")) s->stiles = std::atoi(v) == 2 ? 2 : 1;
+    if (!s->tiles) s->tiles = rows_tiles(s->g.NZ);
+    if (!s->stiles) s->stiles = std::min(2, rows_tiles(s->g.NZ));
     if (s->concurrent && s->has_clean) {
         if (hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -409,6 +412,12 @@ RowsArgs make_rows(fdtd_sim *s) {
         for (int a = 0; a < 2; a++) r.ik0[e][a] = s->ik0[e][a];
     return r;
 }
+
+// Default row tiles per block of the row-blocked kernel: two tiles (whose
+// halo rows the block then reads once) where a z column takes at most 5
+// waves, so a block holds at most 10; with 8-wave columns the 16-wave block
+// of two tiles measured slower than two 8-wave blocks per CU.
+int rows_tiles(int NZ) { return rows_waves(NZ) <= 5 ? 2 : 1; }
 
 // Whether the TFSF fix may run right after the clean-box kernel, before the
 // side pieces finish: its cells (planes t0-1, t0, t1, t1+1 of each axis over
@@ -515,6 +524,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     if (s->nxbox) {
         RowsArgs rx = r;
         rx.ichunk = s->schunk;
+        rx.tiles = s->stiles * rows_waves(s->g.NZ) <= 16 ? s->stiles : 1;
         rx.npiece = s->nxbox;
         std::memcpy(rx.box, s->xbox, sizeof(s->xbox));
         HIPCHK(s, launch_step_rows(rx, 2, 1, side));
@@ -522,6 +532,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     if (s->nybox) {
         RowsArgs ry = r;
         ry.ichunk = s->schunk;
+        ry.tiles = s->stiles * rows_waves(s->g.NZ) <= 16 ? s->stiles : 1;
         ry.npiece = s->nybox;
         std::memcpy(ry.box, s->ybox, sizeof(s->ybox));
         HIPCHK(s, launch_step_rows(ry, 2, 2, side));
@@ -530,7 +541,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
-    r.tiles = s->tiles == 3 && 3 * rows_waves(s->g.NZ) <= 16 ? 3 : 1;
+    r.tiles = s->tiles * rows_waves(s->g.NZ) <= 16 ? s->tiles : 1;
     if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
     HIPCHK(s, launch_step_rows(r, s->rows, 0, s->stream));
     if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));

@@ -200,7 +200,7 @@ struct RowsArgs {
     int box[4][4];
     int nseg;
     int start[5];
-    int tiles;                          // row tiles per block: 1, or 3 for the clean variant when 3 W <= 16
+    int tiles;                          // row tiles per block: 1, or 2-3 for the clean variant when tiles * W <= 16
 };
 // Words of the table image and its layout (see kernels.hip).
 constexpr int kRowsAuxWords = 1548 + 48;

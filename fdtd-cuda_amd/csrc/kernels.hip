@@ -2422,6 +2422,22 @@ hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st) {
 
 template <int W, bool XG, bool YG, bool ADJ>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
+    if (ra.tiles == 2) {
+        if constexpr (W * 2 <= 16 && !ADJ) {
+            switch (R) {
+                case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, false, 2>), grid, dim3(64, 2 * W), 0, st, ra); break;
+                case 3:
+                    if constexpr (!XG && !YG) {
+                        hipLaunchKernelGGL((k_step_rows<W, 3, false, false, false, 2>), grid, dim3(64, 2 * W), 0, st, ra);
+                        break;
+                    }
+                    return hipErrorInvalidValue;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
     if (ra.tiles == 3) {
         if constexpr (W * 3 <= 16 && !XG && !YG && !ADJ) {
             switch (R) {
@@ -2466,7 +2482,8 @@ long rows_prepare(RowsArgs &ra, int R, int variant) {
     if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 3) return -1;
     const int W = rows_waves(ra.NZ);
     if (ra.tiles < 1) ra.tiles = 1;
-    if (ra.tiles != 1 && (ra.tiles != 3 || 3 * W > 16 || variant != 0)) return -1;
+    if (ra.tiles != 1 && (ra.tiles < 2 || ra.tiles > 3 || ra.tiles * W > 16 || variant == 3 || (variant != 0 && ra.tiles != 2)))
+        return -1;
     const int T = ra.tiles;
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
     long total = 0;

@@ -95,7 +95,10 @@ struct fdtd_sim {
     bool early = true;              // TFSF fix and dispersive box ahead of the join when their cells allow (FDTD_EARLY=0 to keep them after)
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
     int rows = 2, cchunk = 16;      // own rows per thread and x chunk of the row-blocked kernel (clean box)
-    int tiles = 1;                  // row tiles per block of the clean-box launch (FDTD_TILES: 1 or 3; 3 needs 3 W <= 16)
+    // row tiles per block of the clean-box launch (FDTD_TILES: 1-3) and of the
+    // side-slab launches (FDTD_STILES: 1 or 2); 0: two when the z column takes
+    // at most 5 waves, else one (see rows_tiles)
+    int tiles = 0, stiles = 0;
     int schunk = 16;                // x chunk of the side slabs
     int adj_schunk = 8;             // x chunk of the side slabs in the transposed step (FDTD_ADJ_SCHUNK)
     int adj_stj = 8;                // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ)
@@ -152,6 +155,7 @@ DispDev make_disp(fdtd_sim *s, int parity);
 RowsArgs make_rows(fdtd_sim *s);
 // Cell-set tests of the two-stream schedule (the clean box and the TFSF,
 // dispersive and monitor cells; see api.cpp).
+int rows_tiles(int NZ);
 bool tfsf_in_clean(const fdtd_sim *s);
 bool disp_in_clean(const fdtd_sim *s);
 // DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.

@@ -255,20 +255,23 @@ def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypa
             assert np.array_equal(dft[f], ref_dft[f]), (name, f)
 
 
-@pytest.mark.parametrize("ny,rows", [(24, 2), (26, 2), (28, 2), (26, 3)])
-def test_row_tiles_per_block_bit_identical(ny, rows, monkeypatch):
-    """Three row tiles per block on the clean box (FDTD_TILES=3) against one:
-    the same cells with the same arithmetic, so the same bits, also when the
-    row-tile count is not a multiple of three (idle tiles store nothing)."""
+@pytest.mark.parametrize("ny,rows,tiles,stiles", [(24, 2, 3, 1), (26, 2, 3, 1), (28, 2, 3, 1), (26, 3, 3, 1),
+                                                (26, 2, 2, 2), (28, 3, 2, 2), (27, 2, 1, 2)])
+def test_row_tiles_per_block_bit_identical(ny, rows, tiles, stiles, monkeypatch):
+    """Two or three row tiles per block on the clean box (FDTD_TILES) and two
+    on the side slabs (FDTD_STILES) against one: the same cells with the same
+    arithmetic, so the same bits, also when the row-tile count is not a
+    multiple of the tiles (idle tiles store nothing)."""
     p = small_params(O, steps=300)
     p.NCELLY = ny
     monkeypatch.setenv("FDTD_ROWS", str(rows))
     out = {}
-    for tiles in ("1", "3"):
-        monkeypatch.setenv("FDTD_TILES", tiles)
+    for t, st in ((1, 1), (tiles, stiles)):
+        monkeypatch.setenv("FDTD_TILES", str(t))
+        monkeypatch.setenv("FDTD_STILES", str(st))
         s = fdtd_amd.Simulation(p)
         s.step(300)
-        out[tiles] = [s.get_field(c) for c in range(6)]
-    assert np.abs(out["3"][0]).max() > 0
+        out[t, st] = [s.get_field(c) for c in range(6)]
+    assert np.abs(out[tiles, stiles][0]).max() > 0
     for c in range(6):
-        assert np.array_equal(out["1"][c], out["3"][c]), c
+        assert np.array_equal(out[1, 1][c], out[tiles, stiles][c]), c
