@@ -145,6 +145,7 @@ FusedItem fused_corners(fdtd_sim *s) {
 RowsArgs rows_item(fdtd_sim *s, int variant, int R, int ic) {
     RowsArgs r = make_rows(s);
     r.xcd = 0;
+    r.tiles = rows_batch_tiles(rows_waves(s->g.NZ));
     if (variant == 0) {
         r.ichunk = ic;
         r.npiece = 1;

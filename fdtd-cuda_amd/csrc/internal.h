@@ -276,6 +276,9 @@ hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st);
 // rows_prepare (same W = rows_waves(NZ), R and variant 0-2 for all of them).
 hipError_t launch_rows_batch(const RowsArgs *items, const unsigned *map, int nblocks, int W, int R, int variant,
                              hipStream_t st);
+// Row tiles per block of launch_rows_batch for W waves per z column (items'
+// tiles must be set to it before rows_prepare).
+int rows_batch_tiles(int W);
 // TFSF fix-up over many simulations (item.s as for launch_tfsf_fix).
 hipError_t launch_tfsf_fix_batch(const FusedItem *items, const unsigned *map, int nblocks, hipStream_t st);
 // Blocks of launch_tfsf_fix_batch for one simulation (four 64-lane rows each).

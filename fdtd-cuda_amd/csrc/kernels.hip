@@ -2273,10 +2273,10 @@ __global__ __launch_bounds__(64 * TJ) void k_fused_batch(const FusedItem *items,
     fused_body<TJ>(it.s, it.ichunk, it.nkt, it.pc, (int)(m & ((1u << kBatchLocalBits) - 1)));
 }
 
-template <int W, int R, bool XG, bool YG>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_rows_batch(const RowsArgs *items, const unsigned *map) {
+template <int W, int R, bool XG, bool YG, int T>
+__global__ __launch_bounds__(64 * W * T) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_rows_batch(const RowsArgs *items, const unsigned *map) {
     const unsigned m = batch_map(map);
-    rows_body<W, R, XG, YG, false>(batch_item(items, m >> kBatchLocalBits), (int)(m & ((1u << kBatchLocalBits) - 1)));
+    rows_body<W, R, XG, YG, false, T>(batch_item(items, m >> kBatchLocalBits), (int)(m & ((1u << kBatchLocalBits) - 1)));
 }
 
 // Local block L of a simulation's TFSF fix-up grid ((span + 63) / 64, span, 12) in x-fastest order.
@@ -2359,14 +2359,17 @@ hipError_t launch_fused_batch(const FusedItem *items, const unsigned *map, int n
 template <int W>
 static hipError_t launch_rows_batch_w(const RowsArgs *items, const unsigned *map, int nblocks, int R, int variant,
                                       hipStream_t st) {
-    const dim3 grid(nblocks), block(64, W);
-    if (R == 3 && variant == 0) hipLaunchKernelGGL((k_rows_batch<W, 3, false, false>), grid, block, 0, st, items, map);
-    else if (R == 2 && variant == 0) hipLaunchKernelGGL((k_rows_batch<W, 2, false, false>), grid, block, 0, st, items, map);
-    else if (R == 2 && variant == 1) hipLaunchKernelGGL((k_rows_batch<W, 2, true, false>), grid, block, 0, st, items, map);
-    else if (R == 2 && variant == 2) hipLaunchKernelGGL((k_rows_batch<W, 2, false, true>), grid, block, 0, st, items, map);
+    constexpr int T = W <= 5 ? 2 : 1;   // rows_batch_tiles(W)
+    const dim3 grid(nblocks), block(64, W * T);
+    if (R == 3 && variant == 0) hipLaunchKernelGGL((k_rows_batch<W, 3, false, false, T>), grid, block, 0, st, items, map);
+    else if (R == 2 && variant == 0) hipLaunchKernelGGL((k_rows_batch<W, 2, false, false, T>), grid, block, 0, st, items, map);
+    else if (R == 2 && variant == 1) hipLaunchKernelGGL((k_rows_batch<W, 2, true, false, T>), grid, block, 0, st, items, map);
+    else if (R == 2 && variant == 2) hipLaunchKernelGGL((k_rows_batch<W, 2, false, true, T>), grid, block, 0, st, items, map);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
+
+int rows_batch_tiles(int W) { return W <= 5 ? 2 : 1; }
 
 hipError_t launch_rows_batch(const RowsArgs *items, const unsigned *map, int nblocks, int W, int R, int variant,
                              hipStream_t st) {
