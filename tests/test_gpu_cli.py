@@ -1,0 +1,64 @@
+"""fdtd_run end to end on the device: an Input.txt in the reference format
+in, SourceFile.txt and ScatAbs.<t>.txt out (main.cu:2629-2644, 3042;
+Spectral.cu:1153-1171).  The spectra files are checked against the library
+run in this process on the same parameters (columns 5-16 are fdtd_flux's
+face fluxes, 17-19 the source spectrum; %e keeps 7 digits), and the
+normalised totals against the reference formulas; the source file against
+the pulse from fdtd_get_info."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import fdtd_amd
+from cli_input import write_input
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "fdtd-cuda_amd", "fdtd_run")
+C0 = 2.99792458e8
+
+
+def test_fdtd_run_matches_library(tmp_path):
+    vals = write_input(tmp_path / "Input.txt", Tend=200, NCELLX=40, NCELLY=40, NCELLZ=40, NcpmlX=6, NcpmlY=6,
+                       NcpmlZ=6, nano_sphere_radius=8e-9, NUM_freq=8, BandWidth=2.5e16, lam_0=5e-7)
+    print("running fdtd_run on 53^3, Tend 200", flush=True)
+    r = subprocess.run([EXE, "Input.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    dumps = sorted(int(f.split(".")[1]) for f in os.listdir(tmp_path) if f.startswith("ScatAbs."))
+    assert dumps == list(range(0, 201, 20))
+
+    p = fdtd_amd.default_params()
+    for k, v in vals.items():
+        if hasattr(p, k):
+            setattr(p, k, v)
+    s = fdtd_amd.Simulation(p)
+    info = s.get_info()
+    s.step(201)
+    fl = s.flux()
+    raw = np.asarray(fl["raw"]).reshape(12, -1)
+    freq, einc = s.spectrum_setup()
+    out = np.loadtxt(tmp_path / "ScatAbs.200.txt")
+    assert out.shape == (8, 19)
+    np.testing.assert_allclose(out[:, 4:16], raw.T, rtol=2e-6, atol=1e-30)
+    sca = -raw[0] + raw[1] - raw[2] + raw[3] - raw[4] + raw[5]
+    ab = -raw[6] + raw[7] - raw[8] + raw[9] - raw[10] + raw[11]
+    area = np.pi * vals["nano_sphere_radius"] ** 2
+    np.testing.assert_allclose(out[:, 1], sca / area, rtol=2e-6)
+    np.testing.assert_allclose(out[:, 2], -ab / area, rtol=2e-6)
+    np.testing.assert_allclose(out[:, 3], sca / area - ab / area, rtol=2e-6, atol=1e-9 * np.abs(sca / area).max())
+    np.testing.assert_allclose(out[:, 0], C0 / freq, rtol=2e-6)
+    tol = 1e-6 * np.abs(einc).max()
+    np.testing.assert_allclose(out[:, 16], np.abs(einc), rtol=2e-6, atol=tol)
+    np.testing.assert_allclose(out[:, 17], einc.real, rtol=2e-6, atol=tol)
+    np.testing.assert_allclose(out[:, 18], einc.imag, rtol=2e-6, atol=tol)
+
+    src = np.loadtxt(tmp_path / "SourceFile.txt")
+    assert src.shape == (200, 2)
+    t = np.arange(200)
+    b = ((t - info.delay) / info.width).astype(np.float32)
+    pulse = vals["MAX_AMP"] * np.exp(-(b * b).astype(np.float64) / 2) * np.sin(2 * np.pi * info.f_0 * t * info.dt)
+    np.testing.assert_allclose(src[:, 0], t * info.dt, rtol=2e-6)
+    np.testing.assert_allclose(src[:, 1], pulse, rtol=2e-6, atol=1e-6 * np.abs(pulse).max())
