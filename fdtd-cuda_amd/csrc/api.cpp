@@ -1333,6 +1333,25 @@ int fdtd_get_field(fdtd_sim *s, int comp, float *out) {
     return FDTD_OK;
 }
 
+int fdtd_get_slice(fdtd_sim *s, int comp, int axis, int index, float *out) {
+    if (!s || !out || comp < 0 || comp > 5 || axis < 0 || axis > 2) return FDTD_ERR_INVALID;
+    const Geom &g = s->g;
+    const int n[3] = {g.NX, g.NY, g.NZ};
+    if (index < 0 || index >= n[axis]) return set_err(s, FDTD_ERR_INVALID, "fdtd_get_slice: index outside the grid");
+    const float *f = comp < 3 ? s->E[s->p][comp] : s->H[s->p][comp - 3];
+    const size_t fl = sizeof(float);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (axis == 0)   // rows j of NZ values, pitch sz
+        HIPCHK(s, hipMemcpy2D(out, g.NZ * fl, f + (size_t)index * g.sxy, g.sz * fl, g.NZ * fl, g.NY, hipMemcpyDeviceToHost));
+    else if (axis == 1)   // rows i of NZ values, pitch sxy
+        HIPCHK(s, hipMemcpy2D(out, g.NZ * fl, f + (size_t)index * g.sz, g.sxy * fl, g.NZ * fl, g.NX, hipMemcpyDeviceToHost));
+    else   // per i, NY single values with pitch sz
+        for (int i = 0; i < g.NX; i++)
+            HIPCHK(s, hipMemcpy2D(out + (size_t)i * g.NY, fl, f + (size_t)i * g.sxy + index, g.sz * fl, fl, g.NY,
+                                  hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
 int fdtd_set_field(fdtd_sim *s, int comp, const float *in) {
     if (!s || !in || comp < 0 || comp > 5) return FDTD_ERR_INVALID;
     float *f = comp < 3 ? s->E[s->p][comp] : s->H[s->p][comp - 3];

@@ -5,7 +5,10 @@
 // (main.cu:2724) and after step t, whenever t % (Tend/10) == 0, writes
 // ScatAbs.<t>.txt with the 19 columns of CalculateAbsScatt
 // (Spectral.cu:1153-1171).  This driver does the same with fdtd_step and
-// fdtd_flux.
+// fdtd_flux.  With Snap_in != 0 it also writes, every t_skip steps, the
+// |E| plane at i = NX/2 in the format of SNAPSHOT_2D (snapshot.cu:25-60,
+// videofiles/YZ_sim.<n>.txt, every y_skip-th row and z_skip-th column); the
+// shipped reference leaves that call commented out (main.cu:2948-2951).
 //
 //   fdtd_run [Input.txt] [--dir DIR] [--parse-only]
 //
@@ -13,6 +16,10 @@
 // tokens the solver does not use) and exits without touching the device.
 // Input.txt: one header token, then 43 "value name" pairs in the order
 // below; the names are not checked, as in the reference.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -87,6 +94,7 @@ std::vector<Token> tokens(fdtd_params &p) {
 // Parses Input.txt into p (defaults first); unused tokens go to `unused`.
 bool read_input(const char *path, fdtd_params &p, std::vector<std::pair<std::string, std::string>> &unused,
                 std::string &err) {
+    unused.clear();
     std::ifstream in(path);
     if (!in) { err = std::string("cannot open ") + path; return false; }
     fdtd_default_params(&p);
@@ -149,6 +157,34 @@ bool write_scatabs(const std::string &dir, int t, const fdtd_params &p, const st
     return std::fclose(f) == 0;
 }
 
+// YZ |E| snapshot (SNAPSHOT_2D, snapshot.cu:62-76): rows j, columns k.
+bool write_snapshot(fdtd_sim *sim, const fdtd_grid_info &gi, const std::string &dir, int n, int jskip, int kskip) {
+    const size_t plane = (size_t)gi.NY * gi.NZ;
+    std::vector<float> e[3];
+    for (int c = 0; c < 3; c++) {
+        e[c].resize(plane);
+        if (fdtd_get_slice(sim, c, 0, gi.NX / 2, e[c].data()) != FDTD_OK) return false;
+    }
+    const std::string path = dir + "/videofiles/YZ_sim." + std::to_string(n) + ".txt";
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    for (int j = 0; j < gi.NY; j += jskip) {
+        for (int k = 0; k < gi.NZ; k += kskip) {
+            const size_t q = (size_t)j * gi.NZ + k;
+            const double v = (double)e[0][q] * e[0][q] + (double)e[1][q] * e[1][q] + (double)e[2][q] * e[2][q];
+            std::fprintf(f, "%e ", std::sqrt(v));
+        }
+        std::fprintf(f, "\n");
+    }
+    return std::fclose(f) == 0;
+}
+
+int unused_int(const std::vector<std::pair<std::string, std::string>> &u, const char *name, int dflt) {
+    for (const auto &x : u)
+        if (x.first == name) return std::atoi(x.second.c_str());
+    return dflt;
+}
+
 int fail(const std::string &m) {
     std::fprintf(stderr, "fdtd_run: %s\n", m.c_str());
     return 1;
@@ -198,15 +234,27 @@ int main(int argc, char **argv) {
         std::fclose(f);
     }
     // the loop t = 0..Tend; after step t the spectra when t % (Tend/10) == 0
+    // and, with Snap_in != 0, a snapshot when t % t_skip == 0
     const int every = p.Tend / 10 > 0 ? p.Tend / 10 : 1;
-    int done = 0;
-    for (int t = 0; t <= p.Tend; t += every) {
+    const int snap_in = unused_int(unused, "Snap_in", 0), t_skip = unused_int(unused, "t_skip", 0);
+    const int jskip = std::max(1, unused_int(unused, "y_skip", 1)), kskip = std::max(1, unused_int(unused, "z_skip", 1));
+    const bool snap = snap_in != 0 && t_skip > 0;
+    if (snap) {
+        const std::string vd = dir + "/videofiles";
+        if (::mkdir(vd.c_str(), 0755) != 0 && errno != EEXIST) return bail("videofiles");
+    }
+    int done = 0, nsnap = 0;
+    for (int t = 0; t <= p.Tend; t++) {
+        const bool spectra = t % every == 0 && p.StaticField == 0, shot = snap && t % t_skip == 0;
+        if (!spectra && !shot) continue;
         if (fdtd_step(sim, t + 1 - done) != FDTD_OK) return bail("fdtd_step");
         done = t + 1;
-        if (p.StaticField != 0) continue;
-        if (fdtd_flux(sim, nullptr, nullptr, nullptr, raw.data()) != FDTD_OK) return bail("fdtd_flux");
-        if (!write_scatabs(dir, t, p, freq, einc, raw)) return bail("ScatAbs");
-        std::printf("t = %d: ScatAbs.%d.txt\n", t, t);
+        if (spectra) {
+            if (fdtd_flux(sim, nullptr, nullptr, nullptr, raw.data()) != FDTD_OK) return bail("fdtd_flux");
+            if (!write_scatabs(dir, t, p, freq, einc, raw)) return bail("ScatAbs");
+            std::printf("t = %d: ScatAbs.%d.txt\n", t, t);
+        }
+        if (shot && !write_snapshot(sim, gi, dir, nsnap++, jskip, kskip)) return bail("snapshot");
     }
     if (done < p.Tend + 1 && fdtd_step(sim, p.Tend + 1 - done) != FDTD_OK) return bail("fdtd_step");
     if (fdtd_sync(sim) != FDTD_OK) return bail("fdtd_sync");

@@ -18,7 +18,7 @@ EXPORTS = [
     "fdtd_default_params", "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_get_info",
     "fdtd_set_materials", "fdtd_add_sphere", "fdtd_get_materials", "fdtd_update_h", "fdtd_tfsf", "fdtd_update_e",
     "fdtd_dft_accumulate", "fdtd_step", "fdtd_sync", "fdtd_get_field", "fdtd_set_field",
-    "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
+    "fdtd_get_slice", "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
     "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod", "fdtd_set_point_monitor", "fdtd_point_spectrum",
     "fdtd_batch_create", "fdtd_batch_run", "fdtd_batch_tile", "fdtd_batch_last_error", "fdtd_batch_destroy",
 ]
@@ -58,6 +58,7 @@ def load_library(path=LIB_PATH):
         "fdtd_set_field": (ip, [vp, ip, fp]),
         "fdtd_get_line": (ip, [vp, dp, dp]),
         "fdtd_get_spectrum_setup": (ip, [vp, dp, dp]),
+        "fdtd_get_slice": (ip, [vp, ip, ip, ip, C.POINTER(C.c_float)]),
         "fdtd_get_dft": (ip, [vp, ip, ip, dp]),
         "fdtd_flux": (ip, [vp, dp, dp, dp, dp]),
         "fdtd_kernel_timing": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
@@ -225,6 +226,14 @@ class Simulation:
         e, h = np.empty(L), np.empty(L)
         self._check(self.lib.fdtd_get_line(self.h, _ptr(e, C.c_double), _ptr(h, C.c_double)))
         return e, h
+
+    def get_slice(self, comp, axis, index):
+        """One plane of a field component: axis 0 -> [j, k], 1 -> [i, k], 2 -> [i, j]."""
+        i = self.info
+        shape = {0: (i.NY, i.NZ), 1: (i.NX, i.NZ), 2: (i.NX, i.NY)}[axis]
+        out = np.empty(shape, dtype=np.float32)
+        self._check(self.lib.fdtd_get_slice(self.h, int(comp), int(axis), int(index), _ptr(out, C.c_float)))
+        return out
 
     def spectrum_setup(self):
         F = self.info.num_freq

@@ -195,6 +195,12 @@ int  fdtd_sync(fdtd_sim *sim);
 
 /* Field copies, NX*NY*NZ floats in the reference layout. */
 int  fdtd_get_field(fdtd_sim *sim, int comp, float *out);
+/* One plane of a field component, copied straight from the device (no
+ * reference counterpart; SNAPSHOT_2D, snapshot.cu:25-60, reads planes of a
+ * full host copy): axis 0 gives the plane i = index as out[j*NZ + k], axis 1
+ * the plane j = index as out[i*NZ + k], axis 2 the plane k = index as
+ * out[i*NY + j] (augmented grid). */
+int  fdtd_get_slice(fdtd_sim *sim, int comp, int axis, int index, float *out);
 int  fdtd_set_field(fdtd_sim *sim, int comp, const float *in);
 
 /* 1D incident line (e_inc, h_inc), inc_length values each. */

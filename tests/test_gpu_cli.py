@@ -1,6 +1,6 @@
 """fdtd_run end to end on the device: an Input.txt in the reference format
-in, SourceFile.txt and ScatAbs.<t>.txt out (main.cu:2629-2644, 3042;
-Spectral.cu:1153-1171).  The spectra files are checked against the library
+in, SourceFile.txt, ScatAbs.<t>.txt and the YZ |E| snapshots out
+(main.cu:2629-2644, 3042; Spectral.cu:1153-1171; snapshot.cu:62-76).  The spectra files are checked against the library
 run in this process on the same parameters (columns 5-16 are fdtd_flux's
 face fluxes, 17-19 the source spectrum; %e keeps 7 digits), and the
 normalised totals against the reference formulas; the source file against
@@ -23,7 +23,8 @@ C0 = 2.99792458e8
 
 def test_fdtd_run_matches_library(tmp_path):
     vals = write_input(tmp_path / "Input.txt", Tend=200, NCELLX=40, NCELLY=40, NCELLZ=40, NcpmlX=6, NcpmlY=6,
-                       NcpmlZ=6, nano_sphere_radius=8e-9, NUM_freq=8, BandWidth=2.5e16, lam_0=5e-7)
+                       NcpmlZ=6, nano_sphere_radius=8e-9, NUM_freq=8, BandWidth=2.5e16, lam_0=5e-7,
+                       Snap_in=2, t_skip=50, y_skip=2, z_skip=3)
     print("running fdtd_run on 53^3, Tend 200", flush=True)
     r = subprocess.run([EXE, "Input.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -55,6 +56,15 @@ def test_fdtd_run_matches_library(tmp_path):
     np.testing.assert_allclose(out[:, 17], einc.real, rtol=2e-6, atol=tol)
     np.testing.assert_allclose(out[:, 18], einc.imag, rtol=2e-6, atol=tol)
 
+    # snapshots after steps 0, 50, ..., 200: |E| on the plane i = NX/2, rows j::2, columns k::3
+    snaps = sorted(os.listdir(tmp_path / "videofiles"))
+    assert snaps == ["YZ_sim.%d.txt" % n for n in range(5)]
+    shot = np.loadtxt(tmp_path / "videofiles" / "YZ_sim.4.txt")
+    e = [s.get_field(c)[info.NX // 2].astype(np.float64) for c in range(3)]
+    mag = np.sqrt(e[0] ** 2 + e[1] ** 2 + e[2] ** 2)[::2, ::3]
+    assert shot.shape == mag.shape and mag.max() > 0
+    np.testing.assert_allclose(shot, mag, rtol=2e-6, atol=1e-6 * mag.max())
+
     src = np.loadtxt(tmp_path / "SourceFile.txt")
     assert src.shape == (200, 2)
     t = np.arange(200)
@@ -62,3 +72,21 @@ def test_fdtd_run_matches_library(tmp_path):
     pulse = vals["MAX_AMP"] * np.exp(-(b * b).astype(np.float64) / 2) * np.sin(2 * np.pi * info.f_0 * t * info.dt)
     np.testing.assert_allclose(src[:, 0], t * info.dt, rtol=2e-6)
     np.testing.assert_allclose(src[:, 1], pulse, rtol=2e-6, atol=1e-6 * np.abs(pulse).max())
+
+
+def test_get_slice_matches_field():
+    """fdtd_get_slice on each axis against the full-field copy."""
+    import oracle as O
+    from conftest import small_params
+    p = small_params(O, steps=120)
+    p.NCELLY = 26
+    s = fdtd_amd.Simulation(p)
+    s.step(120)
+    for comp in (0, 4):
+        f = s.get_field(comp)
+        assert np.abs(f).max() > 0
+        for axis, idx in ((0, 17), (1, 3), (2, 20)):
+            want = np.take(f, idx, axis=axis)
+            assert np.array_equal(s.get_slice(comp, axis, idx), want), (comp, axis)
+    with pytest.raises(RuntimeError):
+        s.get_slice(0, 2, s.info.NZ)
