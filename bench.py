@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=128,
                     help="steps of the 32-rod sweep window, batched and one by one (0: skip)")
-    ap.add_argument("--adjoint-steps", type=int, default=400,
+    ap.add_argument("--adjoint-steps", type=int, default=10000,
                     help="steps of the adjoint/forward timing on the inverse-design grid (0: skip)")
     ap.add_argument("--hydro-steps", type=int, default=200,
                     help="steps of the hydrodynamic 384^3 timing window (0: skip)")
@@ -340,11 +340,13 @@ def adjoint_bench(steps, ckpt=None, seed=7):
     interior at dx = 2 nm (281^3 cells), 12-cell CPML, a 64x64x32 density
     block at the centre with rho ~ U[0, 1) (seeded), objective |E|^2 at 633,
     785 and 850 nm at the cell 5 nm above the block's top face, x-polarised
-    plane wave along z.  Forward = fdtd_step over `steps`; adjoint =
-    fdtd_adjoint over the same steps (checkpointed forward, recompute,
-    transposed sweep)."""
+    plane wave along z (600 nm centre, BandWidth 6.79e14 Hz; the pulse
+    reaches the block after about 4000 steps, so J and dJ/drho are zero for
+    shorter windows).  Forward = fdtd_step over `steps`; adjoint =
+    fdtd_adjoint over the same steps (checkpoints every `ckpt` steps with
+    recompute, then the stored trajectory)."""
     import fdtd_amd
-    ckpt = ckpt or max(1, steps // 4)
+    ckpt = ckpt or (500 if steps >= 2000 else max(1, steps // 4))
     p = fdtd_amd.default_params()
     p.NCELLX = p.NCELLY = p.NCELLZ = 256
     p.NcpmlX = p.NcpmlY = p.NcpmlZ = 12
@@ -356,6 +358,8 @@ def adjoint_bench(steps, ckpt=None, seed=7):
     p.inc_phi = 90.0
     p.TimeFactor = 3 ** 0.5
     p.tfsf_inset = 10
+    p.lam_0 = 600e-9
+    p.BandWidth = 6.79e14
     T = 256 + 25
     n = (64, 64, 32)
     lo = tuple(T // 2 - k // 2 for k in n)
@@ -391,6 +395,7 @@ def adjoint_bench(steps, ckpt=None, seed=7):
         "ratio_checkpoint_recompute": round(t_rec / t_fwd, 3),
         "J": res["recompute"][1],
         "grad_norm": res["recompute"][2],
+        "stored_equals_recompute": res["stored"][1] == res["recompute"][1] and res["stored"][2] == res["recompute"][2],
     }
 
 
