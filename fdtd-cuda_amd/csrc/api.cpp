@@ -379,6 +379,9 @@ int plan_rows(fdtd_sim *s) {
     if (const char *v = std::getenv("FDTD_TILES")) s->tiles = std::min(3, std::max(1, std::atoi(v)));
     if (const char *v = std::getenv("FDTD_STILES")) s->stiles = std::atoi(v) == 2 ? 2 : 1;
     if (!s->tiles) s->tiles = rows_tiles(s->g.NZ);
+    // two-tile blocks take 32 planes per chunk (fewer prologue planes; measured
+    // best on the 281^3 grid), one-tile blocks 16 (measured best at 409^3)
+    if (!s->cchunk) s->cchunk = s->tiles >= 2 && s->tiles * rows_waves(s->g.NZ) <= 16 ? 32 : 16;
     if (!s->stiles) s->stiles = std::min(2, rows_tiles(s->g.NZ));
     if (s->concurrent && s->has_clean) {
         if (hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
