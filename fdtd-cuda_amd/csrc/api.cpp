@@ -379,9 +379,18 @@ int plan_rows(fdtd_sim *s) {
     if (const char *v = std::getenv("FDTD_TILES")) s->tiles = std::min(3, std::max(1, std::atoi(v)));
     if (const char *v = std::getenv("FDTD_STILES")) s->stiles = std::atoi(v) == 2 ? 2 : 1;
     if (!s->tiles) s->tiles = rows_tiles(s->g.NZ);
-    // two-tile blocks take 32 planes per chunk (fewer prologue planes; measured
-    // best on the 281^3 grid), one-tile blocks 16 (measured best at 409^3)
-    if (!s->cchunk) s->cchunk = s->tiles >= 2 && s->tiles * rows_waves(s->g.NZ) <= 16 ? 32 : 16;
+    // x chunk of the clean box: 16, or 32 for two-tile blocks when that still
+    // leaves at least 512 blocks (two per CU; fewer prologue planes, measured
+    // best on the 281^3 grid; on small grids 32 leaves the device half idle)
+    if (!s->cchunk) {
+        s->cchunk = 16;
+        const int W = rows_waves(s->g.NZ), T = s->tiles * W <= 16 ? s->tiles : 1;
+        if (T >= 2) {
+            const long njb = ((s->cbox[3] - s->cbox[2] + s->rows - 1) / s->rows + T - 1) / T;
+            const long nseg = (s->g.NZ + 1 + 64 * W - 3) / (64 * W - 2);
+            if (njb * nseg * ((s->cbox[1] - s->cbox[0] + 31) / 32) >= 512) s->cchunk = 32;
+        }
+    }
     if (!s->stiles) s->stiles = std::min(2, rows_tiles(s->g.NZ));
     if (s->concurrent && s->has_clean) {
         if (hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
