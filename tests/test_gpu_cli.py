@@ -90,3 +90,14 @@ def test_get_slice_matches_field():
             assert np.array_equal(s.get_slice(comp, axis, idx), want), (comp, axis)
     with pytest.raises(RuntimeError):
         s.get_slice(0, 2, s.info.NZ)
+
+
+def test_fdtd_run_static_field_writes_no_spectra(tmp_path):
+    """StaticField = 1 (ramped static source, no DFT): the reference writes
+    no ScatAbs files (main.cu:3042); SourceFile.txt is still written."""
+    write_input(tmp_path / "Input.txt", Tend=60, NCELLX=40, NCELLY=40, NCELLZ=40, NcpmlX=6, NcpmlY=6, NcpmlZ=6,
+                nano_sphere_radius=8e-9, NUM_freq=4, StaticField=1)
+    r = subprocess.run([EXE, "Input.txt", "--dir", "."], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert not [f for f in os.listdir(tmp_path) if f.startswith("ScatAbs.")]
+    assert np.loadtxt(tmp_path / "SourceFile.txt").shape == (60, 2)
