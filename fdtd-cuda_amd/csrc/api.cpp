@@ -1346,10 +1346,38 @@ int fdtd_get_field(fdtd_sim *s, int comp, float *out) {
 }
 
 int fdtd_get_slice(fdtd_sim *s, int comp, int axis, int index, float *out) {
-    if (!s || !out || comp < 0 || comp > 5 || axis < 0 || axis > 2) return FDTD_ERR_INVALID;
+    if (!s || !out || comp < 0 || comp > 6 || axis < 0 || axis > 2) return FDTD_ERR_INVALID;
     const Geom &g = s->g;
     const int n[3] = {g.NX, g.NY, g.NZ};
     if (index < 0 || index >= n[axis]) return set_err(s, FDTD_ERR_INVALID, "fdtd_get_slice: index outside the grid");
+    if (comp == 6) {
+        // hydrodynamic density perturbation N_EQ (n~x + n~y + n~z) / 3 at time
+        // level n+1 (the level the last step left in the n~^n buffer), as the
+        // ND snapshots average the three component densities (snapshot.cu:80-133);
+        // zero outside the metal box
+        if (!s->S.hydro || s->nbox <= 0) return set_err(s, FDTD_ERR_INVALID, "fdtd_get_slice: no hydrodynamic model");
+        const size_t nb = (size_t)s->nbox;
+        std::vector<float> nd(3 * nb);
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        for (int c = 0; c < 3; c++) HIPCHK(s, hipMemcpy(nd.data() + c * nb, s->h_ndp[c], nb * sizeof(float), hipMemcpyDeviceToHost));
+        const int ua = axis == 0 ? 1 : 0, ub = axis == 2 ? 1 : 2;   // plane axes: out[u * n[ub] + v]
+        const double neq = s->S.ade.N_EQ;
+        for (int u = 0; u < n[ua]; u++)
+            for (int v = 0; v < n[ub]; v++) {
+                int x[3];
+                x[axis] = index; x[ua] = u; x[ub] = v;
+                float val = 0.f;
+                bool in = true;
+                for (int a = 0; a < 3; a++) in = in && x[a] >= s->box_lo[a] && x[a] < s->box_lo[a] + s->box_n[a];
+                if (in) {
+                    const size_t q = ((size_t)(x[0] - s->box_lo[0]) * s->box_n[1] + (x[1] - s->box_lo[1])) * s->box_n[2] +
+                                     (x[2] - s->box_lo[2]);
+                    val = (float)(neq * ((double)nd[q] + nd[nb + q] + nd[2 * nb + q]) / 3.0);
+                }
+                out[(size_t)u * n[ub] + v] = val;
+            }
+        return FDTD_OK;
+    }
     const float *f = comp < 3 ? s->E[s->p][comp] : s->H[s->p][comp - 3];
     const size_t fl = sizeof(float);
     HIPCHK(s, hipStreamSynchronize(s->stream));

@@ -7,7 +7,8 @@
 // (Spectral.cu:1153-1171).  This driver does the same with fdtd_step and
 // fdtd_flux.  With Snap_in != 0 it also writes, every t_skip steps, the
 // |E| plane at i = NX/2 in the format of SNAPSHOT_2D (snapshot.cu:25-60,
-// videofiles/YZ_sim.<n>.txt, every y_skip-th row and z_skip-th column); the
+// videofiles/YZ_sim.<n>.txt, every y_skip-th row and z_skip-th column, and
+// with the hydrodynamic model the density plane YZ_sim.<n>_N.txt); the
 // shipped reference leaves that call commented out (main.cu:2948-2951).
 //
 //   fdtd_run [Input.txt] [--dir DIR] [--parse-only]
@@ -174,6 +175,18 @@ bool write_snapshot(fdtd_sim *sim, const fdtd_grid_info &gi, const std::string &
             const double v = (double)e[0][q] * e[0][q] + (double)e[1][q] * e[1][q] + (double)e[2][q] * e[2][q];
             std::fprintf(f, "%e ", std::sqrt(v));
         }
+        std::fprintf(f, "\n");
+    }
+    if (std::fclose(f) != 0) return false;
+    if (!gi.hydro) return true;
+    // with the hydrodynamic model also the density plane (snapshot.cu:120-133,
+    // videofiles/YZ_sim.<n>_N.txt: (NDx + NDy + NDz) / 3)
+    std::vector<float> nd(plane);
+    if (fdtd_get_slice(sim, 6, 0, gi.NX / 2, nd.data()) != FDTD_OK) return false;
+    f = std::fopen((dir + "/videofiles/YZ_sim." + std::to_string(n) + "_N.txt").c_str(), "w");
+    if (!f) return false;
+    for (int j = 0; j < gi.NY; j += jskip) {
+        for (int k = 0; k < gi.NZ; k += kskip) std::fprintf(f, "%e ", (double)nd[(size_t)j * gi.NZ + k]);
         std::fprintf(f, "\n");
     }
     return std::fclose(f) == 0;

@@ -228,7 +228,8 @@ class Simulation:
         return e, h
 
     def get_slice(self, comp, axis, index):
-        """One plane of a field component: axis 0 -> [j, k], 1 -> [i, k], 2 -> [i, j]."""
+        """One plane of a field component (0-5) or, with the hydrodynamic model,
+        of the density perturbation (6): axis 0 -> [j, k], 1 -> [i, k], 2 -> [i, j]."""
         i = self.info
         shape = {0: (i.NY, i.NZ), 1: (i.NX, i.NZ), 2: (i.NX, i.NY)}[axis]
         out = np.empty(shape, dtype=np.float32)

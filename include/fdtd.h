@@ -199,7 +199,10 @@ int  fdtd_get_field(fdtd_sim *sim, int comp, float *out);
  * reference counterpart; SNAPSHOT_2D, snapshot.cu:25-60, reads planes of a
  * full host copy): axis 0 gives the plane i = index as out[j*NZ + k], axis 1
  * the plane j = index as out[i*NZ + k], axis 2 the plane k = index as
- * out[i*NY + j] (augmented grid). */
+ * out[i*NY + j] (augmented grid).  comp 0-5 as fdtd_get_field; comp 6 (with
+ * the hydrodynamic model) the density perturbation ND = N_EQ (n~x + n~y +
+ * n~z) / 3 in m^-3, zero outside the metal box (the ND snapshots,
+ * snapshot.cu:80-133). */
 int  fdtd_get_slice(fdtd_sim *sim, int comp, int axis, int index, float *out);
 int  fdtd_set_field(fdtd_sim *sim, int comp, const float *in);
 

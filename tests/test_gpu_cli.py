@@ -101,3 +101,28 @@ def test_fdtd_run_static_field_writes_no_spectra(tmp_path):
     assert r.returncode == 0, r.stderr
     assert not [f for f in os.listdir(tmp_path) if f.startswith("ScatAbs.")]
     assert np.loadtxt(tmp_path / "SourceFile.txt").shape == (60, 2)
+
+
+def test_density_slice_of_the_hydrodynamic_model():
+    """fdtd_get_slice(6): N_EQ (n~x + n~y + n~z) / 3 on a plane; zero outside
+    the metal box, nonzero inside once the pulse has driven the sphere, and
+    the planes of two axes agree on the line where they cross.  (No FP64
+    reference holds the density field; the values are pinned only by these
+    properties and by the hydrodynamic field parity tests.)"""
+    import oracle as O
+    from conftest import small_params
+    p = small_params(O, steps=300)
+    p.NONLOCAL = 3
+    p.hydro_beta = 1.27e6
+    s = fdtd_amd.Simulation(p)
+    s.step(300)
+    info = s.get_info()
+    c = info.NX // 2
+    a = s.get_slice(6, 0, c)          # [j, k] at i = c
+    b = s.get_slice(6, 1, c)          # [i, k] at j = c
+    lo, hi = list(info.box_lo), list(info.box_hi)
+    assert np.abs(a).max() > 0
+    inside = np.zeros_like(a, dtype=bool)
+    inside[lo[1]:hi[1], lo[2]:hi[2]] = True
+    assert np.all(a[~inside] == 0)
+    assert np.array_equal(a[c], b[c])
