@@ -189,6 +189,7 @@ def main():
             "grid": [info.NX, info.NY, info.NZ],
             "dispersive_cells": list(info.n_dispersive[:]),
             "tile": {"tj": int(os.environ.get("FDTD_TJ", 8)), "ichunk": int(os.environ.get("FDTD_ICHUNK", 32)),
+                     "corner_tj": int(os.environ.get("FDTD_CTJ", 4)),
                      "rows": int(os.environ.get("FDTD_ROWS", 2)), "cchunk": int(os.environ.get("FDTD_CCHUNK", 32)),
                      "schunk": int(os.environ.get("FDTD_SCHUNK", 16)),
                      "row_tiles": int(os.environ.get("FDTD_TILES", 2)), "side_row_tiles": int(os.environ.get("FDTD_STILES", 2)),

@@ -137,7 +137,7 @@ FusedItem fused_corners(fdtd_sim *s) {
     FusedItem it{};
     it.s = fused_step(s, false);
     it.ichunk = 2;
-    if (s->ncorner) fused_pieces(s->g, s->tj, 2, s->cbox4, s->ncorner, &it.pc, &it.nkt);
+    if (s->ncorner) fused_pieces(s->g, s->ctj, 2, s->cbox4, s->ncorner, &it.pc, &it.nkt);
     return it;
 }
 
@@ -352,7 +352,7 @@ int upload_items(fdtd_batch *b) {
                 if (rows_blocks(ri[2][i])) add_blocks(group(ls, Launch::Rows, W, 2, 2), i, rows_blocks(ri[2][i]));
             }
             for (int i = 0; i < n; i++) {
-                const int tj = b->rowpath[i] ? b->sims[i]->tj : b->ta[i];
+                const int tj = b->rowpath[i] ? b->sims[i]->ctj : b->ta[i];
                 const long nb = b->rowpath[i] && !b->sims[i]->ncorner ? 0 : fi[i].pc.start[fi[i].pc.n];
                 if (nb) add_blocks(group(ls, Launch::Fused, tj, 0, 0), i, nb);
             }

@@ -80,6 +80,7 @@ struct fdtd_sim {
     size_t hsz[6] = {}, esz[6] = {};
     int t = 0;
     int tj = 8, ichunk = 32;
+    int ctj = 4;                    // tile rows of the general kernel on the corner columns (FDTD_CTJ: 4, 8, 16)
     // fused step: row-blocked kernel over the whole grid + TFSF fix-up, or the general kernel alone
     bool use_rows = false;
     bool has_clean = false;         // clean box non-empty: clean variant there, general variant on the rest
