@@ -89,7 +89,17 @@ LineDev make_line(fdtd_sim *s) {
     l.w0dt = 2 * fdtd::kPI * S.f0 * S.dt;
     l.static_field = S.P.StaticField == 1;
     l.t = s->t;
+    l.probe = s->E[s->p][0] + s->probe_cell;
+    l.status = s->status_d;
     return l;
+}
+
+int check_finite(fdtd_sim *s) {
+    if (s->status_h && *(volatile unsigned *)s->status_h)
+        return set_err(s, FDTD_ERR_NONFINITE,
+                       "non-finite field value (NaN or Inf) detected at the probe cell or on a monitor face by step " +
+                           std::to_string(s->t));
+    return FDTD_OK;
 }
 
 DispDev make_disp(fdtd_sim *s, int parity) {
@@ -246,15 +256,27 @@ void build_faces(fdtd_sim *s) {
     s->mtot = f.mE_pad * 2;
 }
 
+hipEvent_t next_event(fdtd_sim *s, int which);
+
 int contract(fdtd_sim *s) {
     if (s->slot == 0) return FDTD_OK;
     const int F = s->S.P.NUM_freq;
+    const bool timed = s->timing && s->timing_what == 3;
+    if (timed) {
+        hipEvent_t a = next_event(s, 0);
+        if (!a) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
+        HIPCHK(s, hipEventRecord(a, s->stream));
+    }
     if (s->S.P.dft_exact) {
         HIPCHK(s, launch_twiddle(s->W, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
         HIPCHK(s, launch_dft_contract(s->X, s->W, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
     } else {
         HIPCHK(s, launch_twiddle_frag(s->Wf, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
         HIPCHK(s, launch_dft_mfma(s->X, s->Wf, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
+    }
+    if (timed) {
+        HIPCHK(s, hipEventRecord(next_event(s, 1), s->stream));
+        s->ev_used++;
     }
     s->slot = 0;
     return FDTD_OK;
@@ -589,14 +611,19 @@ const char *fdtd_last_error(const fdtd_sim *sim) { return sim ? sim->err.c_str()
 int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
     if (!p || !out) return set_err(nullptr, FDTD_ERR_INVALID, "null argument");
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, FDTD_ERR_DEVICE, "no HIP device");
     fdtd_sim *s = new (std::nothrow) fdtd_sim();
     if (!s) return set_err(nullptr, FDTD_ERR_ALLOC, "host allocation failed");
+    // parameters are checked before the device is touched, so invalid input
+    // reports the same error with or without a GPU
     std::string err;
-    if (!fdtd::build_setup(*p, s->S, err)) {
+    if (const int rc = fdtd::build_setup(*p, s->S, err)) {
         delete s;
-        return set_err(nullptr, FDTD_ERR_INVALID, err);
+        return set_err(nullptr, rc, err);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete s;
+        return set_err(nullptr, FDTD_ERR_DEVICE, "no HIP device");
     }
     const Setup &S = s->S;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -664,8 +691,17 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
         return fail(rc);
     // dispersive box
     if ((rc = build_dispersive(s))) return fail(rc);
+    // divergence watch word in pinned host memory, written by the kernels
+    if (hipHostMalloc((void **)&s->status_h, 256, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&s->status_d, s->status_h, 0) != hipSuccess) {
+        s->err = "pinned status word could not be allocated";
+        return fail(FDTD_ERR_ALLOC);
+    }
+    *(volatile unsigned *)s->status_h = 0u;
+    s->probe_cell = gidx(s->g, g.NX / 2, g.NY / 2, g.NZ / 2 - g.NZ / 4);
     // DFT buffers
     build_faces(s);
+    s->fd.status = s->status_d;
     s->K = S.P.dft_block;
     s->slot_t.assign(s->K, 0);
     const int F = S.P.NUM_freq;
@@ -694,6 +730,7 @@ int fdtd_destroy(fdtd_sim *s) {
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     if (s->ev_line) (void)hipEventDestroy(s->ev_line);
     for (void *p : s->allocs) (void)hipFree(p);
+    if (s->status_h) (void)hipHostFree(s->status_h);
     if (s->stream2) (void)hipStreamDestroy(s->stream2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -726,6 +763,19 @@ int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
     const bool two = s->use_rows && s->stream2 && s->early;
     const bool tf = two && tfsf_in_clean(s), dp = two && disp_in_clean(s);
     g->schedule = (tf ? 1 : 0) | (dp ? 2 : 0) | (tf && (s->nbox <= 0 || dp) && sample_in_clean(s) ? 4 : 0);
+    if (s->use_rows) {
+        const int W = rows_waves(s->g.NZ);
+        g->tile[0] = s->rows;
+        g->tile[1] = s->cchunk;
+        g->tile[2] = s->schunk;
+        g->tile[3] = s->tiles * W <= 16 ? s->tiles : 1;
+        g->tile[4] = s->stiles * W <= 16 ? s->stiles : 1;
+        g->tile[5] = s->ctj;
+        g->tile[6] = s->stream2 ? 1 : 0;
+    } else {
+        g->tile[0] = s->tj;
+        g->tile[1] = s->ichunk;
+    }
     return FDTD_OK;
 }
 
@@ -1231,6 +1281,7 @@ int fdtd_dft_accumulate(fdtd_sim *s) {
 
 int fdtd_step(fdtd_sim *s, int nsteps) {
     if (!s || nsteps < 0) return FDTD_ERR_INVALID;
+    if (int rc = check_finite(s)) return rc;
     for (int n = 0; n < nsteps; n++) {
         LineDev l = make_line(s);
         StepDev d = make_step(s, true);
@@ -1242,31 +1293,32 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = s->psiH[pp][a]; d.psiH_out.p[a] = s->psiH[1 - pp][a]; d.psiE.p[a] = s->psiE[a]; }
         const bool samp = s->sampling && sampling_step(s);
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        if (s->timing) {
+        const bool step_timing = s->timing && s->timing_what != 3;
+        if (step_timing) {
             ev0 = next_event(s, 0);
             ev1 = next_event(s, 1);
             if (!ev0 || !ev1) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
         }
-        const bool only_clean = s->timing_what == 2 && s->use_rows;
+        const bool only_clean = step_timing && s->timing_what == 2 && s->use_rows;
         DispDev dd{};
         if (s->nbox) dd = make_disp(s, 1 - q);
         bool disp_done = false, samp_done = false;
         if (s->use_rows) {
             d.tfsf_on = 0;
             if (samp) d.sample_row = s->X + (size_t)s->slot * s->mtot;
-            if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev0, s->stream));
+            if (step_timing && !only_clean) HIPCHK(s, hipEventRecord(ev0, s->stream));
             int rc2 = launch_rows_update(s, d, l, s->nbox ? &dd : nullptr, samp, &disp_done, &samp_done,
                                          only_clean ? ev0 : nullptr, only_clean ? ev1 : nullptr);
             if (rc2) return rc2;
-            if (s->timing && !only_clean) HIPCHK(s, hipEventRecord(ev1, s->stream));
+            if (step_timing && !only_clean) HIPCHK(s, hipEventRecord(ev1, s->stream));
         } else {
             HIPCHK(s, launch_line(l, s->stream));
             const int full[1][4] = {{0, s->g.NX, 0, s->g.NY}};
-            if (s->timing) HIPCHK(s, hipEventRecord(ev0, s->stream));
+            if (step_timing) HIPCHK(s, hipEventRecord(ev0, s->stream));
             HIPCHK(s, launch_fused_step(d, s->tj, s->ichunk, full, 1, s->stream));
-            if (s->timing) HIPCHK(s, hipEventRecord(ev1, s->stream));
+            if (step_timing) HIPCHK(s, hipEventRecord(ev1, s->stream));
         }
-        if (s->timing) s->ev_used++;
+        if (step_timing) s->ev_used++;
         if (s->nbox && !disp_done) HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
         if (s->probe) {
             const float *E1[3] = {s->E[1 - q][0], s->E[1 - q][1], s->E[1 - q][2]};
@@ -1318,6 +1370,7 @@ int fdtd_point_spectrum(fdtd_sim *s, int nf, const double *freq, double *e2, dou
     const int n = std::min(s->t - s->mon_t0, s->mon_cap);
     std::vector<float> e((size_t)3 * std::max(n, 1));
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (int rc = check_finite(s)) return rc;
     if (n > 0) HIPCHK(s, hipMemcpy(e.data(), s->mon, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost));
     const double dt = s->S.dt;
     for (int f = 0; f < nf; f++) {
@@ -1336,7 +1389,7 @@ int fdtd_point_spectrum(fdtd_sim *s, int nf, const double *freq, double *e2, dou
 int fdtd_sync(fdtd_sim *s) {
     if (!s) return FDTD_ERR_INVALID;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    return FDTD_OK;
+    return check_finite(s);
 }
 
 int fdtd_get_field(fdtd_sim *s, int comp, float *out) {
@@ -1431,6 +1484,7 @@ int fdtd_get_dft(fdtd_sim *s, int face, int comp, double *out) {
     int rc = contract(s);
     if (rc) return rc;
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    if ((rc = check_finite(s))) return rc;
     const FaceDev &f = s->fd;
     const long blk = (long)f.na[face] * f.nb[face];
     const long col = (comp < 2 ? 0 : f.mE_pad) + f.off[face] + (comp % 2) * blk;
@@ -1457,6 +1511,9 @@ int fdtd_flux(fdtd_sim *s, double *sca, double *abs_, double *ext, double *raw) 
     std::vector<double> Q(12 * F);
     HIPCHK(s, hipStreamSynchronize(s->stream));
     HIPCHK(s, hipMemcpy(Q.data(), s->flux_d, Q.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if ((rc = check_finite(s))) return rc;
+    for (double v : Q)
+        if (!std::isfinite(v)) return set_err(s, FDTD_ERR_NONFINITE, "non-finite face flux (the monitor spectra diverged)");
     // normalise by |E_incident|^2 and combine the faces (Spectral.cu:1162-1168)
     for (int m = 0; m < F; m++) {
         const double a = std::abs(S.Einc[m]);
@@ -1491,7 +1548,7 @@ int fdtd_kernel_timing(fdtd_sim *s, int enable, double *total_ms, long *launches
     if (total_ms) *total_ms = tot;
     if (launches) *launches = (long)s->ev_used;
     s->ev_used = 0;
-    if (enable < 0 || enable > 2) return set_err(s, FDTD_ERR_INVALID, "kernel_timing: enable must be 0, 1 or 2");
+    if (enable < 0 || enable > 3) return set_err(s, FDTD_ERR_INVALID, "kernel_timing: enable must be 0, 1, 2 or 3");
     s->timing = enable != 0;
     if (enable) s->timing_what = enable;
     return FDTD_OK;

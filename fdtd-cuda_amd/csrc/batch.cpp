@@ -577,6 +577,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
 int fdtd_batch_run(fdtd_batch *b, int nsteps) {
     if (!b || nsteps < 0) return FDTD_ERR_INVALID;
     if (int rc = check_sims(b)) return rc;
+    for (fdtd_sim *s : b->sims)
+        if (int rc = check_finite(s)) return berr(b, rc, s->err);
     if (nsteps == 0) return FDTD_OK;
     // every simulation's own queue drains first; pending samples are contracted so all rings start empty
     for (fdtd_sim *s : b->sims) {

@@ -58,8 +58,9 @@ struct Setup {
 };
 
 void default_params(fdtd_params *p);
-// Builds everything fdtd_create needs; returns false with a message on invalid input.
-bool build_setup(const fdtd_params &p, Setup &s, std::string &err);
+// Builds everything fdtd_create needs; returns FDTD_OK, or FDTD_ERR_INVALID /
+// FDTD_ERR_UNSUPPORTED with a message.
+int build_setup(const fdtd_params &p, Setup &s, std::string &err);
 // Staircased sphere masks (setup.cu:219-248).
 void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]);
 // Same rule around an arbitrary centre (cells).

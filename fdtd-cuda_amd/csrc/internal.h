@@ -68,6 +68,7 @@ struct FaceDev {
     long off[12];            // first column of component 0 of this face (E half)
     long mE;                 // columns in the E half (H half starts at mE_pad)
     long mE_pad;
+    unsigned *status;        // set to 1 when a sampled value is NaN or Inf (host-visible word)
 };
 
 // Uniform update coefficients (setup.cu:2015-2062 with uniform media).
@@ -160,6 +161,10 @@ struct LineDev {
     double amp, delay, width, w0dt; // pulse: amp*exp(-((t-delay)/width)^2/2)*sin(w0dt*t)
     int static_field;
     int t;
+    // divergence probe (main.cu:2728-2731): ex^n at one cell; status set to 1
+    // when it is NaN or Inf (either may be null)
+    const float *probe;
+    unsigned *status;
 };
 
 // ---------------------------------------------------------------------------

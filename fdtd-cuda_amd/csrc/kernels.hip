@@ -1286,8 +1286,11 @@ __device__ __forceinline__ void sample_body(const Geom &g, const FaceDev &f, con
     // E_a/E_b and H_a/H_b components per normal axis: x (y,z), y (z,x), z (x,y)
     const int ca = (axis + 1) % 3, cb = (axis + 2) % 3;
     const int c = comp ? cb : ca;
-    row[col] = E[c][p];
-    row[f.mE_pad + col] = H[c][p];
+    const float ev = E[c][p], hv = H[c][p];
+    row[col] = ev;
+    row[f.mE_pad + col] = hv;
+    // divergence watch: every writer stores the same word, so no atomic is needed
+    if (f.status && !(__builtin_isfinite(ev) && __builtin_isfinite(hv))) *f.status = 1u;
 }
 
 __global__ void k_dft_sample_split(StepDev s) {
@@ -1559,6 +1562,7 @@ __device__ __forceinline__ void line_body(const LineDev &l, int t) {
             pulse = l.amp * exp(-(double)b2 / 2.0) * sin(l.w0dt * (double)t);
         }
         l.e_new[l.src] += pulse;
+        if (l.probe && l.status && !__builtin_isfinite(*l.probe)) *l.status = 1u;
     }
 }
 

@@ -154,17 +154,30 @@ void sphere_masks(const Setup &s, double radius, std::vector<uint8_t> m[4]) {
     sphere_masks_at(s, c, radius, m);
 }
 
-bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {
+int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
+    const int bad = FDTD_ERR_INVALID;
     s.P = p;
-    if (!(p.dx > 0) || p.dx != p.dy || p.dx != p.dz) { err = "cubic cells required (dx == dy == dz > 0)"; return false; }
-    if (p.NCELLX < 4 || p.NCELLY < 4 || p.NCELLZ < 18) { err = "grid too small (NCELLZ must exceed 17 for the source cell)"; return false; }
-    if (p.NcpmlX < 1 || p.NcpmlY < 1 || p.NcpmlZ < 1) { err = "Ncpml must be >= 1"; return false; }
-    if (p.NUM_freq < 1 || p.NUM_freq > 512) { err = "NUM_freq must be in [1, 512]"; return false; }
-    if (p.first_medium > 1) { err = "first_medium must be <= 1"; return false; }
-    if (p.dft_stride < 1) { err = "dft_stride must be >= 1"; return false; }
-    if (p.dft_block < 1 || p.dft_block > 256) { err = "dft_block must be in [1, 256]"; return false; }
-    if (!(p.TimeFactor > 0)) { err = "TimeFactor must be > 0"; return false; }
-    if (!(p.BandWidth > 0)) { err = "BandWidth must be > 0"; return false; }
+    if (!(p.dx > 0) || p.dx != p.dy || p.dx != p.dz) { err = "cubic cells required (dx == dy == dz > 0)"; return bad; }
+    if (p.NCELLX < 4 || p.NCELLY < 4 || p.NCELLZ < 18) { err = "grid too small (NCELLZ must exceed 17 for the source cell)"; return bad; }
+    if (p.NcpmlX < 1 || p.NcpmlY < 1 || p.NcpmlZ < 1) { err = "Ncpml must be >= 1"; return bad; }
+    if (p.NUM_freq < 1 || p.NUM_freq > 512) { err = "NUM_freq must be in [1, 512]"; return bad; }
+    if (p.first_medium > 1) { err = "first_medium must be <= 1"; return bad; }
+    if (p.dft_stride < 1) { err = "dft_stride must be >= 1"; return bad; }
+    if (p.dft_block < 1 || p.dft_block > 256) { err = "dft_block must be in [1, 256]"; return bad; }
+    if (!(p.TimeFactor > 0)) { err = "TimeFactor must be > 0"; return bad; }
+    // dt = 0.5 dy / c / TimeFactor (setup.cu:359) must not exceed the 3D Courant
+    // limit dx / (c sqrt(3)) for cubic cells, i.e. TimeFactor >= sqrt(3)/2: a
+    // larger step diverges (the reference runs it and stops on a NaN probe,
+    // main.cu:2728-2731)
+    if (p.TimeFactor < 0.5 * std::sqrt(3.0)) {
+        err = "TimeFactor below sqrt(3)/2: dt exceeds the 3D Courant limit dx/(c*sqrt(3))";
+        return bad;
+    }
+    if (!(p.BandWidth > 0)) { err = "BandWidth must be > 0"; return bad; }
+    if (p.NONLOCAL != 0 && p.NONLOCAL != 3) {
+        err = "NONLOCAL must be 0 (local Drude-CP) or 3 (hydrodynamic); the reference's other values are not built";
+        return FDTD_ERR_UNSUPPORTED;
+    }
     const int Nint[3] = {p.NCELLX, p.NCELLY, p.NCELLZ};
     const int Nc[3] = {p.NcpmlX, p.NcpmlY, p.NcpmlZ};
     // SETUP_CONST (setup.cu:347-400): dt, grid augmentation by 2*Ncpml + 1
@@ -180,7 +193,7 @@ bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         s.sca_e[a] = s.N[a] - s.tf0[a] + p.sca_offset;
         if (s.sca_s[a] < 1 || s.sca_e[a] > s.N[a] - 1 || s.abs_e[a] - s.abs_s[a] < 2 || s.tf0[a] <= Nc[a]) {
             err = "TFSF or monitor boxes do not fit the grid";
-            return false;
+            return bad;
         }
     }
     const double d = p.dx;
@@ -206,9 +219,12 @@ bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         A[0] = 0.26698; ph[0] = -1.2371; Om[0] = 3.8711e15; Ga[0] = 4.4642e14;
         A[1] = 3.0834; ph[1] = -1.0968; Om[1] = 4.1684e15; Ga[1] = 2.3555e15;
         s.ade.N_EQ = 5.476415562682574e28;
+    } else if (p.material == 7) {
+        err = "material 7 (Lorentz-Drude) is not built";
+        return FDTD_ERR_UNSUPPORTED;
     } else {
         err = "material must be 2 (silver) or 3 (gold)";
-        return false;
+        return bad;
     }
     const double dt = s.dt;
     double s3 = 0, s4 = 0, s5 = 0;
@@ -256,7 +272,7 @@ bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {
     s.psi = p.polar_psi * kPI / 180.0;
     s.m0 = (int)std::floor(s.L / 2);
     s.src = s.m0 - p.source_offset;
-    if (s.src < 1) { err = "source cell outside the incident line"; return false; }
+    if (s.src < 1) { err = "source cell outside the incident line"; return bad; }
     s.width = std::ceil((6 / (p.BandWidth * 2 * kPI)) / dt);
     s.delay = 6 * s.width;
     s.f0 = kC0 / p.lam_0;
@@ -265,7 +281,7 @@ bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         const double st = std::sin(s.theta), ct = std::cos(s.theta);
         const double ux = std::fabs(st * std::cos(s.phi)), uy = std::fabs(st * std::sin(s.phi)), uz = std::fabs(ct);
         const double reach = ux * (s.tf1[0] - s.tf0[0] + 1) + uy * (s.tf1[1] - s.tf0[1] + 1) + uz * (s.tf1[2] - s.tf0[2] + 1);
-        if (s.m0 + reach + 2 >= s.L || s.m0 - reach - 2 < 0) { err = "incident line too short for the TFSF box"; return false; }
+        if (s.m0 + reach + 2 >= s.L || s.m0 - reach - 2 < 0) { err = "incident line too short for the TFSF box"; return bad; }
     }
     // source spectrum over t < Tend (main.cu:2629-2644)
     s.Einc.assign(F, 0.0);
@@ -277,7 +293,7 @@ bool build_setup(const fdtd_params &p, Setup &s, std::string &err) {
     const size_t Ncell = (size_t)s.N[0] * s.N[1] * s.N[2];
     for (int c = 0; c < 4; c++) s.mat[c].assign(Ncell, (uint8_t)p.first_medium);
     if (p.nano_sphere == 1) sphere_masks(s, p.nano_sphere_radius, s.mat);
-    return true;
+    return FDTD_OK;
 }
 
 }  // namespace fdtd

@@ -79,6 +79,10 @@ struct fdtd_sim {
     long fbase = 0;                 // offset of element (0,0,0) in it
     size_t hsz[6] = {}, esz[6] = {};
     int t = 0;
+    // divergence watch: a pinned host word the line probe and the DFT sample
+    // kernels set to 1 on a NaN or Inf (status_d: its device address)
+    unsigned *status_h = nullptr, *status_d = nullptr;
+    long probe_cell = 0;            // ex probe of the line kernel (main.cu:2728: NX/2, NY/2, NZ/2 - NZ/4)
     int tj = 8, ichunk = 32;
     int ctj = 4;                    // tile rows of the general kernel on the corner columns (FDTD_CTJ: 4, 8, 16)
     // fused step: row-blocked kernel over the whole grid + TFSF fix-up, or the general kernel alone
@@ -160,6 +164,8 @@ RowsArgs make_rows(fdtd_sim *s);
 int rows_tiles(int NZ);
 bool tfsf_in_clean(const fdtd_sim *s);
 bool disp_in_clean(const fdtd_sim *s);
+// FDTD_ERR_NONFINITE (with a message) when a kernel has flagged a NaN or Inf.
+int check_finite(fdtd_sim *s);
 // DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.
 bool sampling_step(const fdtd_sim *s);
 int after_sample(fdtd_sim *s);

@@ -21,6 +21,7 @@ EXPORTS = [
     "fdtd_get_slice", "fdtd_get_line", "fdtd_get_spectrum_setup", "fdtd_get_dft", "fdtd_flux", "fdtd_kernel_timing",
     "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod", "fdtd_set_point_monitor", "fdtd_point_spectrum",
     "fdtd_batch_create", "fdtd_batch_run", "fdtd_batch_tile", "fdtd_batch_last_error", "fdtd_batch_destroy",
+    "fdtd_checkpoint_size", "fdtd_checkpoint_save", "fdtd_checkpoint_load",
 ]
 
 BATCH_NO_GRAPH = 1
@@ -72,6 +73,9 @@ def load_library(path=LIB_PATH):
         "fdtd_batch_tile": (ip, [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "fdtd_batch_last_error": (C.c_char_p, [vp]),
         "fdtd_batch_destroy": (ip, [vp]),
+        "fdtd_checkpoint_size": (ip, [vp, C.POINTER(C.c_size_t)]),
+        "fdtd_checkpoint_save": (ip, [vp, vp, C.c_size_t]),
+        "fdtd_checkpoint_load": (ip, [vp, vp, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -253,6 +257,20 @@ class Simulation:
         sca, ab, ext, raw = np.empty(F), np.empty(F), np.empty(F), np.empty(12 * F)
         self._check(self.lib.fdtd_flux(self.h, *[_ptr(a, C.c_double) for a in (sca, ab, ext, raw)]))
         return {"sca": sca, "abs": ab, "ext": ext, "raw": raw.reshape(12, F)}
+
+    def checkpoint(self):
+        """The whole simulation state as a bytes-like numpy array (fdtd_checkpoint_save)."""
+        n = C.c_size_t()
+        self._check(self.lib.fdtd_checkpoint_size(self.h, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._check(self.lib.fdtd_checkpoint_save(self.h, buf.ctypes.data_as(C.c_void_p), n.value))
+        return buf
+
+    def restore(self, buf):
+        """Loads an image written by checkpoint() (fdtd_checkpoint_load)."""
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._check(self.lib.fdtd_checkpoint_load(self.h, b.ctypes.data_as(C.c_void_p), b.size))
+        self.info = self.get_info()
 
     def kernel_timing(self, enable):
         ms, n = C.c_double(), C.c_long()

@@ -73,6 +73,7 @@ class GridInfo(C.Structure):
         ("fused_path", C.c_int),
         ("clean", C.c_int * 4),
         ("schedule", C.c_int),
+        ("tile", C.c_int * 7),
     ]
 
 

@@ -29,6 +29,7 @@
 #ifndef FDTD_H_
 #define FDTD_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -110,6 +111,10 @@ typedef struct fdtd_grid_info {
     int    schedule;           /* fdtd_step on two streams: bit 0 the TFSF fix-up, bit 1 the
                                   dispersive box, bit 2 the DFT sample follow the clean box without
                                   waiting for the side pieces (no reference counterpart)          */
+    int    tile[7];            /* tiles fdtd_step uses (no reference counterpart): row-blocked kernel
+                                  rows per thread, clean-box x chunk, side-slab x chunk, row tiles
+                                  per block on the clean box and on the side slabs, tile rows of the
+                                  corner columns, 1 when the side pieces run on a second stream    */
 } fdtd_grid_info;
 
 typedef struct fdtd_sim fdtd_sim;
@@ -190,7 +195,14 @@ int  fdtd_step(fdtd_sim *sim, int nsteps);
 int  fdtd_set_point_monitor(fdtd_sim *sim, const int r[3], int capacity);
 int  fdtd_point_spectrum(fdtd_sim *sim, int nf, const double *freq, double *e2, double *amp);
 
-/* Blocks until queued work finished; reports device errors. */
+/* Blocks until queued work finished; reports device errors, and
+ * FDTD_ERR_NONFINITE once a NaN or Inf has been seen.  The check follows the
+ * reference's per-step NaN probe (ex at NX/2, NY/2, NZ/2 - NZ/4 of the
+ * augmented grid, main.cu:2728-2731), read by the 1D-line kernel every step,
+ * and every value the DFT monitors sample; the flag is a host-visible word,
+ * so fdtd_step and fdtd_batch_run also return FDTD_ERR_NONFINITE (without
+ * stepping) once it is set, and fdtd_flux, fdtd_get_dft and
+ * fdtd_point_spectrum report it. */
 int  fdtd_sync(fdtd_sim *sim);
 
 /* Field copies, NX*NY*NZ floats in the reference layout. */
@@ -205,6 +217,21 @@ int  fdtd_get_field(fdtd_sim *sim, int comp, float *out);
  * snapshot.cu:80-133). */
 int  fdtd_get_slice(fdtd_sim *sim, int comp, int axis, int index, float *out);
 int  fdtd_set_field(fdtd_sim *sim, int comp, const float *in);
+
+/* Checkpoint / resume (no reference counterpart: the reference keeps its
+ * state in process globals for one run of main(), global_var.h:32-815).
+ * fdtd_checkpoint_size gives the bytes of an image of the whole simulation
+ * state at the current step: masks and design densities, E and H, the CPML
+ * psi slabs, the dispersive (and hydrodynamic) state, the 1D line, the DFT
+ * sample ring and accumulators, and the point-monitor samples.
+ * fdtd_checkpoint_save writes it into buf (at least that many bytes);
+ * fdtd_checkpoint_load restores it into a handle created with the same
+ * parameters (other materials are replaced by the saved ones), after which
+ * stepping gives the same bits as the saving handle would have.  Loading
+ * clears a NaN/Inf flag. */
+int  fdtd_checkpoint_size(fdtd_sim *sim, size_t *bytes);
+int  fdtd_checkpoint_save(fdtd_sim *sim, void *buf, size_t bytes);
+int  fdtd_checkpoint_load(fdtd_sim *sim, const void *buf, size_t bytes);
 
 /* 1D incident line (e_inc, h_inc), inc_length values each. */
 int  fdtd_get_line(fdtd_sim *sim, double *e_inc, double *h_inc);
@@ -229,7 +256,8 @@ int  fdtd_flux(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *ra
  * reset (HIP events on the solver stream; no reference counterpart).  Returns
  * the time and count recorded so far, then resets and sets the mode: 0 off,
  * 1 all field-update kernels of a step, 2 the clean-box launch of the
- * row-blocked kernel alone (the dominant one; same as 1 when there is none). */
+ * row-blocked kernel alone (the dominant one; same as 1 when there is none),
+ * 3 the K-sample DFT contractions (twiddles and contraction kernel). */
 int  fdtd_kernel_timing(fdtd_sim *sim, int enable, double *total_ms, long *launches);
 
 /*

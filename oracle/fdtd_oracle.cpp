@@ -1288,10 +1288,52 @@ int orc_set_materials(void *h, const uint8_t *mx, const uint8_t *my, const uint8
     return FDTD_OK;
 }
 
+// One more staircased sphere of the configured material at an arbitrary
+// centre (cells): the per-component rule of MATERIAL_MATRIX (setup.cu:221-244),
+// |x - centre| < (int)(radius/dx) at each component's Yee position.
+int orc_add_sphere(void *h, const double *centre, double radius) {
+    Oracle &o = *(Oracle *)h;
+    const int r = (int)(radius / o.dx);
+    const double cx = centre[0], cy = centre[1], cz = centre[2];
+    auto in = [&](double x, double y, double z) {
+        return pow((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz), 0.5) < r;
+    };
+#pragma omp parallel for
+    for (int i = 0; i < o.NX; i++)
+        for (int j = 0; j < o.NY; j++)
+            for (int k = 0; k < o.NZ; k++) {
+                const size_t id = o.I3(i, j, k);
+                if (in(i, j, k)) o.mat[id] = o.P.material;
+                if (in(i + 0.5, j, k)) o.matX[id] = o.P.material;
+                if (in(i, j + 0.5, k)) o.matY[id] = o.P.material;
+                if (in(i, j, k + 0.5)) o.matZ[id] = o.P.material;
+            }
+    return FDTD_OK;
+}
+
 int orc_get_materials(void *h, int comp, uint8_t *out) {
     Oracle &o = *(Oracle *)h;
     const std::vector<int> &m = comp == 0 ? o.mat : comp == 1 ? o.matX : comp == 2 ? o.matY : o.matZ;
     for (size_t i = 0; i < m.size(); i++) out[i] = (uint8_t)m[i];
+    return FDTD_OK;
+}
+
+// Dispersive state of one E component over the grid (checks of the ADE and
+// hydrodynamic updates): which 0 = hydrodynamic density ND at the newest
+// level (0 without the model), 1 = summed critical-point polarisation
+// P_cp1 + P_cp2 at the newest level, 2 = the Drude pole (polarisation, or the
+// electron velocity with the hydrodynamic model) at the newest level.
+int orc_get_state(void *h, int which, int comp, double *out) {
+    Oracle &o = *(Oracle *)h;
+    if (comp < 0 || comp > 2 || which < 0 || which > 2) return fail("orc_get_state: bad argument");
+    const size_t N = o.ex.size();
+    for (size_t i = 0; i < N; i++) {
+        double v = 0;
+        if (which == 0) v = o.hydro ? o.ND_prev[comp][i] : 0.0;
+        else if (which == 1) v = o.Pcp_n[comp][2 * i] + o.Pcp_n[comp][2 * i + 1];
+        else v = o.Pd_n[comp][i];
+        out[i] = v;
+    }
     return FDTD_OK;
 }
 

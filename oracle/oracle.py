@@ -36,6 +36,8 @@ def load():
         "orc_info": (ip, [vp, C.POINTER(GridInfo)]),
         "orc_set_materials": (ip, [vp, u8p, u8p, u8p]),
         "orc_get_materials": (ip, [vp, ip, u8p]),
+        "orc_add_sphere": (ip, [vp, dp, C.c_double]),
+        "orc_get_state": (ip, [vp, ip, ip, dp]),
         "orc_update_h": (ip, [vp]), "orc_tfsf": (ip, [vp]), "orc_update_e": (ip, [vp]),
         "orc_end_step": (ip, [vp]), "orc_step": (ip, [vp, ip]),
         "orc_get_field": (ip, [vp, ip, dp]), "orc_set_field": (ip, [vp, ip, dp]),
@@ -108,6 +110,11 @@ class Oracle:
         arrs = [np.ascontiguousarray(m, dtype=np.uint8).reshape(-1) for m in (mx, my, mz)]
         self.lib.orc_set_materials(self.h, *[_ptr(a, C.c_uint8) for a in arrs])
 
+    def add_sphere(self, centre, radius):
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        self.lib.orc_add_sphere(self.h, _ptr(c), float(radius))
+        self.info = self.get_info()
+
     def get_materials(self, comp):
         out = np.empty(self.shape, dtype=np.uint8)
         self.lib.orc_get_materials(self.h, comp, _ptr(out, C.c_uint8))
@@ -136,6 +143,14 @@ class Oracle:
     def set_field(self, comp, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
         self.lib.orc_set_field(self.h, comp, _ptr(a))
+
+    def get_state(self, which, comp):
+        """Dispersive state of E component comp: which 0 = hydrodynamic density ND,
+        1 = summed CP polarisation, 2 = Drude pole (velocity with the hydrodynamic model)."""
+        out = np.empty(self.shape)
+        if self.lib.orc_get_state(self.h, int(which), int(comp), _ptr(out)) != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return out
 
     def get_line(self):
         L = self.info.inc_length

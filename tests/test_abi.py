@@ -64,3 +64,40 @@ def test_create_without_device_fails_cleanly():
     rc = lib.fdtd_create(C.byref(p), C.byref(h))
     assert rc == fdtd_amd.abi.FDTD_ERR_DEVICE
     assert b"device" in lib.fdtd_last_error(None)
+
+
+def _create_rc(**over):
+    lib = fdtd_amd.load_library()
+    p = fdtd_amd.default_params()
+    for k, v in over.items():
+        setattr(p, k, v)
+    h = C.c_void_p()
+    rc = lib.fdtd_create(C.byref(p), C.byref(h))
+    if rc == 0:
+        lib.fdtd_destroy(h)
+    return rc, lib.fdtd_last_error(None).decode()
+
+
+@pytest.mark.parametrize("over,code,word", [
+    ({"TimeFactor": 0.8}, fdtd_amd.abi.FDTD_ERR_INVALID, "Courant"),
+    ({"NONLOCAL": 1}, fdtd_amd.abi.FDTD_ERR_UNSUPPORTED, "NONLOCAL"),
+    ({"NONLOCAL": 2}, fdtd_amd.abi.FDTD_ERR_UNSUPPORTED, "NONLOCAL"),
+    ({"material": 9}, fdtd_amd.abi.FDTD_ERR_INVALID, "material"),
+    ({"dx": 1e-9}, fdtd_amd.abi.FDTD_ERR_INVALID, "cubic"),
+])
+def test_invalid_and_unsupported_parameters_are_rejected_before_the_device(over, code, word):
+    """Parameters are checked before any device call, so these codes hold with
+    or without a GPU: a time step beyond the 3D Courant limit (TimeFactor <
+    sqrt(3)/2) and the reference's inputs this build does not run."""
+    rc, msg = _create_rc(**over)
+    assert rc == code, (rc, msg)
+    assert word in msg
+
+
+def test_time_factor_at_the_courant_limit_is_accepted_by_validation():
+    import torch
+    rc, msg = _create_rc(TimeFactor=0.5 * 3 ** 0.5)
+    if torch.cuda.device_count() == 0:
+        assert rc == fdtd_amd.abi.FDTD_ERR_DEVICE, msg
+    else:
+        assert rc == 0, msg

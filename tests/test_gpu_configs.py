@@ -18,31 +18,13 @@ import pytest
 
 import fdtd_amd as G
 import oracle as O
+from fdtd_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
 
 def plain_method_params(mod):
-    p = mod.default_params()
-    p.NCELLX = p.NCELLY = p.NCELLZ = 64
-    p.NcpmlX = p.NcpmlY = p.NcpmlZ = 10
-    p.dx = p.dy = p.dz = 2e-9
-    p.cpml_exp, p.kappa_max, p.alpha_max = 3, 5.0, 0.05
-    p.tfsf_inset = 8
-    p.sca_offset = 4
-    p.abs_offset = 5
-    p.inc_phi = 90.0
-    p.TimeFactor = 3 ** 0.5
-    p.lam_0 = 600e-9
-    p.BandWidth = 6.79e14
-    p.nano_sphere = 1
-    p.nano_sphere_radius = 20e-9
-    p.material = 3
-    p.Tend = 1000
-    p.NUM_freq = 16
-    p.lam_min, p.lam_max = 400e-9, 900e-9
-    p.dft_block = 64
-    return p
+    return W.plain_method(mod)
 
 
 def field_errors(s, o):
@@ -136,3 +118,113 @@ def test_adjoint_gradient_against_finite_differences():
         fd = (objective(rp) - objective(rm)) / (2 * h)
         print("voxel", v, "adjoint %.6e fd %.6e rel-to-max %.2e" % (g[v], fd, abs(fd - g[v]) / gmax), flush=True)
         assert abs(fd - g[v]) < 1e-3 * gmax
+
+
+# ---------------------------------------------------------------- the 32-rod sweep
+
+def test_sweep_32_seeded_rods_batched_equals_alone():
+    """The sweep workload itself: the 32 seeded nanorod draws (edges 96-192,
+    dx = 1.5 nm) advanced in one batch with the simulations' own tiles equal
+    each simulation stepped alone, bit for bit, over 64 steps (graph replay of
+    16 sampling periods)."""
+    from fdtd_amd import sweep
+    cfgs, redraws = sweep.draw(20240601)
+    assert len(cfgs) == 32 and redraws == 3
+    a = [sweep.make_sim(G.Simulation, G, c) for c in cfgs]
+    b = G.Batch(a, flags=G.BATCH_NO_TUNE)
+    b.run(64)
+    b.close()
+    for i, c in enumerate(cfgs):
+        s = sweep.make_sim(G.Simulation, G, c)
+        s.step(64)
+        for comp in range(6):
+            assert np.array_equal(a[i].get_field(comp), s.get_field(comp)), (i, comp)
+        assert np.array_equal(a[i].flux()["raw"], s.flux()["raw"]), i
+        s.close()
+        a[i].close()
+        print("draw", i, c["edges"], "bit-exact", flush=True)
+
+
+@pytest.mark.parametrize("which", ["smallest", "largest"])
+def test_sweep_extreme_draws_against_the_oracle(which):
+    """The smallest and the largest grid of the 32 draws (their rods, CPML,
+    monitors and polarisation), batched with the autotuned tiles, against the
+    FP64 oracle; a short pulse (BandWidth 2.5e16 Hz) so the fields are large
+    within the 300 steps the oracle runs."""
+    from fdtd_amd import sweep
+    cfgs, _ = sweep.draw(20240601)
+    cells = [np.prod(c["edges"]) for c in cfgs]
+    cfg = cfgs[int(np.argmin(cells) if which == "smallest" else np.argmax(cells))]
+    steps = 300
+
+    def params(mod):
+        p = sweep.params_for(mod, cfg, steps)
+        p.BandWidth = 2.5e16
+        p.lam_0 = 600e-9
+        return p
+
+    s = G.Simulation(params(G))
+    c = [n // 2 for n in s.shape]
+    s.add_rod(c, cfg["axis"], cfg["length"], cfg["diameter"] / 2)
+    b = G.Batch([s])
+    b.run(steps)
+    b.close()
+    o = O.Oracle(params(O))
+    o.set_materials(s.get_materials(1), s.get_materials(2), s.get_materials(3))
+    o.step(steps)
+    errs = field_errors(s, o)
+    fs, fo = s.flux(), o.flux()
+    e = spec_err(fs["raw"], fo["raw"])
+    print(which, cfg["edges"], "fields", ["%.1e" % x for x in errs], "face fluxes %.1e" % e, flush=True)
+    assert s.get_info().n_dispersive[0] > 0
+    assert max(errs) < 2e-4
+    assert e < 1e-4
+
+
+# ---------------------------------------------------------------- inverse design at full size
+
+@pytest.mark.timeout(900)
+def test_inverse_design_gradient_at_full_size():
+    """The inverse-design workload as configured: 256^3 interior at dx = 2 nm
+    (281^3 cells), a 64x64x32 density block (rho ~ U[0, 1), seeded), |E|^2 at
+    633/785/850 nm 5 nm above the block, 10,000 steps with a checkpoint every
+    500 and segment recompute.  dJ/drho against central finite differences of
+    the GPU's own objective with h = 1e-3 on 16 seeded voxels, within 1e-3 of
+    the largest gradient entry."""
+    p = W.design(G)
+    T = 256 + 25
+    lo, r0 = W.design_geometry(T)
+    rho = np.random.default_rng(7).uniform(0.0, 1.0, size=W.DESIGN_BLOCK).astype(np.float32)
+    freqs = np.array(W.design_freqs())
+    steps = W.DESIGN_STEPS
+    sim = G.Simulation(p)
+    sim.set_design(lo, rho)
+    J, g = sim.adjoint(steps, r0, freqs, checkpoint=W.DESIGN_CHECKPOINT)
+    sim.close()
+    print("J %.6e |g| %.6e max|g| %.6e" % (J, np.linalg.norm(g), np.abs(g).max()), flush=True)
+    assert J > 0 and np.all(np.isfinite(g))
+
+    def objective(r):
+        s = G.Simulation(p)
+        s.set_design(lo, r)
+        s.set_point_monitor(r0, steps)
+        s.step(steps)
+        e2, _ = s.point_spectrum(freqs)
+        s.close()
+        return float(np.sum(e2))
+
+    assert objective(rho) == pytest.approx(J, rel=1e-12)
+    h = np.float32(1e-3)
+    rng = np.random.default_rng(11)
+    flat = [tuple(int(v) for v in np.unravel_index(q, rho.shape)) for q in rng.choice(rho.size, 16, replace=False)]
+    gmax = np.abs(g).max()
+    worst = 0.0
+    for v in flat:
+        rp, rm = rho.copy(), rho.copy()
+        rp[v] = min(rho[v] + h, 1.0)
+        rm[v] = max(rho[v] - h, 0.0)
+        fd = (objective(rp) - objective(rm)) / (float(rp[v]) - float(rm[v]))
+        err = abs(fd - g[v]) / gmax
+        worst = max(worst, err)
+        print("voxel", v, "adjoint %.6e fd %.6e rel-to-max %.2e" % (g[v], fd, err), flush=True)
+    assert worst < 1e-3
