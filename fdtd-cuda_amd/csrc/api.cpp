@@ -48,6 +48,7 @@ StepDev make_step(fdtd_sim *s, bool fused) {
     }
     TfsfDev &t = d.tf;
     for (int a = 0; a < 3; a++) { t.t0[a] = S.tf0[a]; t.t1[a] = S.tf1[a]; t.inv_d[a] = (float)(1.0 / S.P.dx); }
+    if (S.P.periodic_xy) t.t1[2] = 1 << 30;   // the far z plane is not corrected (main.cu:698)
     const double st = std::sin(S.theta), ct = std::cos(S.theta), sp = std::sin(S.phi), cph = std::cos(S.phi);
     const double sps = std::sin(S.psi), cps = std::cos(S.psi);
     t.s[0] = (float)(st * cph); t.s[1] = (float)(st * sp); t.s[2] = (float)ct;
@@ -148,6 +149,12 @@ DispDev make_disp(fdtd_sim *s, int parity) {
         d.traj_prev[0][c] = s->traj_first[0][c];
         d.traj_prev[1][c] = s->traj_first[1][c];
     }
+    if (S.nonlocal1) {
+        for (int c = 0; c < 3; c++) d.gd[c] = s->nl_gd[c];
+        d.nl_e = (float)(A.dNL * e0 / CE);
+        d.nl_p = (float)A.dNL;
+        d.nl_i2 = (float)(1.0 / (S.P.dx * S.P.dx));
+    }
     d.hydro = S.hydro ? 1 : 0;
     if (S.hydro) {
         const double dt = S.dt, dx = S.P.dx;
@@ -176,7 +183,7 @@ int build_dispersive(fdtd_sim *s) {
             for (int j = 0; j < NY; j++)
                 for (int k = 0; k < NZ; k++) {
                     const uint8_t v = m[((size_t)i * NY + j) * NZ + k];
-                    if (v > 1 && v < 6) {
+                    if (fdtd::dispersive_id(v)) {
                         s->n_disp[c]++;
                         lo[0] = std::min(lo[0], i); lo[1] = std::min(lo[1], j); lo[2] = std::min(lo[2], k);
                         hi[0] = std::max(hi[0], i); hi[1] = std::max(hi[1], j); hi[2] = std::max(hi[2], k);
@@ -189,14 +196,24 @@ int build_dispersive(fdtd_sim *s) {
         s->nbox = 0;
         return FDTD_OK;
     }
-    for (int a = 0; a < 3; a++) {
-        // dispersive cells must sit strictly inside the absorption monitor box, which keeps
-        // them off the CPML, the TFSF faces and every monitor face
-        if (lo[a] <= S.abs_s[a] || hi[a] >= S.abs_e[a] - 1)
-            return set_err(s, FDTD_ERR_INVALID, "dispersive cells must lie strictly inside the absorption box");
+    if (S.P.periodic_xy) {
+        // between the source plane and the transmission plane (off the z CPML, the TFSF
+        // plane and both monitor planes); x and y unrestricted, no halo across the wrap
+        if (lo[2] <= S.tf0[2] || hi[2] >= S.sca_e[2] - 1)
+            return set_err(s, FDTD_ERR_INVALID, "periodic: dispersive cells must lie between the source plane and the transmission plane");
+        for (int a = 0; a < 2; a++) { s->box_lo[a] = lo[a]; s->box_n[a] = hi[a] - lo[a] + 1; }
+        s->box_lo[2] = lo[2] - 1;
+        s->box_n[2] = hi[2] - lo[2] + 3;
+    } else {
+        for (int a = 0; a < 3; a++) {
+            // dispersive cells must sit strictly inside the absorption monitor box, which keeps
+            // them off the CPML, the TFSF faces and every monitor face
+            if (lo[a] <= S.abs_s[a] || hi[a] >= S.abs_e[a] - 1)
+                return set_err(s, FDTD_ERR_INVALID, "dispersive cells must lie strictly inside the absorption box");
+        }
+        // one halo cell on every side: hydro stencils read neighbours of metal cells
+        for (int a = 0; a < 3; a++) { s->box_lo[a] = lo[a] - 1; s->box_n[a] = hi[a] - lo[a] + 3; }
     }
-    // one halo cell on every side: hydro stencils read neighbours of metal cells
-    for (int a = 0; a < 3; a++) { s->box_lo[a] = lo[a] - 1; s->box_n[a] = hi[a] - lo[a] + 3; }
     s->nbox = (long)s->box_n[0] * s->box_n[1] * s->box_n[2];
     for (int c = 0; c < 3; c++) {
         std::vector<uint8_t> ids(s->nbox, 0);
@@ -205,7 +222,7 @@ int build_dispersive(fdtd_sim *s) {
             for (int bj = 0; bj < s->box_n[1]; bj++)
                 for (int bk = 0; bk < s->box_n[2]; bk++) {
                     const uint8_t v = m[((size_t)(s->box_lo[0] + bi) * NY + s->box_lo[1] + bj) * NZ + s->box_lo[2] + bk];
-                    ids[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk] = (v > 1 && v < 6) ? 1 : 0;
+                    ids[((size_t)bi * s->box_n[1] + bj) * s->box_n[2] + bk] = fdtd::dispersive_id(v) ? 1 : 0;
                 }
         int rc = upload(s, &s->disp_id[c], ids);
         if (rc) return rc;
@@ -218,6 +235,7 @@ int build_dispersive(fdtd_sim *s) {
                 (rc = dalloc(s, &s->h_ndp[c], s->nbox)))
                 return rc;
         }
+        if (S.nonlocal1 && (rc = dalloc(s, &s->nl_gd[c], s->nbox))) return rc;
     }
     s->rho_box = nullptr;
     if (!s->des_rho.empty()) {
@@ -248,6 +266,8 @@ void build_faces(fdtd_sim *s) {
         f.na[q] = en[outer] - st[outer];
         f.lo_b[q] = st[inner];
         f.nb[q] = en[inner] - st[inner];
+        // periodic x, y: the reflection and transmission planes are the z faces of box 0
+        if (S.P.periodic_xy && (box == 1 || axis != 2)) f.na[q] = f.nb[q] = 0;
         f.off[q] = off;
         off += 2L * f.na[q] * f.nb[q];
     }
@@ -312,6 +332,7 @@ hipEvent_t next_event(fdtd_sim *s, int which) {
 int plan_rows(fdtd_sim *s) {
     const Setup &S = s->S;
     s->use_rows = false;
+    if (S.P.periodic_xy) return FDTD_OK;   // periodic steps run their own kernels
     if (const char *v = std::getenv("FDTD_PATH"))
         if (std::string(v) == "general") return FDTD_OK;
     if (const char *v = std::getenv("FDTD_ROWS")) s->rows = std::min(3, std::max(2, std::atoi(v)));
@@ -833,7 +854,7 @@ int fdtd_set_design(fdtd_sim *s, const int lo[3], const int n[3], const float *r
         if (n[a] < 1 || lo[a] <= S.abs_s[a] || lo[a] + n[a] - 1 >= S.abs_e[a] - 1)
             return set_err(s, FDTD_ERR_INVALID, "design block must lie strictly inside the absorption box");
     }
-    if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "design densities need the local Drude-CP model");
+    if (S.hydro || S.nonlocal1) return set_err(s, FDTD_ERR_UNSUPPORTED, "design densities need the local model (NONLOCAL 0)");
     const size_t nv = (size_t)n[0] * n[1] * n[2];
     for (size_t v = 0; v < nv; v++)
         if (!(rho[v] >= 0.f && rho[v] <= 1.f)) return set_err(s, FDTD_ERR_INVALID, "densities must lie in [0, 1]");
@@ -996,7 +1017,8 @@ int fdtd_adjoint(fdtd_sim *s, int nsteps, int ckpt, const int r0[3], int nf, con
     if (!s || !r0 || !freq || !J || !grad || nsteps < 1 || nf < 1) return FDTD_ERR_INVALID;
     const Setup &S = s->S;
     if (s->des_rho.empty() || !s->rho_box) return set_err(s, FDTD_ERR_INVALID, "adjoint: no design region set");
-    if (S.hydro) return set_err(s, FDTD_ERR_UNSUPPORTED, "adjoint: hydrodynamic model not supported");
+    if (S.hydro || S.nonlocal1) return set_err(s, FDTD_ERR_UNSUPPORTED, "adjoint: nonlocal models not supported");
+    if (S.P.periodic_xy) return set_err(s, FDTD_ERR_UNSUPPORTED, "adjoint: periodic simulations not supported");
     for (int a = 0; a < 3; a++)
         if (r0[a] < 0 || r0[a] >= S.N[a]) return set_err(s, FDTD_ERR_INVALID, "adjoint: objective cell outside the grid");
     for (int f = 0; f < nf; f++)
@@ -1222,15 +1244,21 @@ int fdtd_get_materials(const fdtd_sim *s, int comp, uint8_t *out) {
     return FDTD_OK;
 }
 
-int fdtd_update_h(fdtd_sim *s) {
-    if (!s) return FDTD_ERR_INVALID;
-    StepDev d = make_step(s, false);
+// Split-path arguments updating the current level in place.
+static StepDev inplace_step(fdtd_sim *s, bool tfsf) {
+    StepDev d = make_step(s, tfsf);
     for (int c = 0; c < 3; c++) {
         d.Ein[c] = s->E[s->p][c]; d.Hin[c] = s->H[s->p][c];
         d.Eout[c] = s->E[s->p][c]; d.Hout[c] = s->H[s->p][c];
     }
     for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = d.psiH_out.p[a] = s->psiH[s->pp][a]; d.psiE.p[a] = s->psiE[a]; }
-    HIPCHK(s, launch_split_h(d, s->stream));
+    return d;
+}
+
+int fdtd_update_h(fdtd_sim *s) {
+    if (!s) return FDTD_ERR_INVALID;
+    const StepDev d = inplace_step(s, false);
+    HIPCHK(s, s->S.P.periodic_xy ? launch_periodic_h(d, s->stream) : launch_split_h(d, s->stream));
     return FDTD_OK;
 }
 
@@ -1240,20 +1268,16 @@ int fdtd_tfsf(fdtd_sim *s) {
     HIPCHK(s, launch_line(l, s->stream));
     StepDev d = make_step(s, false);
     for (int c = 0; c < 3; c++) { d.Eout[c] = s->E[s->p][c]; d.Hout[c] = s->H[s->p][c]; }
-    HIPCHK(s, launch_tfsf_split(d, s->stream));
+    HIPCHK(s, s->S.P.periodic_xy ? launch_periodic_tfsf(d, s->stream) : launch_tfsf_split(d, s->stream));
     s->le ^= 1;
     return FDTD_OK;
 }
 
 int fdtd_update_e(fdtd_sim *s) {
     if (!s) return FDTD_ERR_INVALID;
-    StepDev d = make_step(s, false);
-    for (int c = 0; c < 3; c++) {
-        d.Ein[c] = s->E[s->p][c]; d.Hin[c] = s->H[s->p][c];
-        d.Eout[c] = s->E[s->p][c]; d.Hout[c] = s->H[s->p][c];
-    }
-    for (int a = 0; a < 6; a++) { d.psiH_in.p[a] = d.psiH_out.p[a] = s->psiH[s->pp][a]; d.psiE.p[a] = s->psiE[a]; }
-    HIPCHK(s, launch_split_e(d, s->stream));
+    const StepDev d = inplace_step(s, false);
+    if (s->nbox && s->S.nonlocal1) HIPCHK(s, launch_nonlocal_gd(make_disp(s, s->p), s->stream));
+    HIPCHK(s, s->S.P.periodic_xy ? launch_periodic_e(d, s->stream) : launch_split_e(d, s->stream));
     if (s->nbox) {
         DispDev dd = make_disp(s, s->p);
         HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
@@ -1279,9 +1303,29 @@ int fdtd_dft_accumulate(fdtd_sim *s) {
     return rc;
 }
 
+// One periodic step (Periodic_XY): line, E ghosts + H with the source
+// plane, H ghosts + E with the source plane, dispersive box, monitors; the
+// reference's order (main.cu:2724-2934) with its corrections folded into the
+// updates.  Fields update in place.
+static int periodic_step(fdtd_sim *s) {
+    const LineDev l = make_line(s);
+    HIPCHK(s, launch_line(l, s->stream));
+    const StepDev d = inplace_step(s, true);
+    HIPCHK(s, launch_periodic_h(d, s->stream));
+    HIPCHK(s, launch_periodic_e(d, s->stream));
+    s->le ^= 1;
+    if (s->nbox) HIPCHK(s, launch_dispersive(make_disp(s, s->p), s->stream));
+    return fdtd_dft_accumulate(s);
+}
+
 int fdtd_step(fdtd_sim *s, int nsteps) {
     if (!s || nsteps < 0) return FDTD_ERR_INVALID;
     if (int rc = check_finite(s)) return rc;
+    if (s->S.P.periodic_xy) {
+        for (int n = 0; n < nsteps; n++)
+            if (int rc = periodic_step(s)) return rc;
+        return FDTD_OK;
+    }
     for (int n = 0; n < nsteps; n++) {
         LineDev l = make_line(s);
         StepDev d = make_step(s, true);
@@ -1302,6 +1346,9 @@ int fdtd_step(fdtd_sim *s, int nsteps) {
         const bool only_clean = step_timing && s->timing_what == 2 && s->use_rows;
         DispDev dd{};
         if (s->nbox) dd = make_disp(s, 1 - q);
+        // NONLOCAL == 1: grad(div P_d^n) from the box state of the last step, ahead of every
+        // launch that can update the box
+        if (s->nbox && s->S.nonlocal1) HIPCHK(s, launch_nonlocal_gd(dd, s->stream));
         bool disp_done = false, samp_done = false;
         if (s->use_rows) {
             d.tfsf_on = 0;
@@ -1554,3 +1601,43 @@ int fdtd_kernel_timing(fdtd_sim *s, int enable, double *total_ms, long *launches
     return FDTD_OK;
 }
 
+
+int fdtd_reflectance(fdtd_sim *s, double *R, double *T, double *ref_r, double *ref_t) {
+    if (!s) return FDTD_ERR_INVALID;
+    const Setup &S = s->S;
+    if (!S.P.periodic_xy) return set_err(s, FDTD_ERR_INVALID, "fdtd_reflectance needs periodic_xy");
+    const int F = S.P.NUM_freq;
+    std::vector<double> raw(12 * (size_t)F);
+    if (int rc = fdtd_flux(s, nullptr, nullptr, nullptr, raw.data())) return rc;
+    // the incident wave at the source plane, sampled like the monitors: the same
+    // FP64 line replayed on the host for the steps run so far
+    const std::vector<std::complex<double>> inc = fdtd::line_dft(S, s->t);
+    const double area = S.N[0] * S.P.dx * S.N[1] * S.P.dy;
+    for (int m = 0; m < F; m++) {
+        const double a2 = std::norm(S.Einc[m]);
+        // faces 4, 5: S_z = sum Re(Ex Hy* - Ey Hx*) dA, stored over |E_incident|^2
+        const double sr = raw[4 * (size_t)F + m] * a2, st = raw[5 * (size_t)F + m] * a2;
+        const double pin = std::norm(inc[m]) * area;
+        if (R) R[m] = pin > 0 ? -sr / pin : 0.0;
+        if (T) T[m] = pin > 0 ? st / pin : 0.0;
+        if (ref_r) ref_r[m] = -sr / area / a2;
+        if (ref_t) ref_t[m] = -st / area / a2;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_add_layer(fdtd_sim *s, int k0, int k1) {
+    if (!s || k0 >= k1) return FDTD_ERR_INVALID;
+    Setup &S = s->S;
+    const int NX = S.N[0], NY = S.N[1], NZ = S.N[2];
+    if (k0 < 0 || k1 > NZ) return set_err(s, FDTD_ERR_INVALID, "fdtd_add_layer: layer outside the grid");
+    std::vector<uint8_t> m[4] = {S.mat[0], S.mat[1], S.mat[2], S.mat[3]};
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+            for (int k = k0; k < k1; k++)
+                for (int c = 0; c < 4; c++) m[c][((size_t)i * NY + j) * NZ + k] = (uint8_t)S.P.material;
+    const int rc = fdtd_set_materials(s, m[1].data(), m[2].data(), m[3].data());
+    if (rc) return rc;
+    S.mat[0] = m[0];
+    return FDTD_OK;
+}

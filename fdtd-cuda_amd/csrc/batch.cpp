@@ -504,7 +504,8 @@ int check_sims(fdtd_batch *b) {
         if (s->t != s0->t) return berr(b, FDTD_ERR_INVALID, "batch: simulations must be at the same step");
         if (s->S.P.dft_stride != s0->S.P.dft_stride || s->K != s0->K || s->S.P.StaticField != s0->S.P.StaticField)
             return berr(b, FDTD_ERR_INVALID, "batch: simulations must share dft_stride, dft_block and StaticField");
-        if (s->S.hydro) return berr(b, FDTD_ERR_UNSUPPORTED, "batch: hydrodynamic simulations are not batched");
+        if (s->S.hydro || s->S.nonlocal1 || s->S.P.periodic_xy)
+            return berr(b, FDTD_ERR_UNSUPPORTED, "batch: nonlocal (NONLOCAL 1 or 3) and periodic simulations are not batched");
         if (s->probe || !s->sampling) return berr(b, FDTD_ERR_INVALID, "batch: simulation is inside an adjoint run");
         if (s->mon) return berr(b, FDTD_ERR_UNSUPPORTED, "batch: point monitors are recorded by fdtd_step only");
     }

@@ -19,6 +19,7 @@ constexpr double kC0 = 2.99792458E+08;
 constexpr double kZ0 = 376.730313462;
 constexpr double kE0 = 1.60217662E-19;
 constexpr double kME = 9.10938356E-31;
+constexpr double kPLANKS = 4.13566553853599E-15;   // eV s
 
 // One CPML axis: recursion coefficients (b, c) at E positions (Nc+1 per
 // slab) and H positions (Nc per slab), and the kappa-scaled spacings.
@@ -51,11 +52,17 @@ struct Setup {
     double theta = 0, phi = 0, psi = 0;
     double Chh = 1, Che = 0, Cee = 1, Ceh = 0;
     bool hydro = false;
+    bool nonlocal1 = false;         // NONLOCAL == 1: grad-div term on the Drude pole
     Ade ade;
     std::vector<double> freq;
     std::vector<std::complex<double>> Einc;
     std::vector<uint8_t> mat[4];    // mat_matrix, X, Y, Z
 };
+
+// Dispersive E-component cells: the reference's 1 < id < 6 (main.cu:1175),
+// plus id 7 so the Lorentz-Drude material (setup.cu:1256-1275), which the
+// reference's kernels never treat as dispersive, takes the ADE branch.
+inline bool dispersive_id(int v) { return (v > 1 && v < 6) || v == 7; }
 
 void default_params(fdtd_params *p);
 // Builds everything fdtd_create needs; returns FDTD_OK, or FDTD_ERR_INVALID /
@@ -68,6 +75,9 @@ void sphere_masks_at(const Setup &s, const double centre[3], double radius, std:
 void rod_masks_at(const Setup &s, const double centre[3], int axis, double length, double radius,
                   std::vector<uint8_t> m[4]);
 double source_pulse(const Setup &s, int t);
+// DFT of the incident line at the source plane (e_inc[m0]) over the first
+// `steps` steps, sampled and phased as the DFT monitors; one value per frequency.
+std::vector<std::complex<double>> line_dft(const Setup &s, int steps);
 
 }  // namespace fdtd
 

@@ -121,6 +121,10 @@ struct DispDev {
     float h_d1m1, h_d3, h_d4;       // v^{n+1} = v + (d1-1) J + d3 E^n + d4 E^{n-1} + h_nl (gd + gd2) / cbrt(1 + n~)
     float h_nl, h_inv_d, h_cont;    // h_nl = beta^2/dd, 1/dx, 2 dt/dx
     float h_cur;                    // dt e0 N_EQ / C_E: current term of the E update
+    // NONLOCAL == 1: gd[c] = grad(div P~_d^n) at each dispersive cell of component c
+    // (k_nonlocal_gd, before the box update); E^{n+1} -= nl_e gd, J~_d^{n+1} += nl_p gd
+    float *gd[3];
+    float nl_e, nl_p, nl_i2;        // d_NL eps0 / C_E, d_NL, 1/dx^2
 };
 
 // Adjoint step arguments.  The transposed step has the stencil shape of the
@@ -223,6 +227,12 @@ hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st);
 // box[q] = {ia, ib, ja, jb}; one to four disjoint boxes.
 hipError_t launch_fused_step(const StepDev &s, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st);
 hipError_t launch_split_h(const StepDev &s, hipStream_t st);
+// Periodic x, y: E ghost wrap + H update (with the near z plane of the TFSF
+// when s.tfsf_on), H ghost wrap + E update, and the plane corrections alone
+// for the split path.  In place: Ein == Eout, Hin == Hout.
+hipError_t launch_periodic_h(const StepDev &s, hipStream_t st);
+hipError_t launch_periodic_e(const StepDev &s, hipStream_t st);
+hipError_t launch_periodic_tfsf(const StepDev &s, hipStream_t st);
 hipError_t launch_split_e(const StepDev &s, hipStream_t st);
 hipError_t launch_tfsf_split(const StepDev &s, hipStream_t st);
 // TFSF terms added after a fused step run with tfsf_on = 0 (H faces, E faces, curl of dH).
@@ -249,6 +259,8 @@ hipError_t launch_pack_field(const float *f, const Geom &g, float *dense, int to
 hipError_t launch_disp_sync_state(const DispDev &d, int from_field, hipStream_t st);
 // Hydrodynamic model: velocity update, then per component (x, y, z) continuity and E update.
 hipError_t launch_hydro(const DispDev &d, hipStream_t st);
+// NONLOCAL == 1: grad(div) of the Drude polarisation into d.gd (no-op when d.gd[0] is null).
+hipError_t launch_nonlocal_gd(const DispDev &d, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Batched launches: one launch runs the same kernel over many independent

@@ -1200,6 +1200,133 @@ __global__ void k_split_e(StepDev s) {
     s.Eout[0][p] = ex; s.Eout[1][p] = ey; s.Eout[2][p] = ez;
 }
 
+// ----------------------------------------------------------------- periodic x, y (Periodic_XY)
+//
+// x and y wrap around (the i == NCELLX-1 / j == NCELLY-1 and i == 0 / j == 0
+// branches of UPDATE_h* and UPDATE_e*, main.cu:781-1044, 1067-1912, real
+// build): every column is updated, the neighbours across the x and y edges
+// are read from the ghost layers, which k_wrap_e / k_wrap_h fill with the
+// opposite face before each half step, so the stencils need no wrap logic;
+// CPML along z only.  With tfsf_on the near z plane of the TFSF box adds the
+// incident wave (CORRECT_Z near face, main.cu:638-697; Periodic_XY skips the
+// far face, main.cu:698): H at k0-1 after its update, E^n at k0 before.
+// Updates run in place (E and H of the current level), as the split path.
+
+// E ghosts for the H update: plane x = NX <- x = 0 (rows 0..NY-1), then row
+// y = NY <- y = 0 for x = 0..NX (the corner follows from the ghost plane).
+__global__ void k_wrap_e(float *ex, float *ey, float *ez, Geom g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = blockIdx.y;
+    if (k >= g.NZ) return;
+    float *f[3] = {ex, ey, ez};
+    if (blockIdx.z == 0) {
+        if (u >= g.NY) return;
+        const long d = gidx(g, g.NX, u, k), src = gidx(g, 0, u, k);
+        for (int c = 0; c < 3; c++) f[c][d] = f[c][src];
+    } else {
+        if (u > g.NX) return;
+        const long d = gidx(g, u, g.NY, k), src = gidx(g, u == g.NX ? 0 : u, 0, k);
+        for (int c = 0; c < 3; c++) f[c][d] = f[c][src];
+    }
+}
+
+// H ghosts for the E update: plane x = -1 <- x = NX-1, row y = -1 <- y = NY-1
+// for x = -1..NX-1.
+__global__ void k_wrap_h(float *hx, float *hy, float *hz, Geom g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = blockIdx.y;
+    if (k >= g.NZ) return;
+    float *f[3] = {hx, hy, hz};
+    if (blockIdx.z == 0) {
+        if (u >= g.NY) return;
+        const long d = gidx(g, -1, u, k), src = gidx(g, g.NX - 1, u, k);
+        for (int c = 0; c < 3; c++) f[c][d] = f[c][src];
+    } else {
+        if (u > g.NX) return;
+        const int i = u - 1;
+        const long d = gidx(g, i, -1, k), src = gidx(g, i < 0 ? g.NX - 1 : i, g.NY - 1, k);
+        for (int c = 0; c < 3; c++) f[c][d] = f[c][src];
+    }
+}
+
+__global__ void k_per_h(StepDev s) {
+    const Geom &g = s.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.NZ) return;
+    const long p = gidx(g, i, j, k);
+    const CpmlDev &cz = s.cp[2];
+    const float Chh = s.c.Chh, Che = s.c.Che;
+    const float ikx = s.cp[0].ik_h[i], iky = s.cp[1].ik_h[j];
+    const float *ex = s.Ein[0], *ey = s.Ein[1], *ez = s.Ein[2];
+    if (k < g.NZ - 1) {
+        const int lz = h_layer(cz, k), Lz = 2 * cz.Nc;
+        const float ikz = cz.ik_h[k];
+        // hx (main.cu:781-865)
+        const float dEy = ey[p + 1] - ey[p], dEzj = ez[p + g.sz] - ez[p];
+        float h = Chh * s.Hin[0][p] + Che * (dEy * ikz - dEzj * iky);
+        if (lz >= 0) h += Che * psi_step(s.psiH_in.p[0], s.psiH_out.p[0], zslab(g, Lz, i, j, lz), cz.bh[lz], cz.ch[lz], dEy, true);
+        if (s.tfsf_on) h += tfsf_term_ct<false, 0, 2>(s.tf, i, j, k);
+        s.Hout[0][p] = h;
+        // hy (main.cu:877-948)
+        const float dEzi = ez[p + g.sxy] - ez[p], dEx = ex[p + 1] - ex[p];
+        h = Chh * s.Hin[1][p] + Che * (dEzi * ikx - dEx * ikz);
+        if (lz >= 0) h -= Che * psi_step(s.psiH_in.p[2], s.psiH_out.p[2], zslab(g, Lz, i, j, lz), cz.bh[lz], cz.ch[lz], dEx, true);
+        if (s.tfsf_on) h += tfsf_term_ct<false, 1, 2>(s.tf, i, j, k);
+        s.Hout[1][p] = h;
+    }
+    // hz (main.cu:962-1044), every k
+    const float dExj = ex[p + g.sz] - ex[p], dEyi = ey[p + g.sxy] - ey[p];
+    s.Hout[2][p] = Chh * s.Hin[2][p] + Che * (dExj * iky - dEyi * ikx);
+}
+
+__global__ void k_per_e(StepDev s) {
+    const Geom &g = s.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.NZ - 1) return;
+    const long p = gidx(g, i, j, k);
+    const CpmlDev &cz = s.cp[2];
+    const float Cee = s.c.Cee, Ceh = s.c.Ceh;
+    const float ikx = s.cp[0].ik_e[i], iky = s.cp[1].ik_e[j];
+    const float *hx = s.Hin[0], *hy = s.Hin[1], *hz = s.Hin[2];
+    if (k > 0) {
+        const int lz = e_layer(cz, k), Lz = 2 * (cz.Nc + 1);
+        const float ikz = cz.ik_e[k];
+        // ex (main.cu:1067-1170)
+        float eo = s.Ein[0][p];
+        if (s.tfsf_on) eo += tfsf_term_ct<true, 0, 2>(s.tf, i, j, k);
+        const float dHzj = hz[p] - hz[p - g.sz], dHy = hy[p] - hy[p - 1];
+        float e = Cee * eo + Ceh * (dHzj * iky - dHy * ikz);
+        if (lz >= 0) e -= Ceh * psi_step(s.psiE.p[0], s.psiE.p[0], zslab(g, Lz, i, j, lz), cz.be[lz], cz.ce[lz], dHy, true);
+        s.Eout[0][p] = e;
+        // ey (main.cu:1359-1475)
+        eo = s.Ein[1][p];
+        if (s.tfsf_on) eo += tfsf_term_ct<true, 1, 2>(s.tf, i, j, k);
+        const float dHx = hx[p] - hx[p - 1], dHzi = hz[p] - hz[p - g.sxy];
+        e = Cee * eo + Ceh * (dHx * ikz - dHzi * ikx);
+        if (lz >= 0) e += Ceh * psi_step(s.psiE.p[2], s.psiE.p[2], zslab(g, Lz, i, j, lz), cz.be[lz], cz.ce[lz], dHx, true);
+        s.Eout[1][p] = e;
+    }
+    // ez (main.cu:1663-1766), k < NZ-1
+    const float dHyi = hy[p] - hy[p - g.sxy], dHxj = hx[p] - hx[p - g.sz];
+    s.Eout[2][p] = Cee * s.Ein[2][p] + Ceh * (dHyi * ikx - dHxj * iky);
+}
+
+// Split path of the periodic mode: the near z plane corrections in place
+// (H at k0-1 with e_inc^n, E at k0 with h_inc^{n+1/2}), after the H update.
+__global__ void k_per_tfsf(StepDev s) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= s.g.NY) return;
+    const int k0 = s.tf.t0[2];
+    const long ph = gidx(s.g, i, j, k0 - 1), pe = gidx(s.g, i, j, k0);
+    s.Hout[0][ph] += tfsf_term_ct<false, 0, 2>(s.tf, i, j, k0 - 1);
+    s.Hout[1][ph] += tfsf_term_ct<false, 1, 2>(s.tf, i, j, k0 - 1);
+    s.Eout[0][pe] += tfsf_term_ct<true, 0, 2>(s.tf, i, j, k0);
+    s.Eout[1][pe] += tfsf_term_ct<true, 1, 2>(s.tf, i, j, k0);
+}
+
 // CORRECT_X/Y/Z in place over the bounding box of the TFSF planes.
 __global__ void k_tfsf_split(StepDev s) {
     const TfsfDev &t = s.tf;
@@ -1342,9 +1469,13 @@ __device__ __forceinline__ void disp_body(const DispDev &d, long idx) {
         float e = a0 * curl + a1 * en - a2 * en1;
 #pragma unroll
         for (int q = 0; q < 3; q++) e -= e0[q] * P[q] + e1[q] * J[q];
+        // NONLOCAL == 1: d_NL grad(div P_d^n) in the E update and the Drude pole (updatefields.cu:876-891)
+        const float gdv = d.gd[0] ? d.gd[c][idx] : 0.f;
+        e -= d.nl_e * gdv;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            const float jn = d.g1[q] * J[q] + d.g0[q] * P[q] + r * (d.p3[q] * e + d.p4[q] * en + d.p5[q] * en1);
+            float jn = d.g1[q] * J[q] + d.g0[q] * P[q] + r * (d.p3[q] * e + d.p4[q] * en + d.p5[q] * en1);
+            if (q == 0) jn += d.nl_p * gdv;
             st[2 + 2 * q][idx] = P[q] + jn;
             st[3 + 2 * q][idx] = jn;
         }
@@ -1357,6 +1488,37 @@ __device__ __forceinline__ void disp_body(const DispDev &d, long idx) {
 }
 
 __global__ void k_dispersive(DispDev d) { disp_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x); }
+
+// NONLOCAL == 1: grad(div P~_d) of the Drude pole at every dispersive cell
+// of each component, all neighbours at time level n (Calc_DIV_GRADx/y/z,
+// updatefields.cu:1436-1604: the x component is (Px[i+1] - 2 Px[i] +
+// Px[i-1])/dx^2 + (Py[i+1,j] - Py[i,j] - Py[i+1,j-1] + Py[i,j-1])/(dx dy) +
+// (Pz[i+1,k] - Pz[i,k] - Pz[i+1,k-1] + Pz[i,k-1])/(dx dz), y and z by
+// symmetry).  P~_d is zero on non-metal cells (hard wall).  One thread per
+// box cell; the box keeps a one-cell halo, so every stencil stays inside.
+__global__ void k_nonlocal_gd(DispDev d) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    const int bk = (int)(idx % d.n[2]);
+    const int bj = (int)((idx / d.n[2]) % d.n[1]);
+    const int bi = (int)(idx / ((long)d.n[2] * d.n[1]));
+    if (!(bi > 0 && bj > 0 && bk > 0 && bi < d.n[0] - 1 && bj < d.n[1] - 1 && bk < d.n[2] - 1)) return;
+    const long oi = (long)d.n[1] * d.n[2], oj = d.n[2], ok = 1;
+    const float *P0 = d.st[0][2], *P1 = d.st[1][2], *P2 = d.st[2][2];
+    if (d.id[0][idx])
+        d.gd[0][idx] = d.nl_i2 * ((P0[idx + oi] - 2.f * P0[idx] + P0[idx - oi]) +
+                                  (P1[idx + oi] - P1[idx] - P1[idx + oi - oj] + P1[idx - oj]) +
+                                  (P2[idx + oi] - P2[idx] - P2[idx + oi - ok] + P2[idx - ok]));
+    if (d.id[1][idx])
+        d.gd[1][idx] = d.nl_i2 * ((P1[idx + oj] - 2.f * P1[idx] + P1[idx - oj]) +
+                                  (P0[idx + oj] - P0[idx] - P0[idx - oi + oj] + P0[idx - oi]) +
+                                  (P2[idx + oj] - P2[idx] - P2[idx + oj - ok] + P2[idx - ok]));
+    if (d.id[2][idx])
+        d.gd[2][idx] = d.nl_i2 * ((P2[idx + ok] - 2.f * P2[idx] + P2[idx - ok]) +
+                                  (P1[idx + ok] - P1[idx] - P1[idx - oj + ok] + P1[idx - oj]) +
+                                  (P0[idx + ok] - P0[idx] - P0[idx - oi + ok] + P0[idx - oi]));
+}
 
 // The work that follows the field update of a step in one launch, for cell
 // sets that are pairwise disjoint (the TFSF fix-up adds to its own cells only
@@ -2570,6 +2732,32 @@ hipError_t launch_probe(const float *const E[3], long p0, float *out, hipStream_
     return hipGetLastError();
 }
 
+hipError_t launch_periodic_h(const StepDev &s, hipStream_t st) {
+    const Geom &g = s.g;
+    const dim3 wg((g.NZ + 63) / 64, g.NX + 1 > g.NY ? g.NX + 1 : g.NY, 2), block(64);
+    hipLaunchKernelGGL(k_wrap_e, wg, block, 0, st, const_cast<float *>(s.Ein[0]), const_cast<float *>(s.Ein[1]),
+                       const_cast<float *>(s.Ein[2]), g);
+    const dim3 grid((g.NZ + 63) / 64, g.NY, g.NX);
+    hipLaunchKernelGGL(k_per_h, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_periodic_e(const StepDev &s, hipStream_t st) {
+    const Geom &g = s.g;
+    const dim3 wg((g.NZ + 63) / 64, g.NX + 1 > g.NY ? g.NX + 1 : g.NY, 2), block(64);
+    hipLaunchKernelGGL(k_wrap_h, wg, block, 0, st, const_cast<float *>(s.Hin[0]), const_cast<float *>(s.Hin[1]),
+                       const_cast<float *>(s.Hin[2]), g);
+    const dim3 grid((g.NZ + 63) / 64, g.NY, g.NX);
+    hipLaunchKernelGGL(k_per_e, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_periodic_tfsf(const StepDev &s, hipStream_t st) {
+    const dim3 grid((s.g.NY + 63) / 64, s.g.NX), block(64);
+    hipLaunchKernelGGL(k_per_tfsf, grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
 hipError_t launch_split_h(const StepDev &s, hipStream_t st) {
     dim3 grid((s.g.NZ + 63) / 64, s.g.NY, s.g.NX), block(64);
     hipLaunchKernelGGL(k_split_h, grid, block, 0, st, s);
@@ -2635,6 +2823,13 @@ hipError_t launch_hydro(const DispDev &d, hipStream_t st) {
         hipLaunchKernelGGL(k_hydro_cont, grid, block, 0, st, d, c);
         hipLaunchKernelGGL(k_hydro_e, grid, block, 0, st, d, c);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_nonlocal_gd(const DispDev &d, hipStream_t st) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    if (nb <= 0 || !d.gd[0]) return hipSuccess;
+    hipLaunchKernelGGL(k_nonlocal_gd, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, d);
     return hipGetLastError();
 }
 

@@ -72,6 +72,7 @@ void cpml_profile(const fdtd_params &p, int Nc, int N, double d, double dt, Cpml
     const double n = (double)Nc;
     a.beN.assign(Nc + 1, 0); a.ceN.assign(Nc + 1, 0); a.beF.assign(Nc + 1, 0); a.ceF.assign(Nc + 1, 0);
     a.bhN.assign(Nc, 0); a.chN.assign(Nc, 0); a.bhF.assign(Nc, 0); a.chF.assign(Nc, 0);
+    if (Nc == 0) return;   // periodic axis: no CPML, 1/(kappa d) stays 1/d
     for (int i = 0; i <= Nc; i++) {
         a.ked[i] *= coef((double)(Nc - i) / n, (double)i / n, a.beN[i], a.ceN[i]);
         a.ked[i + a.F] *= coef((double)i / n, (double)(Nc - i) / n, a.beF[i], a.ceF[i]);
@@ -90,6 +91,32 @@ double source_pulse(const Setup &s, int t) {
     const float b = (float)(((double)t - s.delay) / s.width);
     const float b2 = b * b;
     return s.P.MAX_AMP * std::exp(-(double)b2 / 2.0) * std::sin(2 * kPI * s.f0 * t * s.dt);
+}
+
+std::vector<std::complex<double>> line_dft(const Setup &s, int steps) {
+    // the 1D line of k_line (UPDATE_h_inc, UPDATE_e_inc, Mur ends, additive
+    // source: main.cu:2280-2337, 2831-2859) in FP64, and the DFT of e_inc[m0]
+    // after every sampling step t (t % dft_stride == 0) with phase (t+1) dt
+    const int L = s.L, F = (int)s.freq.size();
+    std::vector<double> e(L, 0.0), en(L, 0.0), h(L, 0.0);
+    double mur[2] = {0.0, 0.0};
+    const double d1 = s.P.dx;
+    const double ch = kZ0 * s.dt / (kMU0 * d1), ce = s.dt / (kZ0 * kEP0 * d1);
+    const double fac = kC0 * s.dt, mc = (fac - d1) / (fac + d1);
+    std::vector<std::complex<double>> out(F, 0.0);
+    for (int t = 0; t < steps; t++) {
+        for (int i = 0; i < L - 1; i++) h[i] = h[i] - ch * (e[i + 1] - e[i]);
+        for (int i = 1; i < L - 1; i++) en[i] = e[i] - ce * (h[i] - h[i - 1]);
+        en[0] = mur[0] + mc * (en[1] - e[0]);
+        en[L - 1] = mur[1] + mc * (en[L - 2] - e[L - 1]);
+        mur[0] = en[1];
+        mur[1] = en[L - 2];
+        en[s.src] += source_pulse(s, t);
+        e.swap(en);
+        if (s.P.StaticField == 0 && t % s.P.dft_stride == 0)
+            for (int m = 0; m < F; m++) out[m] += e[s.m0] * std::exp(std::complex<double>(0, 2 * kPI * s.freq[m] * (t + 1.0) * s.dt));
+    }
+    return out;
 }
 
 void sphere_masks_at(const Setup &s, const double centre[3], double radius, std::vector<uint8_t> m[4]) {
@@ -159,7 +186,16 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
     s.P = p;
     if (!(p.dx > 0) || p.dx != p.dy || p.dx != p.dz) { err = "cubic cells required (dx == dy == dz > 0)"; return bad; }
     if (p.NCELLX < 4 || p.NCELLY < 4 || p.NCELLZ < 18) { err = "grid too small (NCELLZ must exceed 17 for the source cell)"; return bad; }
-    if (p.NcpmlX < 1 || p.NcpmlY < 1 || p.NcpmlZ < 1) { err = "Ncpml must be >= 1"; return bad; }
+    if (p.periodic_xy) {
+        if (p.periodic_xy != 1) { err = "periodic_xy must be 0 or 1"; return bad; }
+        if (p.NcpmlX != 0 || p.NcpmlY != 0) { err = "periodic_xy needs NcpmlX = NcpmlY = 0 (CPML along z only)"; return bad; }
+        if (p.inc_theta != 0.0) { err = "periodic_xy runs normal incidence only (inc_theta = 0)"; return FDTD_ERR_UNSUPPORTED; }
+        if (p.NONLOCAL != 0) { err = "periodic_xy runs the local models only (NONLOCAL 0)"; return FDTD_ERR_UNSUPPORTED; }
+        if (p.NcpmlZ < 1) { err = "Ncpml must be >= 1"; return bad; }
+    } else if (p.NcpmlX < 1 || p.NcpmlY < 1 || p.NcpmlZ < 1) {
+        err = "Ncpml must be >= 1";
+        return bad;
+    }
     if (p.NUM_freq < 1 || p.NUM_freq > 512) { err = "NUM_freq must be in [1, 512]"; return bad; }
     if (p.first_medium > 1) { err = "first_medium must be <= 1"; return bad; }
     if (p.dft_stride < 1) { err = "dft_stride must be >= 1"; return bad; }
@@ -174,8 +210,12 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         return bad;
     }
     if (!(p.BandWidth > 0)) { err = "BandWidth must be > 0"; return bad; }
-    if (p.NONLOCAL != 0 && p.NONLOCAL != 3) {
-        err = "NONLOCAL must be 0 (local Drude-CP) or 3 (hydrodynamic); the reference's other values are not built";
+    if (p.NONLOCAL != 0 && p.NONLOCAL != 1 && p.NONLOCAL != 3) {
+        err = "NONLOCAL must be 0 (local), 1 (grad-div nonlocal Drude pole) or 3 (hydrodynamic)";
+        return FDTD_ERR_UNSUPPORTED;
+    }
+    if (p.material == 7 && p.NONLOCAL == 3) {
+        err = "material 7 (Lorentz-Drude) has no hydrodynamic model (the reference defines N_EQ for 2 and 3 only)";
         return FDTD_ERR_UNSUPPORTED;
     }
     const int Nint[3] = {p.NCELLX, p.NCELLY, p.NCELLZ};
@@ -191,6 +231,26 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         s.abs_e[a] = s.N[a] - s.tf0[a] - p.abs_offset;
         s.sca_s[a] = s.tf0[a] - p.sca_offset;
         s.sca_e[a] = s.N[a] - s.tf0[a] + p.sca_offset;
+        if (p.periodic_xy && a < 2) {
+            // setup.cu:401-405, 429: x and y keep NCELL cells (no +1), the TFSF box and
+            // the monitors span them
+            s.N[a] = Nint[a];
+            s.tf0[a] = 0;
+            s.tf1[a] = s.N[a];
+            s.abs_s[a] = s.sca_s[a] = 0;
+            s.abs_e[a] = s.sca_e[a] = s.N[a];
+            continue;
+        }
+        if (p.periodic_xy) {
+            // reflection plane NtfsfZ - sca_offset, transmission plane NZ - NtfsfZ - sca_offset
+            // (Spectral.cu:11-15), the z faces of the scattering box
+            s.sca_e[a] = s.N[a] - s.tf0[a] - p.sca_offset + 1;
+            if (s.sca_s[a] <= Nc[a] || s.sca_e[a] - 1 >= s.N[a] - 1 - Nc[a] || s.sca_e[a] - 1 <= s.tf0[a] + 1) {
+                err = "periodic reflection/transmission planes do not fit between the z CPML slabs";
+                return bad;
+            }
+            continue;
+        }
         if (s.sca_s[a] < 1 || s.sca_e[a] > s.N[a] - 1 || s.abs_e[a] - s.abs_s[a] < 2 || s.tf0[a] <= Nc[a]) {
             err = "TFSF or monitor boxes do not fit the grid";
             return bad;
@@ -208,7 +268,7 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
     }
     // SETUP_Drude_CP (setup.cu:1164-1373)
     s.hydro = (p.NONLOCAL == 3);
-    double wD, gD, A[2], ph[2], Om[2], Ga[2];
+    double wD = 0, gD = 0, A[2] = {0, 0}, ph[2] = {0, 0}, Om[2] = {0, 0}, Ga[2] = {0, 0};
     if (p.material == 2) {
         s.ade.eps_inf = 1.4447; wD = 1.3280e16; gD = 9.1269e13;
         A[0] = -1.5951; ph[0] = 3.1288; Om[0] = 8.2749e15; Ga[0] = 5.1770e15;
@@ -220,15 +280,43 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         A[1] = 3.0834; ph[1] = -1.0968; Om[1] = 4.1684e15; Ga[1] = 2.3555e15;
         s.ade.N_EQ = 5.476415562682574e28;
     } else if (p.material == 7) {
-        err = "material 7 (Lorentz-Drude) is not built";
-        return FDTD_ERR_UNSUPPORTED;
+        // Lorentz-Drude (setup.cu:1256-1275), coefficients below
     } else {
-        err = "material must be 2 (silver) or 3 (gold)";
+        err = "material must be 2 (silver), 3 (gold) or 7 (Lorentz-Drude)";
         return bad;
     }
     const double dt = s.dt;
+    s.nonlocal1 = p.NONLOCAL == 1;
     double s3 = 0, s4 = 0, s5 = 0;
-    for (int n = 0; n < 2; n++) {
+    if (p.material == 7) {
+        // setup.cu:1384-1402: Lorentz poles P^{n+1} = alpha P^n + psi P^{n-1} + eta <E> and
+        // the Drude pole with alpha_HD1, psi_HD, eta_HD; the E update's C1_NL = eps_inf eps0 / dt
+        // and C2_NL = sum eta / (4 dt) average <E> = (E^{n+1} + 2 E^n + E^{n-1}) / 4, so every
+        // pole takes the five-coefficient form k1 = alpha, k2 = psi, k3 = k5 = eta/4, k4 = eta/2
+        const double w = 2.0 * kPI / kPLANKS;
+        s.ade.eps_inf = 3.559;
+        const double deps[2] = {2.912, 1.272}, delta[2] = {w * 1.541, w * 0.525}, omg[2] = {w * 4.693, w * 3.112};
+        const double wd = w * 8.812, gd = w * 0.0752;
+        for (int n = 0; n < 2; n++) {
+            const double den = 1.0 + delta[n] * dt;
+            const double eta = kEP0 * deps[n] * omg[n] * omg[n] * dt * dt / den;
+            double *c = s.ade.c[n];
+            c[0] = (2.0 - omg[n] * omg[n] * dt * dt) / den;
+            c[1] = -1.0 * (1.0 - delta[n] * dt) / den;
+            c[2] = eta / 4.0; c[3] = eta / 2.0; c[4] = eta / 4.0;
+        }
+        const double eta_d = 2.0 * kEP0 * wd * wd * dt * dt / (2.0 + gd * dt);
+        s.ade.d[0] = 4.0 / (2.0 + gd * dt);
+        s.ade.d[1] = -1.0 * (2.0 - gd * dt) / (2.0 + gd * dt);
+        s.ade.d[2] = eta_d / 4.0; s.ade.d[3] = eta_d / 2.0; s.ade.d[4] = eta_d / 4.0;
+        // alpha_HD2 = 2 dt^2 beta^2 / (2 + gamma dt) (setup.cu:1396)
+        s.ade.dNL = s.nonlocal1 ? 2.0 * dt * dt * p.hydro_beta * p.hydro_beta / (2.0 + gd * dt) : 0.0;
+        s.ade.N_EQ = 0.0;
+        s3 = s.ade.d[2] + s.ade.c[0][2] + s.ade.c[1][2];
+        s4 = s.ade.d[3] + s.ade.c[0][3] + s.ade.c[1][3];
+        s5 = s.ade.d[4] + s.ade.c[0][4] + s.ade.c[1][4];
+    }
+    for (int n = 0; n < 2 && p.material != 7; n++) {
         // second-order CP pole, central differences at n with E averaged over n+1, n, n-1
         const double a0 = 2.0 * kEP0 * A[n] * Om[n] * (Om[n] * std::cos(ph[n]) - Ga[n] * std::sin(ph[n]));
         const double a1 = -2.0 * kEP0 * A[n] * Om[n] * std::sin(ph[n]);
@@ -242,17 +330,22 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
         c[4] = (a0 / 4.0 - a1 / (2.0 * dt)) / C;
     }
     const double dd = 1.0 / (dt * dt) + gD / (2.0 * dt);
-    s.ade.d[0] = 2.0 / (dd * dt * dt);
-    s.ade.d[1] = (gD / (2.0 * dt) - 1.0 / (dt * dt)) / dd;
-    if (!s.hydro) {
+    if (p.material == 7) {
+        // coefficients set above
+    } else if (!s.hydro) {
+        s.ade.d[0] = 2.0 / (dd * dt * dt);
+        s.ade.d[1] = (gD / (2.0 * dt) - 1.0 / (dt * dt)) / dd;
         s.ade.d[2] = s.ade.d[4] = kEP0 * wD * wD / (4.0 * dd);
         s.ade.d[3] = kEP0 * wD * wD / (2.0 * dd);
-        s.ade.dNL = 0.0;
+        // NONLOCAL == 1: d_NL = NONLOC^2 / d_d, NONLOC = v_F sqrt(3/4) (setup.cu:1179, 1305)
+        s.ade.dNL = s.nonlocal1 ? p.hydro_beta * p.hydro_beta / dd : 0.0;
         // summed Drude first, then the CP poles, as the reference does (setup.cu:1306-1332)
         s3 = s.ade.d[2] + s.ade.c[0][2] + s.ade.c[1][2];
         s4 = s.ade.d[3] + s.ade.c[0][3] + s.ade.c[1][3];
         s5 = s.ade.d[4] + s.ade.c[0][4] + s.ade.c[1][4];
     } else {
+        s.ade.d[0] = 2.0 / (dd * dt * dt);
+        s.ade.d[1] = (gD / (2.0 * dt) - 1.0 / (dt * dt)) / dd;
         s.ade.d[2] = -1.0 * kE0 / kME / dd / dt;
         s.ade.d[3] = 1.0 * kE0 / kME / dd / dt;
         s.ade.d[4] = -1.0 / dd;

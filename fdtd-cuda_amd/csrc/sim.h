@@ -37,6 +37,7 @@ struct fdtd_sim {
     uint8_t *disp_id[3] = {};
     float *disp_st[3][8] = {};
     float *h_vnew[3] = {}, *h_nd[3] = {}, *h_ndp[3] = {};
+    float *nl_gd[3] = {};           // NONLOCAL == 1: grad(div P_d) per box cell (scratch of each step)
     int n_disp[3] = {0, 0, 0};
     // DFT
     FaceDev fd{};

@@ -22,6 +22,7 @@ EXPORTS = [
     "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod", "fdtd_set_point_monitor", "fdtd_point_spectrum",
     "fdtd_batch_create", "fdtd_batch_run", "fdtd_batch_tile", "fdtd_batch_last_error", "fdtd_batch_destroy",
     "fdtd_checkpoint_size", "fdtd_checkpoint_save", "fdtd_checkpoint_load",
+    "fdtd_reflectance", "fdtd_add_layer",
 ]
 
 BATCH_NO_GRAPH = 1
@@ -76,6 +77,8 @@ def load_library(path=LIB_PATH):
         "fdtd_checkpoint_size": (ip, [vp, C.POINTER(C.c_size_t)]),
         "fdtd_checkpoint_save": (ip, [vp, vp, C.c_size_t]),
         "fdtd_checkpoint_load": (ip, [vp, vp, C.c_size_t]),
+        "fdtd_reflectance": (ip, [vp, dp, dp, dp, dp]),
+        "fdtd_add_layer": (ip, [vp, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -257,6 +260,18 @@ class Simulation:
         sca, ab, ext, raw = np.empty(F), np.empty(F), np.empty(F), np.empty(12 * F)
         self._check(self.lib.fdtd_flux(self.h, *[_ptr(a, C.c_double) for a in (sca, ab, ext, raw)]))
         return {"sca": sca, "abs": ab, "ext": ext, "raw": raw.reshape(12, F)}
+
+    def add_layer(self, k0, k1):
+        """Metal on every E component at k in [k0, k1) over the whole x-y extent (periodic runs)."""
+        self._check(self.lib.fdtd_add_layer(self.h, int(k0), int(k1)))
+        self.info = self.get_info()
+
+    def reflectance(self):
+        """Periodic spectra: R, T (physically normalised) and the reference's columns ref_r, ref_t."""
+        F = self.info.num_freq
+        out = [np.empty(F) for _ in range(4)]
+        self._check(self.lib.fdtd_reflectance(self.h, *[_ptr(a, C.c_double) for a in out]))
+        return dict(zip(("R", "T", "ref_r", "ref_t"), out))
 
     def checkpoint(self):
         """The whole simulation state as a bytes-like numpy array (fdtd_checkpoint_save)."""

@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("hydro_beta", C.c_double),
         ("xs_area", C.c_double),
         ("dft_exact", C.c_int),
+        ("periodic_xy", C.c_int),
     ]
 
     def as_dict(self):

@@ -60,12 +60,15 @@ typedef struct fdtd_params {
     int    NcpmlX, NcpmlY, NcpmlZ;   /* CPML thickness in cells                                */
     int    nano_sphere;        /* 0: none, 1: staircased sphere at the grid centre (setup.cu:219) */
     double nano_sphere_radius; /* metres                                                       */
-    int    material;           /* 2: silver, 3: gold Drude + 2 CP (setup.cu:1164-1211)         */
+    int    material;           /* 2: silver, 3: gold Drude + 2 CP (setup.cu:1164-1211), 7: Lorentz-Drude
+                                  (two Lorentz poles + Drude, setup.cu:1256-1275, 1384-1402)  */
     int    first_medium;       /* background material id; must be <= 1                        */
     int    NUM_freq;           /* DFT frequencies                                              */
     int    WL_or_freq;         /* 1: linear in wavelength, 0: linear in frequency (main.cu:2448) */
     double BandWidth;          /* pulse bandwidth (Hz) (tfsf2.cu:37)                           */
-    int    NONLOCAL;           /* 3: hydrodynamic nonlocal model (setup.cu:995)                */
+    int    NONLOCAL;           /* 1: grad-div term on the Drude pole with velocity hydro_beta
+                                  (updatefields.cu:876-891, 1424-1604), 3: hydrodynamic nonlocal
+                                  model (setup.cu:995)                                         */
     double MAX_AMP;            /* source amplitude (main.cu:2848)                              */
     double TimeFactor;         /* dt = 0.5*dy/c0/TimeFactor (setup.cu:359)                     */
     double lam_0;              /* pulse centre wavelength (m); f_0 = c0/lam_0 (setup.cu:62)    */
@@ -83,9 +86,16 @@ typedef struct fdtd_params {
     int    dft_stride;         /* sample every dft_stride steps, 4 (main.cu:2920)               */
     int    dft_block;          /* K: samples buffered per MFMA contraction                      */
     int    source_offset;      /* source cell m0 - source_offset, 50 (main.cu:2851)             */
-    double hydro_beta;         /* nonlocal velocity beta (m/s), 1.39e6*sqrt(3/4) (setup.cu:980-983) */
+    double hydro_beta;         /* nonlocal velocity beta (m/s) of NONLOCAL 1 and 3,
+                                  1.39e6*sqrt(3/4) (setup.cu:980-983)                          */
     double xs_area;            /* cross-section normaliser (m^2); 0 -> pi*r^2 (Spectral.cu:1166) */
     int    dft_exact;          /* 0: split-BF16 MFMA contraction (default), 1: FP64 contraction */
+    int    periodic_xy;        /* 1: x and y periodic (Periodic_XY, main.cu:781-820 and the periodic
+                                  branches of every update): NcpmlX = NcpmlY = 0, grid NCELLX by
+                                  NCELLY with no +1 (setup.cu:401-405), normal incidence only, the
+                                  plane wave enters through the near z plane of the TFSF box alone
+                                  (main.cu:698), spectra from the reflection and transmission planes
+                                  (fdtd_reflectance)                                              */
 } fdtd_params;
 
 /* Derived grid facts (read-only). */
@@ -138,7 +148,8 @@ int  fdtd_get_info(const fdtd_sim *sim, fdtd_grid_info *info);
 
 /* Replaces the per-component material masks mat_matrix{X,Y,Z}
  * (setup.cu:135-342).  Each array is NX*NY*NZ ids; an E component takes the
- * dispersive branch where 1 < id < 6 (main.cu:1175).  Dispersive cells must
+ * dispersive branch where 1 < id < 6 (main.cu:1175) or id == 7 (the
+ * Lorentz-Drude material, which the reference's test leaves out).  Dispersive cells must
  * lie strictly inside the absorption box. */
 int  fdtd_set_materials(fdtd_sim *sim, const uint8_t *id_x, const uint8_t *id_y,
                         const uint8_t *id_z);
@@ -251,6 +262,26 @@ int  fdtd_get_dft(fdtd_sim *sim, int face, int comp, double *out);
  * xs_area) and the 12 raw face fluxes raw[face*NUM_freq + m].  Any pointer may
  * be NULL. */
 int  fdtd_flux(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *raw);
+
+/* Reflectance and transmittance of a periodic (periodic_xy) simulation
+ * (Reflectance_XZ, main.cu:3195-3330, and the plane DFTs of Fourier_Transform,
+ * Spectral.cu:8-130).  The reflection plane is k_r = Ntfsf_z - sca_offset on
+ * the scattered-field side of the source plane, the transmission plane k_t =
+ * NZ - Ntfsf_z - sca_offset; their tangential E and H are the z faces of the
+ * DFT monitors (fdtd_get_dft faces 4 and 5).  Per frequency: R = reflected
+ * and T = transmitted power through the planes over the incident power, the
+ * incident power taken from the DFT of the 1D line at the source plane with
+ * the monitors' stride and phase (physical normalisation); ref_r and ref_t
+ * are the reference's printed columns Reflectivity/Amplitude and
+ * Transmit/Amplitude: sum Re(Ey Hx* - Ex Hy*) dA / Area over each plane
+ * divided by |E_incident|^2 of the source spectrum (main.cu:3026-3034).  Any
+ * pointer may be NULL. */
+int  fdtd_reflectance(fdtd_sim *sim, double *R, double *T, double *ref_r, double *ref_t);
+/* Metal layer for periodic simulations (the inf_disp_slab masks,
+ * setup.cu:156-200): every E component at k in [k0, k1) over the whole x-y
+ * extent takes the configured material.  The layer must lie between the
+ * source plane and the transmission plane (k0 > Ntfsf_z, k1 <= k_t). */
+int  fdtd_add_layer(fdtd_sim *sim, int k0, int k1);
 
 /* Device time of the field update of fdtd_step accumulated since the last
  * reset (HIP events on the solver stream; no reference counterpart).  Returns

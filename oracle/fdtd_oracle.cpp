@@ -20,8 +20,17 @@
 //   flux      Spectral.cu:411-1177 (CalculateAbsScatt)
 //
 // Decisions where the reference is ambiguous or racy (the Q-list):
-//   * Periodic branches are dead in the reference (Periodic_* = 0,
-//     main.cu:2398-2400) and are not restated.
+//   * Periodic branches (Periodic_XY) are dead in the reference as shipped
+//     (main.cu:2398-2400); they are restated for the real-valued build
+//     (PBC_CTW = 0) with normal incidence: x and y wrap around (the j ==
+//     NCELLY-1 / i == NCELLX-1 branches of main.cu:781-1044 and 1067-1912),
+//     z-CPML only, the near z plane of CORRECT_Z alone (main.cu:698), and the
+//     reflection/transmission planes of Fourier_Transform (Spectral.cu:8-130)
+//     with Reflectance_XZ's sums (main.cu:3195-3330).  Decisions: metal tests
+//     use the per-component masks as the non-periodic branch does (the
+//     periodic ex branch reads mat_matrix, main.cu:1086); the planes are
+//     sampled with the monitors' stride and phase; Hy of the planes is hy
+//     (Fourier_Transform accumulates hx into Hy_Reflected, Spectral.cu:66).
 //   * TFSF trig is evaluated in double; the reference casts angles to float
 //     (main.cu:387), which differs by <= 1e-7 relative.
 //   * The DFT accumulates in FP64 (the reference uses float accumulators,
@@ -39,6 +48,7 @@
 //   * Hydro continuity for ND_z keeps the reference's 0.5*(Vz1 - Vz2) term
 //     without the Nz factors (main.cu:1837); ex/ey keep 0.5*(N*V) (main.cu:1239, 1554).
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <map>
@@ -49,6 +59,8 @@
 #include <string>
 #include <vector>
 #include <omp.h>
+#include <pmmintrin.h>
+#include <xmmintrin.h>
 
 #include "../include/fdtd.h"
 
@@ -62,6 +74,7 @@ const double C0 = 2.99792458E+08;
 const double Z0 = 376.730313462;
 const double E0 = 1.60217662E-19;
 const double ME = 9.10938356E-31;
+const double PLANKS = 4.13566553853599E-15;   // eV s (define.h:27)
 
 typedef std::vector<double> vec;
 
@@ -101,6 +114,10 @@ struct Oracle {
     vec Pcp[3], Pcp_n[3], Pcp_n_1[3];          // 2 poles interleaved, pole index innermost
     vec Pd[3], Pd_n[3], Pd_n_1[3], Pd_n_2[3];  // 1 Drude pole
     vec ND[3], ND_prev[3];                      // hydro charge
+    int nonlocal1 = 0;                          // NONLOCAL == 1: grad-div term on the Drude pole
+    bool periodic = false;                      // Periodic_XY
+    std::vector<std::complex<double>> inc_dft;  // DFT of e_inc[m0] on the sampling steps (monitor phase)
+    vec GD[3];                                  // its grad(div P_d^n) per E component
     vec e_inc, h_inc;
     double e1, e2;
     vec freq;
@@ -123,6 +140,11 @@ struct Oracle {
 
 thread_local std::string g_err;
 
+// Dispersive E-component cells: the reference's test 1 < id < 6 (main.cu:1175;
+// Q11), plus id 7 so the Lorentz-Drude material (setup.cu:1256-1275), whose
+// masks the reference's kernels never treat as dispersive, takes the ADE branch.
+inline bool disp_id(int v) { return (v > 1 && v < 6) || v == 7; }
+
 int fail(const std::string &m) { g_err = m; return FDTD_ERR_INVALID; }
 
 // ---------------------------------------------------------------- setup
@@ -135,6 +157,7 @@ void setup_cpml_axis(int Nc, double d, double dtv, const fdtd_params &P, int F_i
     const double smax = P.sigma_factor * (m + 1) / (Z0 * d);
     be_N.assign(Nc + 1, 0); ce_N.assign(Nc + 1, 0); be_F.assign(Nc + 1, 0); ce_F.assign(Nc + 1, 0);
     bh_N.assign(Nc, 0); ch_N.assign(Nc, 0); bh_F.assign(Nc, 0); ch_F.assign(Nc, 0);
+    if (Nc == 0) return;   // periodic axis: no CPML, 1/(kappa d) stays 1/d
     for (int i = 0; i < Nc + 1; i++) {
         double S = pow((double)(Nc - i) / (double)Nc, m) * smax;
         double A = amax * pow((double)i / (double)Nc, 1.0);
@@ -184,10 +207,18 @@ int setup(Oracle &o, const fdtd_params &p) {
     o.P = p;
     if (p.dx != p.dy || p.dx != p.dz) return fail("oracle requires cubic cells (dx == dy == dz)");
     if (p.NCELLX < 4 || p.NCELLY < 4 || p.NCELLZ < 18) return fail("grid too small (NCELLZ > 17 needed for the source cell)");
-    if (p.NcpmlX < 1 || p.NcpmlY < 1 || p.NcpmlZ < 1) return fail("Ncpml must be >= 1");
+    if (p.NcpmlZ < 1) return fail("Ncpml must be >= 1");
     if (p.NUM_freq < 1) return fail("NUM_freq must be >= 1");
     if (p.first_medium > 1) return fail("first_medium must be <= 1");
     o.dx = p.dx; o.dy = p.dy; o.dz = p.dz;
+    o.periodic = p.periodic_xy == 1;
+    if (o.periodic) {
+        if (p.NcpmlX != 0 || p.NcpmlY != 0) return fail("periodic_xy needs NcpmlX = NcpmlY = 0");
+        if (p.inc_theta != 0.0) return fail("periodic_xy runs normal incidence only (inc_theta = 0)");
+        if (p.NONLOCAL != 0) return fail("periodic_xy runs the local models only");
+    } else if (p.NcpmlX < 1 || p.NcpmlY < 1) {
+        return fail("Ncpml must be >= 1");
+    }
     // SETUP_CONST, setup.cu:347-510
     o.inc_Length = p.NCELLZ * 6;
     o.dt = 0.5 * p.dy / C0 / p.TimeFactor;
@@ -199,11 +230,29 @@ int setup(Oracle &o, const fdtd_params &p) {
     o.NX = nx + 1; o.NY = ny + 1; o.NZ = nz + 1;
     o.cpml_x_lim = o.NX - 1; o.cpml_y_lim = o.NY - 1; o.cpml_z_lim = o.NZ - 1;
     o.NtfsfX = p.tfsf_inset + o.NcX; o.NtfsfY = p.tfsf_inset + o.NcY; o.NtfsfZ = p.tfsf_inset + o.NcZ;
+    if (o.periodic) {
+        // setup.cu:401-405, 429: no +1 in x and y, psi on every column, TFSF spans x and y
+        o.NX = nx; o.NY = ny;
+        o.cpml_x_lim = o.NX; o.cpml_y_lim = o.NY;
+        o.NtfsfX = o.NtfsfY = 0;
+    }
     int Nt[3] = {o.NtfsfX, o.NtfsfY, o.NtfsfZ}, Nn[3] = {o.NX, o.NY, o.NZ};
     for (int a = 0; a < 3; a++) {
         o.abs_s[a] = Nt[a] + p.abs_offset; o.abs_e[a] = Nn[a] - Nt[a] - p.abs_offset;
         o.sca_s[a] = Nt[a] - p.sca_offset; o.sca_e[a] = Nn[a] - Nt[a] + p.sca_offset;
+        if (o.periodic && a < 2) {
+            o.abs_s[a] = o.sca_s[a] = 0;
+            o.abs_e[a] = o.sca_e[a] = Nn[a];
+            continue;
+        }
         if (o.sca_s[a] < 1 || o.abs_e[a] - o.abs_s[a] < 2) return fail("monitor boxes do not fit");
+    }
+    if (o.periodic) {
+        // reflection plane k = NtfsfZ - sca_offset, transmission plane NZ - NtfsfZ - sca_offset
+        // (Spectral.cu:11-15 with 5 = sca_offset): the z faces of the scattering box
+        o.sca_e[2] = o.NZ - o.NtfsfZ - p.sca_offset + 1;
+        if (o.sca_s[2] < o.NcZ + 1 || o.sca_e[2] - 1 >= o.NZ - o.NcZ - 1 || o.sca_e[2] - 1 <= o.NtfsfZ)
+            return fail("periodic reflection/transmission planes do not fit between the z CPML slabs");
     }
     size_t N = (size_t)o.NX * o.NY * o.NZ;
     // MATERIAL_MATRIX, setup.cu:135-248 (masks use the augmented size)
@@ -231,7 +280,7 @@ int setup(Oracle &o, const fdtd_params &p) {
     }
     // SETUP_Drude_CP, setup.cu:979-1378
     o.hydro = (p.NONLOCAL == 3);
-    double w_D, gamma_d, A_cp[2], phi_cp[2], OM[2], GA[2];
+    double w_D = 0, gamma_d = 0, A_cp[2] = {0, 0}, phi_cp[2] = {0, 0}, OM[2] = {0, 0}, GA[2] = {0, 0};
     if (p.material == 2) {
         o.eps_inf = 1.4447; w_D = 1.3280e16; gamma_d = 9.1269e13;
         A_cp[0] = -1.5951; phi_cp[0] = 3.1288; OM[0] = 8.2749e15; GA[0] = 5.1770e15;
@@ -242,10 +291,45 @@ int setup(Oracle &o, const fdtd_params &p) {
         A_cp[0] = 0.26698; phi_cp[0] = -1.2371; OM[0] = 3.8711e15; GA[0] = 4.4642e14;
         A_cp[1] = 3.0834; phi_cp[1] = -1.0968; OM[1] = 4.1684e15; GA[1] = 2.3555e15;
         o.N_EQ = 5.476415562682574e28;
+    } else if (p.material == 7) {
+        // Lorentz-Drude (setup.cu:1256-1275): two Lorentz poles, one Drude
+        // pole, parameters in eV converted with 2 pi / h
+        if (o.hydro) return fail("material 7 has no hydrodynamic model");
     } else {
-        return fail("material must be 2 (silver) or 3 (gold)");
+        return fail("material must be 2 (silver), 3 (gold) or 7 (Lorentz-Drude)");
     }
     double dt = o.dt;
+    o.nonlocal1 = (p.NONLOCAL == 1);
+    if (p.NONLOCAL != 0 && p.NONLOCAL != 1 && p.NONLOCAL != 3) return fail("NONLOCAL must be 0, 1 or 3");
+    if (p.material == 7) {
+        // setup.cu:1384-1402: Lorentz P^{n+1} = alpha P^n + psi P^{n-1} + eta <E>, Drude likewise
+        // with alpha_HD1, psi_HD, eta_HD, and the E update of C1_NL = eps_inf eps0 / dt,
+        // C2_NL = sum eta / (4 dt): <E> = (E^{n+1} + 2 E^n + E^{n-1}) / 4, so each pole takes
+        // the reference's five-coefficient form k1 = alpha, k2 = psi, k3 = k5 = eta/4, k4 = eta/2
+        const double w = 2.0 * PI / PLANKS;
+        o.eps_inf = 3.559;
+        const double deps[2] = {2.912, 1.272}, delta[2] = {w * 1.541, w * 0.525}, omg[2] = {w * 4.693, w * 3.112};
+        const double wd = w * 8.812, gd = w * 0.0752;
+        for (int n = 0; n < 2; n++) {
+            const double den = 1.0 + delta[n] * dt;
+            const double eta = EP0 * deps[n] * omg[n] * omg[n] * dt * dt / den;
+            o.c1[n] = (2.0 - omg[n] * omg[n] * dt * dt) / den;
+            o.c2[n] = -1.0 * (1.0 - delta[n] * dt) / den;
+            o.c3[n] = eta / 4.0; o.c4[n] = eta / 2.0; o.c5[n] = eta / 4.0;
+        }
+        const double eta_d = 2.0 * EP0 * wd * wd * dt * dt / (2.0 + gd * dt);
+        o.d1 = 4.0 / (2.0 + gd * dt);
+        o.d2 = -1.0 * (2.0 - gd * dt) / (2.0 + gd * dt);
+        o.d3 = eta_d / 4.0; o.d4 = eta_d / 2.0; o.d5 = eta_d / 4.0;
+        // alpha_HD2 = 2 dt^2 beta^2 / (2 + gamma dt) (setup.cu:1396)
+        o.dNL = o.nonlocal1 ? 2.0 * dt * dt * p.hydro_beta * p.hydro_beta / (2.0 + gd * dt) : 0.0;
+        o.N_EQ = 0.0;
+        const double s3 = o.d3 + o.c3[0] + o.c3[1], s4 = o.d4 + o.c4[0] + o.c4[1], s5 = o.d5 + o.c5[0] + o.c5[1];
+        o.C_E = EP0 * o.eps_inf + s3;
+        o.C_E_1 = -s4 + EP0 * o.eps_inf;
+        o.C_E_2 = s5;
+    }
+    else {
     double sum3 = 0, sum4 = 0, sum5 = 0;
     for (int i = 0; i < 2; i++) {
         double a0 = 2.0 * EP0 * A_cp[i] * OM[i] * (OM[i] * cos(phi_cp[i]) - GA[i] * sin(phi_cp[i]));
@@ -264,7 +348,8 @@ int setup(Oracle &o, const fdtd_params &p) {
     if (!o.hydro) {
         o.d3 = o.d5 = EP0 * w_D * w_D / (4.0 * dd);
         o.d4 = EP0 * w_D * w_D / (2.0 * dd);
-        o.dNL = 0.0;  // NONLOCAL != 1 in every configuration restated here
+        // NONLOCAL == 1: d_NL = NONLOC^2 / d_d with NONLOC = v_F sqrt(3/4) (setup.cu:1179, 1305)
+        o.dNL = o.nonlocal1 ? p.hydro_beta * p.hydro_beta / dd : 0.0;
         sum3 = o.d3 + o.c3[0] + o.c3[1];
         sum4 = o.d4 + o.c4[0] + o.c4[1];
         sum5 = o.d5 + o.c5[0] + o.c5[1];
@@ -280,6 +365,7 @@ int setup(Oracle &o, const fdtd_params &p) {
     o.C_E = EP0 * o.eps_inf + sum3;
     o.C_E_1 = -sum4 + EP0 * o.eps_inf;
     o.C_E_2 = sum5;
+    }
     // SETUP_TFSF, tfsf2.cu:6-56
     o.inc_phi = p.inc_phi * PI / 180.0; o.inc_theta = p.inc_theta * PI / 180.0; o.polar_psi = p.polar_psi * PI / 180.0;
     o.d_1D = p.dx;
@@ -300,6 +386,7 @@ int setup(Oracle &o, const fdtd_params &p) {
         o.Pcp[c].assign(2 * N, 0); o.Pcp_n[c].assign(2 * N, 0); o.Pcp_n_1[c].assign(2 * N, 0);
         o.Pd[c].assign(N, 0); o.Pd_n[c].assign(N, 0); o.Pd_n_1[c].assign(N, 0); o.Pd_n_2[c].assign(N, 0);
         if (o.hydro) { o.ND[c].assign(N, 0); o.ND_prev[c].assign(N, 0); }
+        if (o.nonlocal1) o.GD[c].assign(N, 0);
     }
     o.e_inc.assign(o.inc_Length, 0); o.h_inc.assign(o.inc_Length, 0);
     // SETUP_CPML, setup.cu:513-977
@@ -334,6 +421,7 @@ int setup(Oracle &o, const fdtd_params &p) {
         (void)a; (void)b;
         o.face_na[q] = e[outer] - s[outer];
         o.face_nb[q] = e[inner] - s[inner];
+        if (o.periodic && (box == 1 || axis != 2)) o.face_na[q] = o.face_nb[q] = 0;
         for (int c = 0; c < 4; c++) o.dft[q * 4 + c].assign((size_t)F * o.face_na[q] * o.face_nb[q] * 2, 0.0);
     }
     o.t = 0;
@@ -528,11 +616,124 @@ void correct(Oracle &o) {
 
 // ey uses Cexh in the reference (main.cu:546); media are uniform so Cexh == Ceyh.
 
+// ---------------------------------------------------------------- periodic x, y (Periodic_XY)
+
+// UPDATE_hx/hy/hz periodic branches (main.cu:781-865, 877-948, 962-1044):
+// every column, the x and y neighbours of the last cells wrap to 0 (real
+// build: no Bloch phase), z-CPML only.  hx, hy on k < NZ-1, hz on all k.
+void update_h_periodic(Oracle &o) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ, NcZ = o.NcZ;
+    const double dz = o.dz, Chh = o.Chh, Che = o.Che;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) {
+            const int ip = (i + 1) % NX, jp = (j + 1) % NY;
+            for (int k = 0; k < NZ; k++) {
+                const size_t id = o.I3(i, j, k);
+                if (k < NZ - 1) {
+                    double dEy = o.ey[o.I3(i, j, k + 1)] - o.ey[id], dEz = o.ez[o.I3(i, jp, k)] - o.ez[id];
+                    o.hx[id] = Chh * o.hx[id] + Che * (dEy / o.khdz[k] - dEz / o.khdy[j]);
+                    double dEx = o.ex[o.I3(i, j, k + 1)] - o.ex[id];
+                    dEz = o.ez[o.I3(ip, j, k)] - o.ez[id];
+                    o.hy[id] = Chh * o.hy[id] + Che * (dEz / o.khdx[i] - dEx / o.khdz[k]);
+                    if (k < o.cpml_N_Z) {
+                        const size_t q = (size_t)k + NcZ * ((size_t)j + (size_t)NY * i);
+                        o.psi_Hx_z_N[q] = o.bh_z_N[k] * o.psi_Hx_z_N[q] + o.ch_z_N[k] * dEy / dz;
+                        o.hx[id] += Che * o.psi_Hx_z_N[q];
+                        o.psi_Hy_z_N[q] = o.bh_z_N[k] * o.psi_Hy_z_N[q] + o.ch_z_N[k] * dEx / dz;
+                        o.hy[id] -= Che * o.psi_Hy_z_N[q];
+                    }
+                    if (k >= o.cpml_F_Z) {
+                        const int k2 = k - o.cpml_F_Z;
+                        const size_t q = (size_t)k2 + NcZ * ((size_t)j + (size_t)NY * i);
+                        o.psi_Hx_z_F[q] = o.bh_z_F[k2] * o.psi_Hx_z_F[q] + o.ch_z_F[k2] * dEy / dz;
+                        o.hx[id] += Che * o.psi_Hx_z_F[q];
+                        o.psi_Hy_z_F[q] = o.bh_z_F[k2] * o.psi_Hy_z_F[q] + o.ch_z_F[k2] * dEx / dz;
+                        o.hy[id] -= Che * o.psi_Hy_z_F[q];
+                    }
+                }
+                const double dEx = o.ex[o.I3(i, jp, k)] - o.ex[id], dEy = o.ey[o.I3(ip, j, k)] - o.ey[id];
+                o.hz[id] = Chh * o.hz[id] + Che * (dEx / o.khdy[j] - dEy / o.khdx[i]);
+            }
+        }
+}
+
+// CORRECT_Z near plane only (main.cu:638-697; the far plane is skipped when
+// Periodic_XY, main.cu:698), over every column; normal incidence.
+void correct_periodic(Oracle &o) {
+    const IncGeom g = inc_geom(o);
+    const int NX = o.NX, NY = o.NY, k = o.NtfsfZ;
+    const double i0 = o.i_0, j0 = o.j_0, k0 = o.k_0, dz = o.dz, Che = o.Che, Ceh = o.Ceh;
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) {
+            const double ey_inc = e_interp(o, g, i - i0, j + 0.5 - j0, k - k0) * g.ey_f;
+            o.hx[o.I3(i, j, k - 1)] -= Che * ey_inc / dz;
+            const double ex_inc = e_interp(o, g, i + 0.5 - i0, j - j0, k - k0) * g.ex_f;
+            o.hy[o.I3(i, j, k - 1)] += Che * ex_inc / dz;
+            const double hx_inc = h_interp(o, g, i - i0, j + 0.5 - j0, k - 0.5 - k0) * g.hx_f;
+            o.ey[o.I3(i, j, k)] -= Ceh * hx_inc / dz;
+            const double hy_inc = h_interp(o, g, i + 0.5 - i0, j - j0, k - 0.5 - k0) * g.hy_f;
+            o.ex[o.I3(i, j, k)] += Ceh * hy_inc / dz;
+        }
+}
+
+inline double ade_update(Oracle &o, int c, size_t id, double curl);
+
+// UPDATE_ex/ey/ez periodic branches (main.cu:1067-1170, 1359-1475,
+// 1663-1766): ex, ey on 0 < k < NZ-1, ez on k < NZ-1, every column, the
+// lower x and y neighbours of the first cells wrap to the last; z-CPML only;
+// dispersive cells by their component mask.
+void update_e_periodic(Oracle &o) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ, NcZ = o.NcZ;
+    const double dz = o.dz, Cee = o.Cee, Ceh = o.Ceh;
+    for (int c = 0; c < 3; c++) {
+        const std::vector<int> &mask = c == 0 ? o.matX : (c == 1 ? o.matY : o.matZ);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < NX; i++)
+            for (int j = 0; j < NY; j++) {
+                const int im = (i + NX - 1) % NX, jm = (j + NY - 1) % NY;
+                for (int k = (c == 2 ? 0 : 1); k < NZ - 1; k++) {
+                    const size_t id = o.I3(i, j, k);
+                    double curl, dHz_ = 0;
+                    if (c == 0) {
+                        dHz_ = o.hy[id] - o.hy[o.I3(i, j, k - 1)];
+                        curl = (o.hz[id] - o.hz[o.I3(i, jm, k)]) / o.kedy[j] - dHz_ / o.kedz[k];
+                    } else if (c == 1) {
+                        dHz_ = o.hx[id] - o.hx[o.I3(i, j, k - 1)];
+                        curl = dHz_ / o.kedz[k] - (o.hz[id] - o.hz[o.I3(im, j, k)]) / o.kedx[i];
+                    } else {
+                        curl = (o.hy[id] - o.hy[o.I3(im, j, k)]) / o.kedx[i] - (o.hx[id] - o.hx[o.I3(i, jm, k)]) / o.kedy[j];
+                    }
+                    double *e = c == 0 ? o.ex.data() : (c == 1 ? o.ey.data() : o.ez.data());
+                    const int m = mask[id];
+                    if (disp_id(m)) ade_update(o, c, id, curl);
+                    else e[id] = Cee * e[id] + Ceh * curl;
+                    if (c == 2) continue;
+                    vec &pn = c == 0 ? o.psi_Ex_z_N : o.psi_Ey_z_N;
+                    vec &pf = c == 0 ? o.psi_Ex_z_F : o.psi_Ey_z_F;
+                    const double sg = c == 0 ? -1.0 : 1.0;   // ex -= Ceh psi, ey += Ceh psi
+                    if (k < o.cpml_N_Z + 1) {
+                        const size_t q = (size_t)k + (NcZ + 1) * ((size_t)j + (size_t)NY * i);
+                        pn[q] = o.be_z_N[k] * pn[q] + o.ce_z_N[k] * dHz_ / dz;
+                        e[id] += sg * Ceh * pn[q];
+                    }
+                    if (k >= o.cpml_F_Z) {
+                        const int k2 = k - o.cpml_F_Z;
+                        const size_t q = (size_t)k2 + (NcZ + 1) * ((size_t)j + (size_t)NY * i);
+                        pf[q] = o.be_z_F[k2] * pf[q] + o.ce_z_F[k2] * dHz_ / dz;
+                        e[id] += sg * Ceh * pf[q];
+                    }
+                }
+            }
+    }
+}
+
 void tfsf(Oracle &o) {
     const int L = o.inc_Length;
     // UPDATE_h_inc, main.cu:2328-2336
     for (int i = 0; i < L - 1; i++) o.h_inc[i] = o.h_inc[i] - (Z0 * o.dt / (MU0 * o.d_1D)) * (o.e_inc[i + 1] - o.e_inc[i]);
-    correct(o);
+    if (o.periodic) correct_periodic(o);
+    else correct(o);
     // UPDATE_e_inc, main.cu:2280-2326
     for (int i = 1; i < L - 1; i++) o.e_inc[i] = o.e_inc[i] - (o.dt / (Z0 * EP0 * o.d_1D)) * (o.h_inc[i] - o.h_inc[i - 1]);
     // Mur ABC + source on the host, main.cu:2831-2859
@@ -619,6 +820,8 @@ inline double ade_update(Oracle &o, int c, size_t id, double curl) {
     double CE, CE1, CE2;
     ade_coefs(o, r, CE, CE1, CE2);
     double CP1 = (o.d1 - 1) * o.Pd_n[c][id], CP3 = o.d2 * o.Pd_n_1[c][id];
+    // NONLOCAL == 1: d_NL grad(div P_d^n) in both the E update and the Drude pole (updatefields.cu:876-891)
+    const double CPNL = o.nonlocal1 ? o.dNL * o.GD[c][id] : 0.0;
     double CP2 = 0, CP4 = 0;
     for (int n = 0; n < 2; n++) {
         CP2 += (o.c1[n] - 1) * o.Pcp_n[c][2 * id + n];
@@ -626,10 +829,10 @@ inline double ade_update(Oracle &o, int c, size_t id, double curl) {
     }
     o.e_n_1[c][id] = o.e_n[c][id];
     o.e_n[c][id] = e[id];
-    e[id] = (1 / CE) * (o.dt * curl / Z0 + CE1 * o.e_n[c][id] - CE2 * o.e_n_1[c][id] - CP1 - CP2 - CP3 - CP4);
+    e[id] = (1 / CE) * (o.dt * curl / Z0 + CE1 * o.e_n[c][id] - CE2 * o.e_n_1[c][id] - CP1 - CP2 - CP3 - CP4 - CPNL);
     for (int n = 0; n < 2; n++)
         o.Pcp[c][2 * id + n] = o.c1[n] * o.Pcp_n[c][2 * id + n] + o.c2[n] * o.Pcp_n_1[c][2 * id + n] + r * (o.c3[n] * e[id] + o.c4[n] * o.e_n[c][id] + o.c5[n] * o.e_n_1[c][id]);
-    o.Pd[c][id] = o.d1 * o.Pd_n[c][id] + o.d2 * o.Pd_n_1[c][id] + r * (o.d3 * e[id] + o.d4 * o.e_n[c][id] + o.d5 * o.e_n_1[c][id]);
+    o.Pd[c][id] = o.d1 * o.Pd_n[c][id] + o.d2 * o.Pd_n_1[c][id] + r * (o.d3 * e[id] + o.d4 * o.e_n[c][id] + o.d5 * o.e_n_1[c][id]) + CPNL;
     o.Pd_n_1[c][id] = o.Pd_n[c][id];
     o.Pd_n[c][id] = o.Pd[c][id];
     for (int n = 0; n < 2; n++) {
@@ -678,7 +881,7 @@ void hydro_continuity(Oracle &o, int c, const std::vector<int> &mask) {
         for (int j = 1; j < NY - 1; j++)
             for (int k = 1; k < NZ - 1; k++) {
                 size_t id = I(i, j, k);
-                if (!(mask[id] > 1 && mask[id] < 6)) continue;
+                if (!disp_id(mask[id])) continue;
                 double Vx1, Vx2, Nx1, Nx2, Vy1, Vy2, Ny1, Ny2, Vz1, Vz2, Nz1, Nz2, r;
                 if (c == 0) {
                     Vx1 = Px[I(i + 1, j, k)]; Vx2 = Px[I(i - 1, j, k)]; Nx1 = Nx[I(i + 1, j, k)]; Nx2 = Nx[I(i - 1, j, k)];
@@ -711,11 +914,42 @@ void hydro_continuity(Oracle &o, int c, const std::vector<int> &mask) {
     o.ND[c].swap(upd);
 }
 
+// grad(div P_d) of the Drude pole at each metal E-component position
+// (Calc_DIV_GRADx/y/z, updatefields.cu:1436-1604, non-periodic branch), every
+// neighbour at time level n (the reference's GPU kernels would race on them;
+// its calls there are commented out, main.cu:1090); P_d is zero outside the
+// metal (hard wall).
+void nonlocal_gd(Oracle &o) {
+    const int NX = o.NX, NY = o.NY, NZ = o.NZ;
+    const double IX = 1.0 / o.dx, IY = 1.0 / o.dy, IZ = 1.0 / o.dz;
+    const vec &Px = o.Pd_n[0], &Py = o.Pd_n[1], &Pz = o.Pd_n[2];
+    auto I = [&](int i, int j, int k) { return o.I3(i, j, k); };
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < NX - 1; i++)
+        for (int j = 1; j < NY - 1; j++)
+            for (int k = 1; k < NZ - 1; k++) {
+                const size_t id = I(i, j, k);
+                if (disp_id(o.matX[id]))
+                    o.GD[0][id] = IX * IX * (Px[I(i + 1, j, k)] - 2.0 * Px[id] + Px[I(i - 1, j, k)])
+                                + IX * IY * (Py[I(i + 1, j, k)] - Py[id] - Py[I(i + 1, j - 1, k)] + Py[I(i, j - 1, k)])
+                                + IX * IZ * (Pz[I(i + 1, j, k)] - Pz[id] - Pz[I(i + 1, j, k - 1)] + Pz[I(i, j, k - 1)]);
+                if (disp_id(o.matY[id]))
+                    o.GD[1][id] = IY * IY * (Py[I(i, j + 1, k)] - 2.0 * Py[id] + Py[I(i, j - 1, k)])
+                                + IX * IY * (Px[I(i, j + 1, k)] - Px[id] - Px[I(i - 1, j + 1, k)] + Px[I(i - 1, j, k)])
+                                + IY * IZ * (Pz[I(i, j + 1, k)] - Pz[id] - Pz[I(i, j + 1, k - 1)] + Pz[I(i, j, k - 1)]);
+                if (disp_id(o.matZ[id]))
+                    o.GD[2][id] = IZ * IZ * (Pz[I(i, j, k + 1)] - 2.0 * Pz[id] + Pz[I(i, j, k - 1)])
+                                + IZ * IY * (Py[I(i, j, k + 1)] - Py[id] - Py[I(i, j - 1, k + 1)] + Py[I(i, j - 1, k)])
+                                + IX * IZ * (Px[I(i, j, k + 1)] - Px[id] - Px[I(i - 1, j, k + 1)] + Px[I(i - 1, j, k)]);
+            }
+}
+
 void update_e(Oracle &o) {
     const int NX = o.NX, NY = o.NY, NZ = o.NZ;
     const double dx = o.dx, dy = o.dy, dz = o.dz, Cee = o.Cee, Ceh = o.Ceh;
     const int NcY = o.NcY, NcZ = o.NcZ;
     if (o.hydro) hydro_p(o);
+    if (o.nonlocal1) nonlocal_gd(o);
     // UPDATE_ex, main.cu:1172-1327
     if (o.hydro) hydro_continuity(o, 0, o.matX);
 #pragma omp parallel for schedule(static)
@@ -725,7 +959,7 @@ void update_e(Oracle &o) {
                 size_t id = o.I3(i, j, k);
                 double curl = (o.hz[id] - o.hz[o.I3(i, j - 1, k)]) / o.kedy[j] - (o.hy[id] - o.hy[o.I3(i, j, k - 1)]) / o.kedz[k];
                 int m = o.matX[id];
-                if (m > 1 && m < 6) {
+                if (disp_id(m)) {
                     if (!o.hydro) ade_update(o, 0, id, curl);
                     else hydro_e_update(o, 0, id, curl);
                 } else {
@@ -763,7 +997,7 @@ void update_e(Oracle &o) {
                 size_t id = o.I3(i, j, k);
                 double curl = (o.hx[id] - o.hx[o.I3(i, j, k - 1)]) / o.kedz[k] - (o.hz[id] - o.hz[o.I3(i - 1, j, k)]) / o.kedx[i];
                 int m = o.matY[id];
-                if (m > 1 && m < 6) {
+                if (disp_id(m)) {
                     if (!o.hydro) ade_update(o, 1, id, curl);
                     else hydro_e_update(o, 1, id, curl);
                 } else {
@@ -801,7 +1035,7 @@ void update_e(Oracle &o) {
                 size_t id = o.I3(i, j, k);
                 double curl = (o.hy[id] - o.hy[o.I3(i - 1, j, k)]) / o.kedx[i] - (o.hx[id] - o.hx[o.I3(i, j - 1, k)]) / o.kedy[j];
                 int m = o.matZ[id];
-                if (m > 1 && m < 6) {
+                if (disp_id(m)) {
                     if (!o.hydro) ade_update(o, 2, id, curl);
                     else hydro_e_update(o, 2, id, curl);
                 } else {
@@ -888,7 +1122,14 @@ void dft_sample(Oracle &o) {
 
 void end_step(Oracle &o) {
     // main.cu:2920: DFT on every dft_stride-th step when StaticField == 0
-    if (o.P.StaticField == 0 && o.t % o.P.dft_stride == 0) dft_sample(o);
+    if (o.P.StaticField == 0 && o.t % o.P.dft_stride == 0) {
+        dft_sample(o);
+        // the incident wave at the source plane, sampled like the monitors (no reference
+        // counterpart: physical normalisation of the periodic spectra)
+        if (o.inc_dft.empty()) o.inc_dft.assign(o.P.NUM_freq, 0.0);
+        for (int m = 0; m < o.P.NUM_freq; m++)
+            o.inc_dft[m] += o.e_inc[o.m0] * std::exp(std::complex<double>(0, 2 * PI * (o.t + 1.0) * o.dt * o.freq[m]));
+    }
     if (o.obj_on) {
         const size_t id = o.I3(o.r0[0], o.r0[1], o.r0[2]);
         const double ev[3] = {o.ex[id], o.ey[id], o.ez[id]};
@@ -1001,7 +1242,7 @@ void adjoint_e(Oracle &o, Adjoint &A, int t, const std::vector<double> &traj, co
                     // (a) transposed main update: vacuum or ADE
                     double Q;
                     const int m = (*mats[c])[id];
-                    if (m > 1 && m < 6) {
+                    if (disp_id(m)) {
                         const double r = o.rho.empty() ? 1.0 : o.rho[id];
                         double CE, CE1, CE2;
                         ade_coefs(o, r, CE, CE1, CE2);
@@ -1247,6 +1488,16 @@ void orc_default_params(fdtd_params *p) {
 }
 
 int orc_create(const fdtd_params *p, void **out) {
+    // Flush denormals to zero on every OpenMP thread: values below 2.2e-308
+    // change no check, and tiny periodic grids whose fields decay into that
+    // range otherwise run two orders of magnitude slower.
+#pragma omp parallel
+    {
+        _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+        _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+    }
+    _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+    _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
     Oracle *o = new (std::nothrow) Oracle();
     if (!o) return FDTD_ERR_ALLOC;
     int rc = setup(*o, *p);
@@ -1275,7 +1526,7 @@ int orc_info(void *h, fdtd_grid_info *g) {
     const std::vector<int> *ms[3] = {&o.matX, &o.matY, &o.matZ};
     for (int c = 0; c < 3; c++) {
         int n = 0;
-        for (int v : *ms[c]) n += (v > 1 && v < 6);
+        for (int v : *ms[c]) n += disp_id(v);
         g->n_dispersive[c] = n;
     }
     return FDTD_OK;
@@ -1311,6 +1562,18 @@ int orc_add_sphere(void *h, const double *centre, double radius) {
     return FDTD_OK;
 }
 
+// Metal layer k in [k0, k1) over every column (inf_disp_slab masks, setup.cu:156-200).
+int orc_add_layer(void *h, int k0, int k1) {
+    Oracle &o = *(Oracle *)h;
+    for (int i = 0; i < o.NX; i++)
+        for (int j = 0; j < o.NY; j++)
+            for (int k = std::max(k0, 0); k < std::min(k1, o.NZ); k++) {
+                const size_t id = o.I3(i, j, k);
+                o.mat[id] = o.matX[id] = o.matY[id] = o.matZ[id] = o.P.material;
+            }
+    return FDTD_OK;
+}
+
 int orc_get_materials(void *h, int comp, uint8_t *out) {
     Oracle &o = *(Oracle *)h;
     const std::vector<int> &m = comp == 0 ? o.mat : comp == 1 ? o.matX : comp == 2 ? o.matY : o.matZ;
@@ -1337,14 +1600,23 @@ int orc_get_state(void *h, int which, int comp, double *out) {
     return FDTD_OK;
 }
 
-int orc_update_h(void *h) { update_h(*(Oracle *)h); return FDTD_OK; }
+void step_h(Oracle &o) {
+    if (o.periodic) update_h_periodic(o);
+    else update_h(o);
+}
+void step_e(Oracle &o) {
+    if (o.periodic) update_e_periodic(o);
+    else update_e(o);
+}
+
+int orc_update_h(void *h) { step_h(*(Oracle *)h); return FDTD_OK; }
 int orc_tfsf(void *h) { tfsf(*(Oracle *)h); return FDTD_OK; }
-int orc_update_e(void *h) { update_e(*(Oracle *)h); return FDTD_OK; }
+int orc_update_e(void *h) { step_e(*(Oracle *)h); return FDTD_OK; }
 int orc_end_step(void *h) { end_step(*(Oracle *)h); return FDTD_OK; }
 
 int orc_step(void *h, int n) {
     Oracle &o = *(Oracle *)h;
-    for (int s = 0; s < n; s++) { update_h(o); tfsf(o); update_e(o); end_step(o); }
+    for (int s = 0; s < n; s++) { step_h(o); tfsf(o); step_e(o); end_step(o); }
     return FDTD_OK;
 }
 
@@ -1389,6 +1661,29 @@ int orc_get_dft(void *h, int face, int comp, double *out) {
     return FDTD_OK;
 }
 
+// Periodic spectra (Reflectance_XZ, main.cu:3195-3330): R, T physically
+// normalised by the incident line DFT; ref_r, ref_t the reference's columns
+// sum Re(Ey Hx* - Ex Hy*) dA / Area / |E_incident|^2 at the two planes.
+int orc_reflectance(void *h, double *R, double *T, double *ref_r, double *ref_t) {
+    Oracle &o = *(Oracle *)h;
+    if (!o.periodic) return fail("reflectance needs periodic_xy");
+    const int F = o.P.NUM_freq;
+    std::vector<double> raw(12 * F);
+    flux(o, nullptr, nullptr, nullptr, raw.data());
+    const double area = o.NX * o.dx * o.NY * o.dy;
+    for (int m = 0; m < F; m++) {
+        const double a2 = std::norm(o.E_incident[m]);
+        // faces 4, 5: S_z = Re(Ex Hy* - Ey Hx*) dA summed, divided by |E_incident|^2
+        const double sr = raw[4 * F + m] * a2, st = raw[5 * F + m] * a2;
+        const double inc = o.inc_dft.empty() ? 0.0 : std::norm(o.inc_dft[m]) * area;
+        if (R) R[m] = inc > 0 ? -sr / inc : 0.0;
+        if (T) T[m] = inc > 0 ? st / inc : 0.0;
+        if (ref_r) ref_r[m] = -sr / area / a2;
+        if (ref_t) ref_t[m] = -st / area / a2;
+    }
+    return FDTD_OK;
+}
+
 int orc_flux(void *h, double *sca, double *abs_, double *ext, double *raw) {
     flux(*(Oracle *)h, sca, abs_, ext, raw);
     return FDTD_OK;
@@ -1425,6 +1720,7 @@ int orc_threads(void) { return omp_get_max_threads(); }
 // component there takes the ADE branch with coefficients blended by rho.
 int orc_set_design(void *h, const int *lo, const int *n, const double *rho) {
     Oracle &o = *(Oracle *)h;
+    if (o.nonlocal1 || o.hydro || o.P.material == 7) return fail("design densities need the local Drude-CP model");
     if (o.rho.empty()) o.rho.assign(o.ex.size(), 1.0);
     for (int a = 0; a < 3; a++) { o.dlo[a] = lo[a]; o.dn[a] = n[a]; }
     size_t v = 0;

@@ -38,6 +38,8 @@ def load():
         "orc_get_materials": (ip, [vp, ip, u8p]),
         "orc_add_sphere": (ip, [vp, dp, C.c_double]),
         "orc_get_state": (ip, [vp, ip, ip, dp]),
+        "orc_add_layer": (ip, [vp, ip, ip]),
+        "orc_reflectance": (ip, [vp, dp, dp, dp, dp]),
         "orc_update_h": (ip, [vp]), "orc_tfsf": (ip, [vp]), "orc_update_e": (ip, [vp]),
         "orc_end_step": (ip, [vp]), "orc_step": (ip, [vp, ip]),
         "orc_get_field": (ip, [vp, ip, dp]), "orc_set_field": (ip, [vp, ip, dp]),
@@ -115,6 +117,11 @@ class Oracle:
         self.lib.orc_add_sphere(self.h, _ptr(c), float(radius))
         self.info = self.get_info()
 
+    def add_layer(self, k0, k1):
+        """Metal on every E component at k in [k0, k1) over all columns."""
+        self.lib.orc_add_layer(self.h, int(k0), int(k1))
+        self.info = self.get_info()
+
     def get_materials(self, comp):
         out = np.empty(self.shape, dtype=np.uint8)
         self.lib.orc_get_materials(self.h, comp, _ptr(out, C.c_uint8))
@@ -175,6 +182,14 @@ class Oracle:
         sca, ab, ext, raw = np.empty(F), np.empty(F), np.empty(F), np.empty(12 * F)
         self.lib.orc_flux(self.h, *[_ptr(a) for a in (sca, ab, ext, raw)])
         return {"sca": sca, "abs": ab, "ext": ext, "raw": raw.reshape(12, F)}
+
+    def reflectance(self):
+        """Periodic spectra: R, T (physical) and the reference's two columns."""
+        F = self.info.num_freq
+        out = [np.empty(F) for _ in range(4)]
+        if self.lib.orc_reflectance(self.h, *[_ptr(a) for a in out]) != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return dict(zip(("R", "T", "ref_r", "ref_t"), out))
 
     def coeffs(self):
         out = np.empty(64)

@@ -80,8 +80,10 @@ def _create_rc(**over):
 
 @pytest.mark.parametrize("over,code,word", [
     ({"TimeFactor": 0.8}, fdtd_amd.abi.FDTD_ERR_INVALID, "Courant"),
-    ({"NONLOCAL": 1}, fdtd_amd.abi.FDTD_ERR_UNSUPPORTED, "NONLOCAL"),
     ({"NONLOCAL": 2}, fdtd_amd.abi.FDTD_ERR_UNSUPPORTED, "NONLOCAL"),
+    ({"NONLOCAL": 3, "material": 7}, fdtd_amd.abi.FDTD_ERR_UNSUPPORTED, "hydrodynamic"),
+    ({"periodic_xy": 1}, fdtd_amd.abi.FDTD_ERR_INVALID, "NcpmlX"),
+    ({"periodic_xy": 1, "NcpmlX": 0, "NcpmlY": 0, "inc_theta": 30.0}, fdtd_amd.abi.FDTD_ERR_UNSUPPORTED, "normal"),
     ({"material": 9}, fdtd_amd.abi.FDTD_ERR_INVALID, "material"),
     ({"dx": 1e-9}, fdtd_amd.abi.FDTD_ERR_INVALID, "cubic"),
 ])

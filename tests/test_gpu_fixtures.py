@@ -1,0 +1,107 @@
+"""Full-length runs of the benchmark configurations against FP64 oracle
+fixtures (tests/golden/, written by tests/golden/make_fixtures.py).
+
+* Precision study: the 281^3 gold dimer (two 30 nm spheres, 2 nm gap, dx =
+  1 nm), 20,000 FP32 steps with the split-BF16 MFMA DFT contraction; the
+  extinction, scattering and absorption spectra at 64 wavelengths within 1e-4
+  of their maximum, and the gap-field |E|^2 within 1e-4 of its maximum.
+* Hydrodynamic model, down-scaled to 96^3 interior at dx = 1 nm (10-cell
+  sphere, beta = 1.27e6 m/s, the 600 nm pulse), all 40,000 steps: the
+  spectra and face fluxes at 48 wavelengths within 1e-4 of their maximum.
+Both also compare the six field components on two lines through the grid
+centre after the last step, relative to the largest value on those lines.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fdtd_amd as G
+from fdtd_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with np.load(os.path.join(HERE, name), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+def line_errors(s, ref, centre, scale=None):
+    """Largest difference of each component on the two lines, normalised by
+    the largest E (or H) value on the fixture's lines, or by `scale`."""
+    errs = []
+    groups = [max(max(np.abs(ref[f"xline_{c}"]).max(), np.abs(ref[f"zline_{c}"]).max()) for c in g)
+              for g in ((0, 1, 2), (3, 4, 5))]
+    for comp in range(6):
+        f = s.get_field(comp)
+        xl, zl = f[:, centre[1], centre[2]], f[centre[0], centre[1], :]
+        m = scale if scale else groups[comp // 3]
+        errs.append(max(np.abs(xl - ref[f"xline_{comp}"]).max(), np.abs(zl - ref[f"zline_{comp}"]).max()) / m)
+    return errs
+
+
+@pytest.mark.timeout(600)
+def test_dimer_20000_steps_against_fp64_fixture():
+    ref = load("dimer281_fp64.npz")
+    p = W.dimer(G)
+    s = G.Simulation(p)
+    T = s.info.NX
+    for c in W.dimer_centres(T):
+        s.add_sphere(c, W.DIMER_RADIUS)
+    assert list(s.get_info().n_dispersive) == list(ref["n_dispersive"])
+    gap = W.dimer_gap(T)
+    assert tuple(ref["gap"]) == gap
+    s.set_point_monitor(gap, int(ref["steps"]))
+    s.step(int(ref["steps"]))
+    fl = s.flux()
+    errs = {k: rel(fl[k], ref[k]) for k in ("ext", "sca", "abs")}
+    errs["raw"] = rel(fl["raw"], ref["raw"])
+    e2, amp = s.point_spectrum(ref["freq"])
+    e2o = np.sum(np.abs(ref["gap_terms"]) ** 2, axis=1)
+    errs["gap |E|^2"] = rel(e2, e2o)
+    lines = line_errors(s, ref, gap)
+    print("dimer 281^3, %d steps:" % int(ref["steps"]), {k: "%.2e" % v for k, v in errs.items()},
+          "field lines", ["%.1e" % x for x in lines], flush=True)
+    print("ext (GPU, FP64) at 400, 700, 1000 nm:", fl["ext"][[0, 32, 63]], ref["ext"][[0, 32, 63]], flush=True)
+    assert max(errs.values()) < 1e-4
+    assert max(lines) < 1e-3
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("exact", [1, 0])
+def test_hydrodynamic_96_40000_steps_against_fp64_fixture(exact):
+    """exact = 1: FP64 DFT contraction; 0: split-BF16 MFMA.  The absorption
+    is the small difference of six face fluxes that each carry the incident
+    power (about 30 times larger), so it amplifies their relative error."""
+    ref = load("hydro96_fp64.npz")
+    p = W.hydro(G, n=96, dx=1e-9)
+    p.dft_exact = exact
+    s = G.Simulation(p)
+    T = s.info.NX
+    assert s.get_info().hydro == 1
+    assert list(s.get_info().n_dispersive) == list(ref["n_dispersive"])
+    steps = int(ref["steps"])
+    done = 0
+    while done < steps:
+        s.step(5000)
+        done += 5000
+        print("  step", done, flush=True)
+    fl = s.flux()
+    errs = {k: rel(fl[k], ref[k]) for k in ("ext", "sca", "abs")}
+    errs["raw"] = rel(fl["raw"], ref["raw"])
+    # the field left at step 40,000 is a residual some 400 times below the
+    # incident amplitude: the lines are judged against that amplitude (MAX_AMP)
+    lines = line_errors(s, ref, (T // 2, T // 2, T // 2), scale=p.MAX_AMP)
+    print("hydro 96^3, %d steps, dft_exact=%d:" % (steps, exact), {k: "%.2e" % v for k, v in errs.items()},
+          "field lines", ["%.1e" % x for x in lines], flush=True)
+    assert errs["raw"] < 1e-4
+    assert max(lines) < 1e-3
+    if exact:
+        assert max(errs.values()) < 1e-4

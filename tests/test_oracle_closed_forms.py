@@ -280,3 +280,68 @@ def test_point_objective_terms_are_the_dft_of_the_recorded_field():
     got = o.objective_terms()
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
     assert o.objective() == pytest.approx(float(np.sum(np.abs(ref) ** 2)), rel=1e-12)
+
+
+# ---------------------------------------------------------------- Lorentz-Drude (material 7)
+def make_with(**over):
+    p = small_params(O)
+    for k, v in over.items():
+        setattr(p, k, v)
+    return O.Oracle(p), p
+
+
+def lorentz_drude_eps(w):
+    """Lorentz-Drude permittivity of material 7 (setup.cu:1256-1275), exp(-i w t):
+    eps_inf - wD^2/(w^2 + i gD w) + sum_L deps_L wL^2/(wL^2 - w^2 - 2 i dL w),
+    parameters in eV converted with 2 pi / h."""
+    h = 4.13566553853599E-15
+    c = 2 * np.pi / h
+    e = 3.559 - (c * 8.812) ** 2 / (w ** 2 + 1j * c * 0.0752 * w)
+    for de, dl, wl in ((2.912, 1.541, 4.693), (1.272, 0.525, 3.112)):
+        e = e + de * (c * wl) ** 2 / ((c * wl) ** 2 - w ** 2 - 2j * c * dl * w)
+    return e
+
+
+def test_lorentz_drude_update_reproduces_its_permittivity():
+    """Material 7: the z-transform of the five-coefficient pole recursions
+    with the E update built from their sums (C_E, C_E_1, C_E_2) equals the
+    continuous Lorentz-Drude permittivity to O((w dt)^2); also the sums."""
+    o, _ = make_with(material=7)
+    c = o.coeffs()
+    lam = np.linspace(400e-9, 1000e-9, 13)
+    w = 2 * np.pi * C0 / lam
+    ed = discrete_eps(c, w)
+    ec = np.conj(lorentz_drude_eps(w))
+    err = np.abs(ed - ec) / np.abs(ec)
+    wdt = w * c["dt"]
+    assert np.all(err < 5 * wdt ** 2 + 1e-9), (err, wdt)
+    assert c["eps_inf"] == 3.559
+    S3 = c["d3"] + c["c3_0"] + c["c3_1"]
+    assert c["C_E"] == pytest.approx(EP0 * 3.559 + S3, rel=1e-14)
+    # the Drude pole keeps k1 + k2 = 1 (no restoring force)
+    assert c["d1"] + c["d2"] == pytest.approx(1.0, abs=1e-15)
+
+
+def test_nonlocal_pole_with_zero_velocity_is_the_local_model():
+    """NONLOCAL = 1 with beta = 0 must give the local run's fields bit for bit."""
+    out = []
+    for nl in (0, 1):
+        o, p = make_with(NONLOCAL=nl, hydro_beta=0.0, Tend=200)
+        o.step(200)
+        out.append([o.get_field(k) for k in range(6)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("material", [3, 7])
+def test_nonlocal_pole_stays_bounded(material):
+    """A sign mistake in the grad-div term turns the longitudinal modes
+    unstable (w^2 = wp^2 - beta^2 k^2 < 0 on the grid's shortest waves): with
+    a large beta the run must stay finite and decay after the pulse."""
+    o, p = make_with(NONLOCAL=1, hydro_beta=8e6, material=material, Tend=900)
+    peaks = []
+    for _ in range(3):
+        o.step(300)
+        peaks.append(max(np.abs(o.get_field(k)).max() for k in range(3)))
+    assert np.all(np.isfinite(peaks)), peaks
+    assert peaks[2] < 2 * max(peaks[:2]), peaks
