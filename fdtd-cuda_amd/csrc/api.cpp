@@ -287,12 +287,13 @@ int contract(fdtd_sim *s) {
         if (!a) return set_err(s, FDTD_ERR_DEVICE, "hipEventCreate failed");
         HIPCHK(s, hipEventRecord(a, s->stream));
     }
-    if (s->S.P.dft_exact) {
+    if (s->S.P.dft_exact == 1) {
         HIPCHK(s, launch_twiddle(s->W, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
         HIPCHK(s, launch_dft_contract(s->X, s->W, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
     } else {
-        HIPCHK(s, launch_twiddle_frag(s->Wf, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, s->stream));
-        HIPCHK(s, launch_dft_mfma(s->X, s->Wf, s->acc, s->mtot, s->fd.mE_pad, s->K, F, s->stream));
+        const int parts = s->S.P.dft_exact == 2 ? 3 : 2;
+        HIPCHK(s, launch_twiddle_frag(s->Wf, s->slot_t.data(), s->K, s->slot, F, s->freq_d, s->S.dt, parts, s->stream));
+        HIPCHK(s, launch_dft_mfma(s->X, s->Wf, s->acc, s->mtot, s->fd.mE_pad, s->K, F, parts, s->stream));
     }
     if (timed) {
         HIPCHK(s, hipEventRecord(next_event(s, 1), s->stream));
@@ -728,7 +729,7 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
     const int F = S.P.NUM_freq;
     // the sample ring holds K rows padded to the MFMA k-step (16); padding rows stay zero
     const int kpad = (s->K + 15) / 16 * 16;
-    const size_t wf_bytes = (size_t)2 * (kpad / 16) * ((2 * F + 31) / 32) * 128 * 16;
+    const size_t wf_bytes = (size_t)2 * (kpad / 16) * ((2 * F + 31) / 32) * 3 * 64 * 16;   // up to three parts
     uint8_t *wf = nullptr;
     if ((rc = dalloc(s, &wf, wf_bytes))) return fail(rc);
     s->Wf = wf;

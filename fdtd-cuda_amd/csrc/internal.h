@@ -248,10 +248,11 @@ hipError_t launch_twiddle(double *W, const int *t_slot_host, int K, int nvalid, 
                           double dt, hipStream_t st);
 hipError_t launch_dft_contract(const float *X, const double *W, double *acc, long mtot, long mE_pad, int K,
                                int F, hipStream_t st);
-// Split-BF16 MFMA path: twiddle fragments (2 * ceil(K/16) * ceil(2F/32) * 128 * 16 bytes) and contraction.
+// Split-BF16 MFMA path in `parts` (2 or 3) BF16 parts: twiddle fragments
+// (2 * ceil(K/16) * ceil(2F/32) * parts * 64 * 16 bytes) and contraction.
 hipError_t launch_twiddle_frag(void *Wf, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
-                               double dt, hipStream_t st);
-hipError_t launch_dft_mfma(const float *X, const void *Wf, double *acc, long mtot, long mE_pad, int K, int F,
+                               double dt, int parts, hipStream_t st);
+hipError_t launch_dft_mfma(const float *X, const void *Wf, double *acc, long mtot, long mE_pad, int K, int F, int parts,
                            hipStream_t st);
 hipError_t launch_flux_reduce(const double *acc, const FaceDev &fd, long mtot, long mE_pad, int F, double dA,
                               double *out, hipStream_t st);

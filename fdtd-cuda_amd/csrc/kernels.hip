@@ -1799,10 +1799,24 @@ __device__ inline float from_bf16(__bf16 b) {
     return __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
 }
 
+// Split of a value into P BF16 parts v ~ p0 + p1 + ... (each the rounded
+// remainder of the previous ones): two parts carry ~16 significant bits,
+// three ~24 (FP32).
+template <int P>
+__device__ inline void split_bf16(double v, __bf16 *out) {
+    double r = v;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+        out[q] = to_bf16((float)r);
+        r -= (double)from_bf16(out[q]);
+    }
+}
+
 // Twiddles in the A-operand fragment order of v_mfma_f32_32x32x16_bf16:
 // lane l of m-tile mt and k-step ks holds W[s][r] for r = 32 mt + (l % 32) and
-// s = 16 ks + 8 (l / 32) + j, j = 0..7, split into hi and lo BF16 halves.
-// Wf index: (((half * nks + ks) * nmt + mt) * 2 + hi_lo) * 64 + lane.
+// s = 16 ks + 8 (l / 32) + j, j = 0..7, split into P BF16 parts.
+// Wf index: (((half * nks + ks) * nmt + mt) * P + part) * 64 + lane.
+template <int P>
 __global__ void k_twiddle_frag(bf16x8 *Wf, SlotTimes ts, int nks, int nmt, int nvalid, int F, const double *freq,
                                double dt) {
     const int n = 2 * nks * nmt * 64;
@@ -1810,7 +1824,7 @@ __global__ void k_twiddle_frag(bf16x8 *Wf, SlotTimes ts, int nks, int nmt, int n
     if (x >= n) return;
     const int l = x % 64, mt = (x / 64) % nmt, ks = (x / (64 * nmt)) % nks, half = x / (64 * nmt * nks);
     const int r = 32 * mt + (l % 32);
-    bf16x8 hi, lo;
+    bf16x8 part[P];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const int sidx = 16 * ks + 8 * (l / 32) + j;
@@ -1821,21 +1835,25 @@ __global__ void k_twiddle_frag(bf16x8 *Wf, SlotTimes ts, int nks, int nmt, int n
             const double ph = 2.0 * kPI * tt * dt * freq[f];
             v = r < F ? cos(ph) : sin(ph);
         }
-        const float vf = (float)v;
-        hi[j] = to_bf16(vf);
-        lo[j] = to_bf16((float)(v - (double)from_bf16(hi[j])));
+        __bf16 b[P];
+        split_bf16<P>(v, b);
+#pragma unroll
+        for (int q = 0; q < P; q++) part[q][j] = b[q];
     }
-    const long base = ((((long)half * nks + ks) * nmt + mt) * 2) * 64 + l;
-    Wf[base] = hi;
-    Wf[base + 64] = lo;
+    const long base = ((((long)half * nks + ks) * nmt + mt) * P) * 64 + l;
+#pragma unroll
+    for (int q = 0; q < P; q++) Wf[base + 64 * q] = part[q];
 }
 
 // acc[r][col] += sum_s W[half][s][r] X[s][col] with split-BF16 products
-// (hi*hi + hi*lo + lo*hi) accumulated in FP32 per block, then added to the
-// FP64 accumulator.  One wave = 32 columns x up to 4 m-tiles (128 rows);
-// grid.y walks groups of 4 m-tiles.  B operand: lane l holds column l % 32,
-// slots 16 ks + 8 (l / 32) + j.  C/D: lane l holds column l % 32, rows
-// 8 i + 4 (l / 32) + j in element 4 i + j.
+// accumulated in FP32 per block, then added to the FP64 accumulator.  P = 2:
+// hi*hi + hi*lo + lo*hi (relative error ~2^-16); P = 3: the six products of
+// total order <= 2 of the three-part splits (~2^-24, FP32 level).  One wave
+// = 32 columns x up to 4 m-tiles (128 rows); grid.y walks groups of 4
+// m-tiles.  B operand: lane l holds column l % 32, slots 16 ks + 8 (l / 32)
+// + j.  C/D: lane l holds column l % 32, rows 8 i + 4 (l / 32) + j in
+// element 4 i + j.
+template <int P>
 __global__ __launch_bounds__(256) void k_dft_mfma(const float *X, const bf16x8 *Wf, double *acc, long mtot,
                                                  long mE_pad, int nks, int nmt, int rows) {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -1852,22 +1870,31 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float *X, const bf16x8 *
 #pragma unroll
         for (int e = 0; e < 16; e++) c[q][e] = 0.f;
     for (int ks = 0; ks < nks; ks++) {
-        bf16x8 bh, bl;
+        bf16x8 b[P];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int sidx = 16 * ks + 8 * (l >> 5) + j;
             const float x = cok ? X[(long)sidx * mtot + col] : 0.f;
-            bh[j] = to_bf16(x);
-            bl[j] = to_bf16(x - from_bf16(bh[j]));
+            __bf16 bs[P];
+            split_bf16<P>((double)x, bs);
+#pragma unroll
+            for (int q = 0; q < P; q++) b[q][j] = bs[q];
         }
-        const long wb = (((long)half * nks + ks) * nmt + 4 * mg) * 2 * 64 + l;
+        const long wb = (((long)half * nks + ks) * nmt + 4 * mg) * P * 64 + l;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (q < nm) {
-                const bf16x8 ah = Wf[wb + (long)q * 128], al = Wf[wb + (long)q * 128 + 64];
-                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c[q], 0, 0, 0);
-                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c[q], 0, 0, 0);
-                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c[q], 0, 0, 0);
+                bf16x8 a[P];
+#pragma unroll
+                for (int u = 0; u < P; u++) a[u] = Wf[wb + (long)q * P * 64 + 64 * u];
+                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], c[q], 0, 0, 0);
+                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], c[q], 0, 0, 0);
+                c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], c[q], 0, 0, 0);
+                if constexpr (P == 3) {
+                    c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], c[q], 0, 0, 0);
+                    c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], c[q], 0, 0, 0);
+                    c[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], c[q], 0, 0, 0);
+                }
             }
         }
     }
@@ -2864,22 +2891,30 @@ hipError_t launch_dft_contract(const float *X, const double *W, double *acc, lon
 }
 
 hipError_t launch_twiddle_frag(void *Wf, const int *t_slot_host, int K, int nvalid, int F, const double *freq,
-                               double dt, hipStream_t st) {
-    if (K > 256) return hipErrorInvalidValue;
+                               double dt, int parts, hipStream_t st) {
+    if (K > 256 || (parts != 2 && parts != 3)) return hipErrorInvalidValue;
     SlotTimes ts;
     for (int i = 0; i < 256; i++) ts.t[i] = i < K ? t_slot_host[i] : 0;
     const int nks = (K + 15) / 16, nmt = (2 * F + 31) / 32;
     const int n = 2 * nks * nmt * 64;
-    hipLaunchKernelGGL(k_twiddle_frag, dim3((n + 255) / 256), dim3(256), 0, st, (bf16x8 *)Wf, ts, nks, nmt, nvalid, F,
-                       freq, dt);
+    if (parts == 2)
+        hipLaunchKernelGGL(k_twiddle_frag<2>, dim3((n + 255) / 256), dim3(256), 0, st, (bf16x8 *)Wf, ts, nks, nmt, nvalid,
+                           F, freq, dt);
+    else
+        hipLaunchKernelGGL(k_twiddle_frag<3>, dim3((n + 255) / 256), dim3(256), 0, st, (bf16x8 *)Wf, ts, nks, nmt, nvalid,
+                           F, freq, dt);
     return hipGetLastError();
 }
 
-hipError_t launch_dft_mfma(const float *X, const void *Wf, double *acc, long mtot, long mE_pad, int K, int F,
+hipError_t launch_dft_mfma(const float *X, const void *Wf, double *acc, long mtot, long mE_pad, int K, int F, int parts,
                            hipStream_t st) {
+    if (parts != 2 && parts != 3) return hipErrorInvalidValue;
     const int nks = (K + 15) / 16, nmt = (2 * F + 31) / 32;
     dim3 grid((unsigned)((mtot + 127) / 128), (nmt + 3) / 4), block(256);
-    hipLaunchKernelGGL(k_dft_mfma, grid, block, 0, st, X, (const bf16x8 *)Wf, acc, mtot, mE_pad, nks, nmt, 2 * F);
+    if (parts == 2)
+        hipLaunchKernelGGL(k_dft_mfma<2>, grid, block, 0, st, X, (const bf16x8 *)Wf, acc, mtot, mE_pad, nks, nmt, 2 * F);
+    else
+        hipLaunchKernelGGL(k_dft_mfma<3>, grid, block, 0, st, X, (const bf16x8 *)Wf, acc, mtot, mE_pad, nks, nmt, 2 * F);
     return hipGetLastError();
 }
 

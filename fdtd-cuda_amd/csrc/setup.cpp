@@ -200,6 +200,7 @@ int build_setup(const fdtd_params &p, Setup &s, std::string &err) {
     if (p.first_medium > 1) { err = "first_medium must be <= 1"; return bad; }
     if (p.dft_stride < 1) { err = "dft_stride must be >= 1"; return bad; }
     if (p.dft_block < 1 || p.dft_block > 256) { err = "dft_block must be in [1, 256]"; return bad; }
+    if (p.dft_exact < 0 || p.dft_exact > 2) { err = "dft_exact must be 0, 1 or 2"; return bad; }
     if (!(p.TimeFactor > 0)) { err = "TimeFactor must be > 0"; return bad; }
     // dt = 0.5 dy / c / TimeFactor (setup.cu:359) must not exceed the 3D Courant
     // limit dx / (c sqrt(3)) for cubic cells, i.e. TimeFactor >= sqrt(3)/2: a

@@ -89,7 +89,8 @@ typedef struct fdtd_params {
     double hydro_beta;         /* nonlocal velocity beta (m/s) of NONLOCAL 1 and 3,
                                   1.39e6*sqrt(3/4) (setup.cu:980-983)                          */
     double xs_area;            /* cross-section normaliser (m^2); 0 -> pi*r^2 (Spectral.cu:1166) */
-    int    dft_exact;          /* 0: split-BF16 MFMA contraction (default), 1: FP64 contraction */
+    int    dft_exact;          /* DFT contraction: 0 split-BF16 MFMA, hi + lo parts (default), 1 FP64,
+                                  2 split-BF16 MFMA in three parts (six products, FP32-level error) */
     int    periodic_xy;        /* 1: x and y periodic (Periodic_XY, main.cu:781-820 and the periodic
                                   branches of every update): NcpmlX = NcpmlY = 0, grid NCELLX by
                                   NCELLY with no +1 (setup.cu:401-405), normal incidence only, the
