@@ -648,6 +648,10 @@ int fdtd_create(const fdtd_params *p, fdtd_sim **out) {
         return set_err(nullptr, FDTD_ERR_DEVICE, "no HIP device");
     }
     const Setup &S = s->S;
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return set_err(nullptr, FDTD_ERR_DEVICE, "hipGetDevice failed");
+    }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return set_err(nullptr, FDTD_ERR_DEVICE, "hipStreamCreate failed");
@@ -1562,6 +1566,14 @@ int fdtd_flux(fdtd_sim *s, double *sca, double *abs_, double *ext, double *raw) 
     if ((rc = check_finite(s))) return rc;
     for (double v : Q)
         if (!std::isfinite(v)) return set_err(s, FDTD_ERR_NONFINITE, "non-finite face flux (the monitor spectra diverged)");
+    flux_combine(s, Q.data(), sca, abs_, ext);
+    if (raw) std::memcpy(raw, Q.data(), Q.size() * sizeof(double));
+    return FDTD_OK;
+}
+
+void fdtd_host::flux_combine(const fdtd_sim *s, double *Q, double *sca, double *abs_, double *ext) {
+    const Setup &S = s->S;
+    const int F = S.P.NUM_freq;
     // normalise by |E_incident|^2 and combine the faces (Spectral.cu:1162-1168)
     for (int m = 0; m < F; m++) {
         const double a = std::abs(S.Einc[m]);
@@ -1580,8 +1592,6 @@ int fdtd_flux(fdtd_sim *s, double *sca, double *abs_, double *ext, double *raw) 
         if (abs_) abs_[m] = ab;
         if (ext) ext[m] = sc + ab;
     }
-    if (raw) std::memcpy(raw, Q.data(), Q.size() * sizeof(double));
-    return FDTD_OK;
 }
 
 int fdtd_kernel_timing(fdtd_sim *s, int enable, double *total_ms, long *launches) {

@@ -80,6 +80,7 @@ struct fdtd_sim {
     long fbase = 0;                 // offset of element (0,0,0) in it
     size_t hsz[6] = {}, esz[6] = {};
     int t = 0;
+    int device = 0;                 // HIP device current at fdtd_create (streams and memory live there)
     // divergence watch: a pinned host word the line probe and the DFT sample
     // kernels set to 1 on a NaN or Inf (status_d: its device address)
     unsigned *status_h = nullptr, *status_d = nullptr;
@@ -171,6 +172,9 @@ int check_finite(fdtd_sim *s);
 bool sampling_step(const fdtd_sim *s);
 int after_sample(fdtd_sim *s);
 int contract(fdtd_sim *s);
+// Face-flux spectra from the 12 x F face sums Q (host, FP64): normalises Q in
+// place by |E_incident|^2 and fills sca/abs/ext (any may be null).
+void flux_combine(const fdtd_sim *s, double *Q, double *sca, double *abs_, double *ext);
 
 }  // namespace fdtd_host
 

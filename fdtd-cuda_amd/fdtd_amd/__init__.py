@@ -22,8 +22,11 @@ EXPORTS = [
     "fdtd_set_design", "fdtd_adjoint", "fdtd_add_rod", "fdtd_set_point_monitor", "fdtd_point_spectrum",
     "fdtd_batch_create", "fdtd_batch_run", "fdtd_batch_tile", "fdtd_batch_last_error", "fdtd_batch_destroy",
     "fdtd_checkpoint_size", "fdtd_checkpoint_save", "fdtd_checkpoint_load",
-    "fdtd_reflectance", "fdtd_add_layer",
+    "fdtd_reflectance", "fdtd_add_layer", "fdtd_flux_noalloc",
 ]
+
+# largest NUM_freq fdtd_flux_noalloc accepts (its fixed device array)
+NOALLOC_MAX_FREQ = 256
 
 BATCH_NO_GRAPH = 1
 BATCH_NO_TUNE = 2
@@ -63,6 +66,7 @@ def load_library(path=LIB_PATH):
         "fdtd_get_slice": (ip, [vp, ip, ip, ip, C.POINTER(C.c_float)]),
         "fdtd_get_dft": (ip, [vp, ip, ip, dp]),
         "fdtd_flux": (ip, [vp, dp, dp, dp, dp]),
+        "fdtd_flux_noalloc": (ip, [vp, dp, dp, dp, dp]),
         "fdtd_kernel_timing": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
         "fdtd_set_design": (ip, [vp, C.POINTER(C.c_int * 3), C.POINTER(C.c_int * 3), fp]),
         "fdtd_adjoint": (ip, [vp, ip, ip, C.POINTER(C.c_int * 3), ip, dp, dp, dp]),
@@ -259,6 +263,13 @@ class Simulation:
         F = self.info.num_freq
         sca, ab, ext, raw = np.empty(F), np.empty(F), np.empty(F), np.empty(12 * F)
         self._check(self.lib.fdtd_flux(self.h, *[_ptr(a, C.c_double) for a in (sca, ab, ext, raw)]))
+        return {"sca": sca, "abs": ab, "ext": ext, "raw": raw.reshape(12, F)}
+
+    def flux_noalloc(self):
+        """flux() through fdtd_flux_noalloc (no device allocation; the same bits)."""
+        F = self.info.num_freq
+        sca, ab, ext, raw = np.empty(F), np.empty(F), np.empty(F), np.empty(12 * F)
+        self._check(self.lib.fdtd_flux_noalloc(self.h, *[_ptr(a, C.c_double) for a in (sca, ab, ext, raw)]))
         return {"sca": sca, "abs": ab, "ext": ext, "raw": raw.reshape(12, F)}
 
     def add_layer(self, k0, k1):

@@ -264,6 +264,18 @@ int  fdtd_get_dft(fdtd_sim *sim, int face, int comp, double *out);
  * be NULL. */
 int  fdtd_flux(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *raw);
 
+/* fdtd_flux without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (set-up of HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The face sums are
+ * reduced in a fixed device array of the library (12 * FDTD_NOALLOC_MAX_FREQ
+ * doubles per device, held until the process ends) instead of a buffer of the
+ * handle; the results equal fdtd_flux's bit for bit.  raw (12 * NUM_freq
+ * doubles) is required; NUM_freq above FDTD_NOALLOC_MAX_FREQ returns
+ * FDTD_ERR_INVALID.  Calls that overlap on one device share that array and
+ * race: serialise them. */
+#define FDTD_NOALLOC_MAX_FREQ 256
+int  fdtd_flux_noalloc(fdtd_sim *sim, double *sca, double *abs_, double *ext, double *raw);
+
 /* Reflectance and transmittance of a periodic (periodic_xy) simulation
  * (Reflectance_XZ, main.cu:3195-3330, and the plane DFTs of Fourier_Transform,
  * Spectral.cu:8-130).  The reflection plane is k_r = Ntfsf_z - sca_offset on
