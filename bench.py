@@ -178,7 +178,7 @@ def main():
             "grid": [info.NX, info.NY, info.NZ],
             "dispersive_cells": list(info.n_dispersive[:]),
             # the tiles fdtd_step chose (fdtd_grid_info.tile)
-            "tile": dict(zip(["rows", "cchunk", "schunk", "row_tiles", "side_row_tiles", "corner_tj", "side_stream"],
+            "tile": dict(zip(["rows", "cchunk", "schunk", "row_tiles", "side_row_tiles", "corner_tj", "side_streams"],
                              list(info.tile)), schedule=info.schedule),
         },
         "roofline": {

@@ -442,6 +442,13 @@ int plan_rows(fdtd_sim *s) {
             hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&s->ev_line, hipEventDisableTiming) != hipSuccess)
             return set_err(s, FDTD_ERR_DEVICE, "second stream or events could not be created");
+        if (const char *v = std::getenv("FDTD_SIDE_STREAMS")) s->side_streams = std::atoi(v) == 1 ? 1 : 3;
+        if (s->side_streams == 3 &&
+            (hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking) != hipSuccess ||
+             hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreateWithFlags(&s->ev_join3, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&s->ev_join4, hipEventDisableTiming) != hipSuccess))
+            return set_err(s, FDTD_ERR_DEVICE, "side streams or events could not be created");
     }
     s->use_rows = s->has_clean;
     return FDTD_OK;
@@ -571,9 +578,16 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     const bool disp_early = fork && s->early && dd && disp_in_clean(s);
     *disp_done = false;
     *samp_done = false;
+    // three side streams: the y slabs and the corner columns beside the line and x slabs
+    const bool three = fork && s->stream3 && s->stream4;
+    hipStream_t side_y = three ? s->stream3 : side, side_c = three ? s->stream4 : side;
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+        if (three) {
+            HIPCHK(s, hipStreamWaitEvent(s->stream3, s->ev_fork, 0));
+            HIPCHK(s, hipStreamWaitEvent(s->stream4, s->ev_fork, 0));
+        }
     }
     HIPCHK(s, launch_line(l, side));
     if (tf_early) HIPCHK(s, hipEventRecord(s->ev_line, side));
@@ -591,11 +605,15 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
         ry.tiles = s->stiles * rows_waves(s->g.NZ) <= 16 ? s->stiles : 1;
         ry.npiece = s->nybox;
         std::memcpy(ry.box, s->ybox, sizeof(s->ybox));
-        HIPCHK(s, launch_step_rows(ry, 2, 2, side));
+        HIPCHK(s, launch_step_rows(ry, 2, 2, side_y));
     }
     // thin columns: short x chunks, 4 rows per block (tile rows 4, 8, 16 measured 0.381, 0.386, 0.386 ms per step)
-    if (s->ncorner) HIPCHK(s, launch_fused_step(d, s->ctj, 2, s->cbox4, s->ncorner, side));
+    if (s->ncorner) HIPCHK(s, launch_fused_step(d, s->ctj, 2, s->cbox4, s->ncorner, side_c));
     if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
+    if (three) {
+        HIPCHK(s, hipEventRecord(s->ev_join3, s->stream3));
+        HIPCHK(s, hipEventRecord(s->ev_join4, s->stream4));
+    }
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
     r.tiles = s->tiles * rows_waves(s->g.NZ) <= 16 ? s->tiles : 1;
@@ -616,6 +634,10 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
         *samp_done = merge_samp;
     }
     if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    if (three) {
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join3, 0));
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join4, 0));
+    }
     if (!tf_early) HIPCHK(s, launch_tfsf_fix(d, s->stream));
     return FDTD_OK;
 }
@@ -755,9 +777,15 @@ int fdtd_destroy(fdtd_sim *s) {
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     if (s->ev_line) (void)hipEventDestroy(s->ev_line);
+    for (hipStream_t q : {s->stream3, s->stream4})
+        if (q) (void)hipStreamSynchronize(q);
+    if (s->ev_join3) (void)hipEventDestroy(s->ev_join3);
+    if (s->ev_join4) (void)hipEventDestroy(s->ev_join4);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->status_h) (void)hipHostFree(s->status_h);
     if (s->stream2) (void)hipStreamDestroy(s->stream2);
+    if (s->stream3) (void)hipStreamDestroy(s->stream3);
+    if (s->stream4) (void)hipStreamDestroy(s->stream4);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return FDTD_OK;
@@ -797,7 +825,7 @@ int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
         g->tile[3] = s->tiles * W <= 16 ? s->tiles : 1;
         g->tile[4] = s->stiles * W <= 16 ? s->stiles : 1;
         g->tile[5] = s->ctj;
-        g->tile[6] = s->stream2 ? 1 : 0;
+        g->tile[6] = s->stream2 ? (s->stream3 ? 3 : 1) : 0;
     } else {
         g->tile[0] = s->tj;
         g->tile[1] = s->ichunk;

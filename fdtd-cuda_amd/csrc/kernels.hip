@@ -1157,10 +1157,12 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     }
 }
 
-// Two rows per thread fit in 128 VGPRs (four waves per SIMD); the cap costs
-// the x-general variant a few spilled registers and gains it a wave per SIMD.
+// Two rows per thread fit in 128 VGPRs (four waves per SIMD) on the clean
+// box.  The side-slab variants are not capped: their launches hold one block
+// per CU at most (a few hundred blocks), so a fourth wave per SIMD gains
+// nothing there, while the cap spilled the x-general variant to scratch.
 template <int W, int R, bool XG, bool YG, bool ADJ = false, int T = 1>
-__global__ __launch_bounds__(64 * W * T) __attribute__((amdgpu_waves_per_eu(R == 2 ? 4 : 1))) void k_step_rows(RowsArgs ra) {
+__global__ __launch_bounds__(64 * W * T) __attribute__((amdgpu_waves_per_eu(R == 2 && !XG && !YG ? 4 : 1))) void k_step_rows(RowsArgs ra) {
     rows_body<W, R, XG, YG, ADJ, T>(ra, ra.xcd ? xcd_order(blockIdx.x, ra.start[ra.npiece]) : (int)blockIdx.x);
 }
 

@@ -99,6 +99,11 @@ struct fdtd_sim {
     bool concurrent = true;         // side slabs on a second stream beside the clean box (FDTD_CONCURRENT=0 to serialise)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_line = nullptr;
+    // side pieces of fdtd_step on separate streams (FDTD_SIDE_STREAMS: 1 = all on stream2, 3 = x
+    // slabs after the line on stream2, y slabs on stream3, corner columns on stream4)
+    int side_streams = 3;
+    hipStream_t stream3 = nullptr, stream4 = nullptr;
+    hipEvent_t ev_join3 = nullptr, ev_join4 = nullptr;
     bool early = true;              // TFSF fix and dispersive box ahead of the join when their cells allow (FDTD_EARLY=0 to keep them after)
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
     int rows = 2, cchunk = 0;       // own rows per thread and x chunk of the row-blocked kernel (clean box;

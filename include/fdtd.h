@@ -125,7 +125,8 @@ typedef struct fdtd_grid_info {
     int    tile[7];            /* tiles fdtd_step uses (no reference counterpart): row-blocked kernel
                                   rows per thread, clean-box x chunk, side-slab x chunk, row tiles
                                   per block on the clean box and on the side slabs, tile rows of the
-                                  corner columns, 1 when the side pieces run on a second stream    */
+                                  corner columns, streams of the side pieces beside the clean box
+                                  (0: none, 1: one, 3: x slabs, y slabs, corners on their own)      */
 } fdtd_grid_info;
 
 typedef struct fdtd_sim fdtd_sim;

@@ -222,7 +222,8 @@ def test_point_monitor_gap_field_of_a_dimer():
 @pytest.mark.parametrize("hydro,radius,offsets,schedule", [(0, 8e-9, (2, 2), 3), (0, 4e-9, (3, 1), 7),
                                                            (1, 8e-9, (2, 2), 3), (1, 4e-9, (3, 1), 7)])
 def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypatch):
-    """fdtd_step on two streams, with the TFSF fix-up and the dispersive box
+    """fdtd_step with the side pieces on three streams (FDTD_SIDE_STREAMS=1:
+    one) beside the clean box, with the TFSF fix-up and the dispersive box
     (schedule bits 0, 1) and the DFT sample (bit 2) following the clean box
     ahead of the side pieces, in one launch with the Drude metal, against
     them after the join (FDTD_EARLY=0) and on one stream (FDTD_CONCURRENT=0):
@@ -237,8 +238,9 @@ def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypa
         p.NONLOCAL = 3
         p.hydro_beta = 1.27e6
     runs = {}
-    for name, env in (("early", {}), ("late", {"FDTD_EARLY": "0"}), ("serial", {"FDTD_CONCURRENT": "0"})):
-        for k in ("FDTD_EARLY", "FDTD_CONCURRENT"):
+    for name, env in (("early", {}), ("late", {"FDTD_EARLY": "0"}), ("one side stream", {"FDTD_SIDE_STREAMS": "1"}),
+                      ("serial", {"FDTD_CONCURRENT": "0"})):
+        for k in ("FDTD_EARLY", "FDTD_CONCURRENT", "FDTD_SIDE_STREAMS"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -248,9 +250,9 @@ def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypa
         s.step(300)
         runs[name] = (info.schedule, [s.get_field(c) for c in range(6)], [s.get_dft(f, 0) for f in range(12)])
     assert runs["early"][0] == schedule, runs["early"][0]
-    assert runs["late"][0] == 0 and runs["serial"][0] == 0
+    assert runs["late"][0] == 0 and runs["serial"][0] == 0 and runs["one side stream"][0] == schedule
     _, ref, ref_dft = runs["serial"]
-    for name in ("early", "late"):
+    for name in ("early", "late", "one side stream"):
         _, fields, dft = runs[name]
         for c in range(6):
             assert np.array_equal(fields[c], ref[c]), (name, c)
