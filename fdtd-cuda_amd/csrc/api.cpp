@@ -167,6 +167,7 @@ DispDev make_disp(fdtd_sim *s, int parity) {
         d.h_inv_d = (float)(1.0 / dx);
         d.h_cont = (float)(2.0 * dt / dx);
         d.h_cur = (float)(dt * fdtd::kE0 * A.N_EQ / CE);
+        d.h_inv_neq = (float)(1.0 / A.N_EQ);
     }
     return d;
 }
@@ -1482,33 +1483,44 @@ int fdtd_get_field(fdtd_sim *s, int comp, float *out) {
 }
 
 int fdtd_get_slice(fdtd_sim *s, int comp, int axis, int index, float *out) {
-    if (!s || !out || comp < 0 || comp > 6 || axis < 0 || axis > 2) return FDTD_ERR_INVALID;
+    if (!s || !out || comp < 0 || comp > 9 || axis < 0 || axis > 2) return FDTD_ERR_INVALID;
     const Geom &g = s->g;
     const int n[3] = {g.NX, g.NY, g.NZ};
     if (index < 0 || index >= n[axis]) return set_err(s, FDTD_ERR_INVALID, "fdtd_get_slice: index outside the grid");
-    if (comp == 6) {
-        // hydrodynamic density perturbation N_EQ (n~x + n~y + n~z) / 3 at time
-        // level n+1 (the level the last step left in the n~^n buffer), as the
-        // ND snapshots average the three component densities (snapshot.cu:80-133);
-        // zero outside the metal box
-        if (!s->S.hydro || s->nbox <= 0) return set_err(s, FDTD_ERR_INVALID, "fdtd_get_slice: no hydrodynamic model");
-        const size_t nb = (size_t)s->nbox;
-        std::vector<float> nd(3 * nb);
-        HIPCHK(s, hipStreamSynchronize(s->stream));
-        for (int c = 0; c < 3; c++) HIPCHK(s, hipMemcpy(nd.data() + c * nb, s->h_ndp[c], nb * sizeof(float), hipMemcpyDeviceToHost));
+    if (comp >= 6) {
+        // state of the dispersive box, zero outside it.  6: hydrodynamic
+        // density perturbation N_EQ (n~x + n~y + n~z) / 3 at time level n+1 (the
+        // level the last step left in the n~^n buffer), as the ND snapshots
+        // average the three component densities (snapshot.cu:80-133).  7-9: the
+        // Drude pole of ex, ey, ez at level n+1 in the reference's units (P_d,
+        // the Px_d/Py_d/Pz_d arrays the Snap_in == 2 snapshots copy,
+        // main.cu:2962-2964): eps0 P~ of the local model, the electron velocity
+        // of the hydrodynamic one
+        if (comp == 6 && !s->S.hydro) return set_err(s, FDTD_ERR_INVALID, "fdtd_get_slice: no hydrodynamic model");
+        const size_t nb = (size_t)std::max(s->nbox, 0L);
+        std::vector<float> st;
+        if (nb) {
+            st.resize(3 * nb);
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            if (comp == 6)
+                for (int c = 0; c < 3; c++) HIPCHK(s, hipMemcpy(st.data() + c * nb, s->h_ndp[c], nb * sizeof(float), hipMemcpyDeviceToHost));
+            else
+                HIPCHK(s, hipMemcpy(st.data(), s->disp_st[comp - 7][2], nb * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        const double scale = comp == 6 ? s->S.ade.N_EQ / 3.0 : (s->S.hydro ? 1.0 : fdtd::kEP0);
         const int ua = axis == 0 ? 1 : 0, ub = axis == 2 ? 1 : 2;   // plane axes: out[u * n[ub] + v]
-        const double neq = s->S.ade.N_EQ;
         for (int u = 0; u < n[ua]; u++)
             for (int v = 0; v < n[ub]; v++) {
                 int x[3];
                 x[axis] = index; x[ua] = u; x[ub] = v;
                 float val = 0.f;
-                bool in = true;
+                bool in = nb > 0;
                 for (int a = 0; a < 3; a++) in = in && x[a] >= s->box_lo[a] && x[a] < s->box_lo[a] + s->box_n[a];
                 if (in) {
                     const size_t q = ((size_t)(x[0] - s->box_lo[0]) * s->box_n[1] + (x[1] - s->box_lo[1])) * s->box_n[2] +
                                      (x[2] - s->box_lo[2]);
-                    val = (float)(neq * ((double)nd[q] + nd[nb + q] + nd[2 * nb + q]) / 3.0);
+                    const double sum = comp == 6 ? (double)st[q] + st[nb + q] + st[2 * nb + q] : (double)st[q];
+                    val = (float)(scale * sum);
                 }
                 out[(size_t)u * n[ub] + v] = val;
             }

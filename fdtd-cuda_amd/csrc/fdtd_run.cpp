@@ -5,17 +5,20 @@
 // (main.cu:2724) and after step t, whenever t % (Tend/10) == 0, writes
 // ScatAbs.<t>.txt with the 19 columns of CalculateAbsScatt
 // (Spectral.cu:1153-1171).  This driver does the same with fdtd_step and
-// fdtd_flux.  With Snap_in != 0 it also writes, every t_skip steps, the
-// |E| plane at i = NX/2 in the format of SNAPSHOT_2D (snapshot.cu:25-60,
-// videofiles/YZ_sim.<n>.txt, every y_skip-th row and z_skip-th column, and
-// with the hydrodynamic model the density plane YZ_sim.<n>_N.txt); the
-// shipped reference leaves that call commented out (main.cu:2948-2951).
+// fdtd_flux.  Every t_skip steps it writes the snapshots of the reference's
+// loop (main.cu:2948-2990) in the formats of snapshot.cu: Snap_in 1 the
+// column ey(2, 2, k) and the 1D line (aux_sim.<n>.txt, aux_sim.<n>_2.txt);
+// 2 |P_d| of the Drude pole and the hydrodynamic density on the plane
+// i = NX/2 (videofiles/YZ_sim.<n>.txt, YZ_sim.<n+1>_N.txt); 3 the 1D files
+// and |E| on that plane; other values none.
 //
-//   fdtd_run [Input.txt] [--dir DIR] [--parse-only]
+//   fdtd_run [Input.txt] [--dir DIR] [--parse-only] [--xz]
 //
 // --parse-only prints the parsed parameters ("name value" lines, then the
 // tokens the solver does not use) and exits without touching the device.
-// Input.txt: one header token, then 43 "value name" pairs in the order
+// --xz also writes the XZ planes j = NY/2 (XZ_sim.<n>.txt, XZ_sim.<n>_N.txt,
+// every x_skip-th row), the branch the reference compiles out (snapshot.cu:32).
+// Input.txt: one header token, then 41 "value name" pairs in the order
 // below; the names are not checked, as in the reference.
 #include <sys/stat.h>
 
@@ -158,39 +161,104 @@ bool write_scatabs(const std::string &dir, int t, const fdtd_params &p, const st
     return std::fclose(f) == 0;
 }
 
-// YZ |E| snapshot (SNAPSHOT_2D, snapshot.cu:62-76): rows j, columns k.
-bool write_snapshot(fdtd_sim *sim, const fdtd_grid_info &gi, const std::string &dir, int n, int jskip, int kskip) {
-    const size_t plane = (size_t)gi.NY * gi.NZ;
-    std::vector<float> e[3];
-    for (int c = 0; c < 3; c++) {
-        e[c].resize(plane);
-        if (fdtd_get_slice(sim, c, 0, gi.NX / 2, e[c].data()) != FDTD_OK) return false;
+// Snapshot files of the reference (snapshot.cu), from planes fdtd_get_slice
+// copies off the device.  snapshot_count numbers the files and advances once
+// per SNAPSHOT_* call, as in the reference.
+struct Snapshots {
+    fdtd_sim *sim;
+    fdtd_grid_info gi;
+    std::string dir;
+    int xskip, yskip, zskip;
+    bool xz;       // also the XZ planes (the reference's XZ switch, snapshot.cu:32, 89)
+    int count = 0;
+
+    // comps' plane at i = NX/2 (axis 0, [j][k]) or j = NY/2 (axis 1, [i][k])
+    bool plane(int comp, int axis, std::vector<float> &out) const {
+        const int idx = axis == 0 ? gi.NX / 2 : gi.NY / 2;
+        out.assign((size_t)(axis == 0 ? gi.NY : gi.NX) * gi.NZ, 0.f);
+        return fdtd_get_slice(sim, comp, axis, idx, out.data()) == FDTD_OK;
     }
-    const std::string path = dir + "/videofiles/YZ_sim." + std::to_string(n) + ".txt";
-    FILE *f = std::fopen(path.c_str(), "w");
-    if (!f) return false;
-    for (int j = 0; j < gi.NY; j += jskip) {
-        for (int k = 0; k < gi.NZ; k += kskip) {
-            const size_t q = (size_t)j * gi.NZ + k;
-            const double v = (double)e[0][q] * e[0][q] + (double)e[1][q] * e[1][q] + (double)e[2][q] * e[2][q];
-            std::fprintf(f, "%e ", std::sqrt(v));
+    // rows every rskip-th of n, columns every zskip-th: sqrt of the sum of squares
+    // of the components (mag) or their value (one component), "%e " separated
+    bool write(const std::string &path, const std::vector<float> *c, int ncomp, int n, int rskip) const {
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) return false;
+        for (int r = 0; r < n; r += rskip) {
+            for (int k = 0; k < gi.NZ; k += zskip) {
+                const size_t q = (size_t)r * gi.NZ + k;
+                double v;
+                if (ncomp == 3) v = std::sqrt((double)c[0][q] * c[0][q] + (double)c[1][q] * c[1][q] + (double)c[2][q] * c[2][q]);
+                else v = c[0][q];
+                std::fprintf(f, "%e ", v);
+            }
+            std::fprintf(f, "\n");
         }
-        std::fprintf(f, "\n");
+        return std::fclose(f) == 0;
     }
-    if (std::fclose(f) != 0) return false;
-    if (!gi.hydro) return true;
-    // with the hydrodynamic model also the density plane (snapshot.cu:120-133,
-    // videofiles/YZ_sim.<n>_N.txt: (NDx + NDy + NDz) / 3)
-    std::vector<float> nd(plane);
-    if (fdtd_get_slice(sim, 6, 0, gi.NX / 2, nd.data()) != FDTD_OK) return false;
-    f = std::fopen((dir + "/videofiles/YZ_sim." + std::to_string(n) + "_N.txt").c_str(), "w");
-    if (!f) return false;
-    for (int j = 0; j < gi.NY; j += jskip) {
-        for (int k = 0; k < gi.NZ; k += kskip) std::fprintf(f, "%e ", (double)nd[(size_t)j * gi.NZ + k]);
-        std::fprintf(f, "\n");
+    // SNAPSHOT_2D (snapshot.cu:25-76): |E| (first = 0) or |P_d| (first = 7, the
+    // Snap_in == 2 copies of Px_d, Py_d, Pz_d into ex, ey, ez, main.cu:2962-2964)
+    // on the YZ plane i = NX/2 (rows j every y_skip-th) and, with xz, the XZ
+    // plane j = NY/2 (rows i every x_skip-th); columns k every z_skip-th
+    bool shot_2d(int first) {
+        const std::string n = std::to_string(count);
+        std::vector<float> c[3];
+        if (xz) {
+            for (int q = 0; q < 3; q++)
+                if (!plane(first + q, 1, c[q])) return false;
+            if (!write(dir + "/videofiles/XZ_sim." + n + ".txt", c, 3, gi.NX, xskip)) return false;
+        }
+        for (int q = 0; q < 3; q++)
+            if (!plane(first + q, 0, c[q])) return false;
+        if (!write(dir + "/videofiles/YZ_sim." + n + ".txt", c, 3, gi.NY, yskip)) return false;
+        count++;
+        return true;
     }
-    return std::fclose(f) == 0;
-}
+    // SNAPSHOT_2D_N (snapshot.cu:80-133): (NDx + NDy + NDz) / 3; zero without
+    // the hydrodynamic model
+    bool shot_2d_n() {
+        const std::string n = std::to_string(count);
+        std::vector<float> c[1];
+        for (int axis = xz ? 1 : 0; axis >= 0; axis--) {
+            const int rows = axis == 0 ? gi.NY : gi.NX;
+            if (gi.hydro) {
+                if (!plane(6, axis, c[0])) return false;
+            } else {
+                c[0].assign((size_t)rows * gi.NZ, 0.f);
+            }
+            const std::string name = std::string(axis == 0 ? "YZ" : "XZ") + "_sim." + n + "_N.txt";
+            if (!write(dir + "/videofiles/" + name, c, 1, rows, axis == 0 ? yskip : xskip)) return false;
+        }
+        count++;
+        return true;
+    }
+    // SNAPSHOT_1D (snapshot.cu:137-175): ey on the column i = j = 2 (every
+    // z_skip-th k) to aux_sim.<n>.txt and the 1D line e_inc (every z_skip-th
+    // cell) to aux_sim.<n>_2.txt, one value per line, in the run directory
+    bool shot_1d() {
+        const std::string n = std::to_string(count);
+        std::vector<float> ey((size_t)gi.NY * gi.NZ, 0.f);   // plane i = 2, row j = 2 read below
+        if (fdtd_get_slice(sim, 1, 0, 2, ey.data()) != FDTD_OK) return false;
+        FILE *f = std::fopen((dir + "/aux_sim." + n + ".txt").c_str(), "w");
+        if (!f) return false;
+        for (int k = 0; k < gi.NZ; k += zskip) std::fprintf(f, "%e\n", (double)ey[(size_t)2 * gi.NZ + k]);
+        if (std::fclose(f) != 0) return false;
+        std::vector<double> e(gi.inc_length), h(gi.inc_length);
+        if (fdtd_get_line(sim, e.data(), h.data()) != FDTD_OK) return false;
+        f = std::fopen((dir + "/aux_sim." + n + "_2.txt").c_str(), "w");
+        if (!f) return false;
+        for (int m = 0; m < gi.inc_length; m += zskip) std::fprintf(f, "%e\n", e[m]);
+        if (std::fclose(f) != 0) return false;
+        count++;
+        return true;
+    }
+    // the snapshot block of the reference loop for one Snap_in value (main.cu:2948-2990)
+    bool take(int snap_in) {
+        if (snap_in == 1) return shot_1d();
+        if (snap_in == 2) return shot_2d(7) && shot_2d_n();
+        if (snap_in == 3) return shot_1d() && shot_2d(0);
+        return true;
+    }
+};
 
 int unused_int(const std::vector<std::pair<std::string, std::string>> &u, const char *name, int dflt) {
     for (const auto &x : u)
@@ -208,9 +276,10 @@ int fail(const std::string &m) {
 int main(int argc, char **argv) {
     const char *input = "Input.txt";
     std::string dir = ".";
-    bool parse_only = false;
+    bool parse_only = false, xz = false;
     for (int a = 1; a < argc; a++) {
         if (!std::strcmp(argv[a], "--parse-only")) parse_only = true;
+        else if (!std::strcmp(argv[a], "--xz")) xz = true;
         else if (!std::strcmp(argv[a], "--dir") && a + 1 < argc) dir = argv[++a];
         else if (argv[a][0] == '-') return fail(std::string("unknown option ") + argv[a]);
         else input = argv[a];
@@ -227,6 +296,11 @@ int main(int argc, char **argv) {
         for (const auto &u : unused) std::printf("unused %s %s\n", u.first.c_str(), u.second.c_str());
         return 0;
     }
+    // the dispersive half-space and slab of inf_disp_slab 1 and 2 (setup.cu:156-200)
+    // are not built by this driver: refuse rather than run another geometry
+    if (unused_int(unused, "inf_disp_slab", 0) != 0)
+        return fail("inf_disp_slab != 0 (dispersive half-space or slab) is not supported; "
+                    "periodic runs take a metal layer through fdtd_add_layer");
     fdtd_sim *sim = nullptr;
     if (fdtd_create(&p, &sim) != FDTD_OK) return fail(fdtd_last_error(nullptr));
     auto bail = [&](const char *what) {
@@ -250,13 +324,14 @@ int main(int argc, char **argv) {
     // and, with Snap_in != 0, a snapshot when t % t_skip == 0
     const int every = p.Tend / 10 > 0 ? p.Tend / 10 : 1;
     const int snap_in = unused_int(unused, "Snap_in", 0), t_skip = unused_int(unused, "t_skip", 0);
-    const int jskip = std::max(1, unused_int(unused, "y_skip", 1)), kskip = std::max(1, unused_int(unused, "z_skip", 1));
-    const bool snap = snap_in != 0 && t_skip > 0;
-    if (snap) {
+    Snapshots shots{sim, gi, dir, std::max(1, unused_int(unused, "x_skip", 1)), std::max(1, unused_int(unused, "y_skip", 1)),
+                    std::max(1, unused_int(unused, "z_skip", 1)), xz};
+    const bool snap = snap_in >= 1 && snap_in <= 3 && t_skip > 0;
+    if (snap && snap_in != 1) {
         const std::string vd = dir + "/videofiles";
         if (::mkdir(vd.c_str(), 0755) != 0 && errno != EEXIST) return bail("videofiles");
     }
-    int done = 0, nsnap = 0;
+    int done = 0;
     for (int t = 0; t <= p.Tend; t++) {
         const bool spectra = t % every == 0 && p.StaticField == 0, shot = snap && t % t_skip == 0;
         if (!spectra && !shot) continue;
@@ -267,7 +342,7 @@ int main(int argc, char **argv) {
             if (!write_scatabs(dir, t, p, freq, einc, raw)) return bail("ScatAbs");
             std::printf("t = %d: ScatAbs.%d.txt\n", t, t);
         }
-        if (shot && !write_snapshot(sim, gi, dir, nsnap++, jskip, kskip)) return bail("snapshot");
+        if (shot && !shots.take(snap_in)) return bail("snapshot");
     }
     if (done < p.Tend + 1 && fdtd_step(sim, p.Tend + 1 - done) != FDTD_OK) return bail("fdtd_step");
     if (fdtd_sync(sim) != FDTD_OK) return bail("fdtd_sync");

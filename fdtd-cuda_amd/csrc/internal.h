@@ -121,6 +121,7 @@ struct DispDev {
     float h_d1m1, h_d3, h_d4;       // v^{n+1} = v + (d1-1) J + d3 E^n + d4 E^{n-1} + h_nl (gd + gd2) / cbrt(1 + n~)
     float h_nl, h_inv_d, h_cont;    // h_nl = beta^2/dd, 1/dx, 2 dt/dx
     float h_cur;                    // dt e0 N_EQ / C_E: current term of the E update
+    float h_inv_neq;                // 1 / N_EQ
     // NONLOCAL == 1: gd[c] = grad(div P~_d^n) at each dispersive cell of component c
     // (k_nonlocal_gd, before the box update); E^{n+1} -= nl_e gd, J~_d^{n+1} += nl_p gd
     float *gd[3];

@@ -1603,7 +1603,8 @@ __global__ void k_hydro_v(DispDev d) {
 // n~^{n+1} = n~^{n-1} - 2 dt/dx * (div(n~ v) + div v), the normal-axis terms
 // halved; earlier components are already at level n+1 (x -> y -> z order).
 // The z update keeps the reference's 0.5 (Vz1 - Vz2) in place of the
-// n~-weighted normal term (main.cu:1837).
+// density-weighted normal term (main.cu:1837); that term is in absolute
+// density units there, so here it is scaled by 1/N_EQ.
 __global__ void k_hydro_cont(DispDev d, int c) {
     const long nb = (long)d.n[0] * d.n[1] * d.n[2];
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1636,7 +1637,9 @@ __global__ void k_hydro_cont(DispDev d, int c) {
         const float Nx1 = 0.5f * (Nx[idx + ok] + Nx[idx]), Nx2 = 0.5f * (Nx[idx - oi + ok] + Nx[idx - oi]);
         const float Vy1 = 0.5f * (Py[idx + ok] + Py[idx]), Vy2 = 0.5f * (Py[idx - oj + ok] + Py[idx - oj]);
         const float Ny1 = 0.5f * (Ny[idx + ok] + Ny[idx]), Ny2 = 0.5f * (Ny[idx - oj + ok] + Ny[idx - oj]);
-        r = (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + 0.5f * (Vz1 - Vz2)
+        // the reference's 0.5 (Vz1 - Vz2) stands where 0.5 (Nz1 Vz1 - Nz2 Vz2) belongs and is
+        // not weighted by the density: in units of N_EQ it carries 1/N_EQ
+        r = (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + 0.5f * (Vz1 - Vz2) * d.h_inv_neq
           + ((Vx1 - Vx2) + (Vy1 - Vy2) + 0.5f * (Vz1 - Vz2));
     }
     d.nd[c][idx] = d.nd[c][idx] - d.h_cont * r;

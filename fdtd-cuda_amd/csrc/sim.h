@@ -114,7 +114,7 @@ struct fdtd_sim {
     int tiles = 0, stiles = 0;
     int schunk = 16;                // x chunk of the side slabs
     int adj_schunk = 8;             // x chunk of the side slabs in the transposed step (FDTD_ADJ_SCHUNK)
-    int adj_stj = 8;                // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ)
+    int adj_stj = 16;               // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ; 4, 8, 16 measured ratios 3.14, 3.16, 3.07 over 2000 steps)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
     bool timing = false;

@@ -227,7 +227,9 @@ int  fdtd_get_field(fdtd_sim *sim, int comp, float *out);
  * out[i*NY + j] (augmented grid).  comp 0-5 as fdtd_get_field; comp 6 (with
  * the hydrodynamic model) the density perturbation ND = N_EQ (n~x + n~y +
  * n~z) / 3 in m^-3, zero outside the metal box (the ND snapshots,
- * snapshot.cu:80-133). */
+ * snapshot.cu:80-133); comp 7-9 the Drude pole of ex, ey, ez (Px_d, Py_d,
+ * Pz_d in the reference's units: the polarisation, or with the hydrodynamic
+ * model the electron velocity; main.cu:2962-2964), zero outside the metal box. */
 int  fdtd_get_slice(fdtd_sim *sim, int comp, int axis, int index, float *out);
 int  fdtd_set_field(fdtd_sim *sim, int comp, const float *in);
 

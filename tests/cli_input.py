@@ -1,4 +1,4 @@
-"""Input.txt writer for the fdtd_run tests: one header token, then the 43
+"""Input.txt writer for the fdtd_run tests: one header token, then the 41
 "value name" pairs in the order READ_DATA_FILE reads them (setup.cu:17-58)."""
 
 ORDER = [

@@ -58,3 +58,13 @@ def test_truncated_and_malformed_inputs_fail_cleanly(tmp_path):
     assert r.returncode != 0 and "bad integer for Tend" in r.stderr
     r = parse(tmp_path / "missing.txt")
     assert r.returncode != 0 and "cannot open" in r.stderr
+
+
+def test_dispersive_slab_inputs_are_refused(tmp_path):
+    """inf_disp_slab 1 or 2 asks for a dispersive half-space or slab
+    (setup.cu:156-200), which fdtd_run does not build: it exits with an error
+    before touching the device instead of running another geometry."""
+    p = tmp_path / "Input.txt"
+    write_input(p, inf_disp_slab=1)
+    r = subprocess.run([EXE, str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "inf_disp_slab" in r.stderr

@@ -280,3 +280,31 @@ def test_row_tiles_per_block_bit_identical(ny, rows, tiles, stiles, monkeypatch)
     assert np.abs(out[tiles, stiles][0]).max() > 0
     for c in range(6):
         assert np.array_equal(out[1, 1][c], out[tiles, stiles][c]), c
+
+
+@pytest.mark.parametrize("phi", [0.0, 90.0])
+def test_hydrodynamic_density_and_drude_pole_planes(phi):
+    """The hydrodynamic state read back through fdtd_get_slice against the
+    oracle: the density (NDx + NDy + NDz)/3 (component 6) and the Drude pole,
+    here the electron velocity (components 7-9), on the planes through the
+    sphere's centre along each axis, after the pulse has driven the sphere.
+    The density barely touches the fields (it enters as 1 + n~), so the field
+    tests do not see it; this one does (within 1e-4 of the largest value)."""
+    p = small_params(O, steps=300)
+    p.NONLOCAL = 3
+    p.hydro_beta = 1.27e6
+    p.inc_phi = phi
+    o, s = run_pair(p, 300, True)
+    c = [n // 2 for n in o.shape]
+    nd = sum(o.get_state(0, q) for q in range(3)) / 3.0
+    pd = [o.get_state(2, q) for q in range(3)]
+    errs = {}
+    for axis in range(3):
+        want = np.take(nd, c[axis], axis=axis)
+        errs["N axis %d" % axis] = float(np.abs(s.get_slice(6, axis, c[axis]) - want).max() / np.abs(nd).max())
+        for q in range(3):
+            w = np.take(pd[q], c[axis], axis=axis)
+            errs["P%d axis %d" % (q, axis)] = float(np.abs(s.get_slice(7 + q, axis, c[axis]) - w).max() / np.abs(pd[q]).max())
+    print("phi %g density / velocity planes:" % phi, {k: "%.1e" % v for k, v in errs.items()}, flush=True)
+    assert np.abs(nd).max() > 0
+    assert max(errs.values()) < 1e-4
