@@ -107,7 +107,8 @@ struct fdtd_sim {
     bool early = true;              // TFSF fix and dispersive box ahead of the join when their cells allow (FDTD_EARLY=0 to keep them after)
     std::vector<float> t_ike[3], t_ikh[3], t_be[3], t_ce[3], t_bh[3], t_ch[3];   // host copies of the CPML tables
     int rows = 2, cchunk = 0;       // own rows per thread and x chunk of the row-blocked kernel (clean box;
-                                    // 0: 32 with two-tile blocks, else 16)
+                                    // 0: 32 with two-tile blocks when that still leaves at least 512
+                                    // clean-box blocks, else 16; see plan_rows)
     // row tiles per block of the clean-box launch (FDTD_TILES: 1-3) and of the
     // side-slab launches (FDTD_STILES: 1 or 2); 0: two when the z column takes
     // at most 5 waves, else one (see rows_tiles)

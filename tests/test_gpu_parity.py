@@ -115,20 +115,39 @@ def test_fused_equals_split():
     assert max(errs) < 1e-5
 
 
-@pytest.mark.parametrize("tj,ichunk,rows,cchunk", [(4, 7, 2, 5), (8, 7, 3, 3), (16, 5, 4, 9), (8, 32, 2, 64)])
-def test_tile_shapes_agree(tj, ichunk, rows, cchunk, monkeypatch):
-    """Tile shapes of both step kernels (general pieces: TJ rows x 63 lanes,
-    x chunk; clean region: rows per thread, x chunk) give the split path's
+@pytest.mark.parametrize("ctj,rows,cchunk", [(4, 2, 5), (8, 3, 3), (16, 2, 9), (8, 2, 64)])
+def test_tile_shapes_agree(ctj, rows, cchunk, monkeypatch):
+    """Tile shapes of the row path (rows per thread, clean-box x chunk) and of
+    the general kernel on the corner columns (FDTD_CTJ tile rows) give the
+    split path's fields."""
+    p = small_params(O, steps=350)
+    ref = fdtd_amd.Simulation(p)
+    ref.split_step(350)
+    for k, v in (("FDTD_CTJ", ctj), ("FDTD_ROWS", rows), ("FDTD_CCHUNK", cchunk)):
+        monkeypatch.setenv(k, str(v))
+    s = fdtd_amd.Simulation(p)
+    assert s.get_info().fused_path == 1
+    s.step(350)
+    errs = group_errors([s.get_field(c) for c in range(6)], [ref.get_field(c) for c in range(6)])
+    print(f"tile ctj={ctj} rows={rows} cchunk={cchunk} errors={['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < 1e-5
+
+
+@pytest.mark.parametrize("tj,ichunk", [(4, 7), (8, 3), (16, 5), (8, 32)])
+def test_general_kernel_tile_shapes_agree(tj, ichunk, monkeypatch):
+    """Every tile of the general fused kernel over the whole grid
+    (FDTD_PATH=general: TJ rows per block, x chunk) gives the split path's
     fields."""
     p = small_params(O, steps=350)
     ref = fdtd_amd.Simulation(p)
     ref.split_step(350)
-    for k, v in (("FDTD_TJ", tj), ("FDTD_ICHUNK", ichunk), ("FDTD_ROWS", rows), ("FDTD_CCHUNK", cchunk)):
+    for k, v in (("FDTD_PATH", "general"), ("FDTD_TJ", tj), ("FDTD_ICHUNK", ichunk)):
         monkeypatch.setenv(k, str(v))
     s = fdtd_amd.Simulation(p)
+    assert s.get_info().fused_path == 0
     s.step(350)
     errs = group_errors([s.get_field(c) for c in range(6)], [ref.get_field(c) for c in range(6)])
-    print(f"tile tj={tj} ichunk={ichunk} rows={rows} cchunk={cchunk} errors={['%.2e' % e for e in errs]}", flush=True)
+    print(f"general tj={tj} ichunk={ichunk} errors={['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) < 1e-5
 
 
