@@ -159,15 +159,29 @@ DispDev make_disp(fdtd_sim *s, int parity) {
     if (S.hydro) {
         const double dt = S.dt, dx = S.P.dx;
         for (int c = 0; c < 3; c++) { d.vnew[c] = s->h_vnew[c]; d.nd[c] = s->h_nd[c]; d.nd_prev[c] = s->h_ndp[c]; }
-        d.h_d1m1 = (float)(A.d[0] - 1.0);
-        d.h_d3 = (float)A.d[2];
-        d.h_d4 = (float)A.d[3];
+        d.h_d1m1 = A.d[0] - 1.0;
+        d.h_d3 = A.d[2];
+        d.h_d4 = A.d[3];
         // dNL / N_EQ^{1/3} = beta^2 / dd with n~ = ND / N_EQ (setup.cu:1362)
-        d.h_nl = (float)(A.dNL / std::cbrt(A.N_EQ));
-        d.h_inv_d = (float)(1.0 / dx);
-        d.h_cont = (float)(2.0 * dt / dx);
-        d.h_cur = (float)(dt * fdtd::kE0 * A.N_EQ / CE);
-        d.h_inv_neq = (float)(1.0 / A.N_EQ);
+        d.h_nl = A.dNL / std::cbrt(A.N_EQ);
+        d.h_inv_d = 1.0 / dx;
+        d.h_cont = 2.0 * dt / dx;
+        d.h_cur = dt * fdtd::kE0 * A.N_EQ / CE;
+        d.h_inv_neq = 1.0 / A.N_EQ;
+        // the E update and CP poles of the box in FP64 (the float set above, unrounded)
+        d.ha0 = S.dt / (fdtd::kZ0 * CE);
+        d.ha1 = A.C_E1 / CE;
+        d.ha2 = A.C_E2 / CE;
+        for (int q = 0; q < 3; q++) {
+            const double sum = q == 0 ? 0.0 : (k[q][0] - 1.0) + k[q][1];
+            d.he0[q] = sum * e0 / CE;
+            d.he1[q] = -k[q][1] * e0 / CE;
+            d.hg0[q] = sum;
+            d.hg1[q] = -k[q][1];
+            d.hp3[q] = k[q][2] / e0;
+            d.hp4[q] = k[q][3] / e0;
+            d.hp5[q] = k[q][4] / e0;
+        }
     }
     return d;
 }

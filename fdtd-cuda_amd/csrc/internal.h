@@ -114,14 +114,17 @@ struct DispDev {
     float a0, a1, a2;
     float e0[3], e1[3], g0[3], g1[3], p3[3], p4[3], p5[3];
     // hydrodynamic model (pole 0 becomes the electron velocity v, st[2] = v^n,
-    // st[3] = v^n - v^{n-1}; charge stored as n~ = ND / N_EQ)
+    // st[3] = v^n - v^{n-1}; charge stored as n~ = ND / N_EQ); its kernels
+    // compute in FP64 with the FP64 coefficients below (ha*, he*, hg*, hp*:
+    // those of the E update and CP poles above)
     int hydro;
     float *vnew[3];                 // v^{n+1} from the momentum update
     float *nd[3], *nd_prev[3];      // n~^{n-1} (updated to n~^{n+1}), n~^n
-    float h_d1m1, h_d3, h_d4;       // v^{n+1} = v + (d1-1) J + d3 E^n + d4 E^{n-1} + h_nl (gd + gd2) / cbrt(1 + n~)
-    float h_nl, h_inv_d, h_cont;    // h_nl = beta^2/dd, 1/dx, 2 dt/dx
-    float h_cur;                    // dt e0 N_EQ / C_E: current term of the E update
-    float h_inv_neq;                // 1 / N_EQ
+    double h_d1m1, h_d3, h_d4;      // v^{n+1} = v + (d1-1) J + d3 E^n + d4 E^{n-1} + h_nl (gd + gd2) / cbrt(1 + n~)
+    double h_nl, h_inv_d, h_cont;   // h_nl = beta^2/dd, 1/dx, 2 dt/dx
+    double h_cur;                   // dt e0 N_EQ / C_E: current term of the E update
+    double h_inv_neq;               // 1 / N_EQ
+    double ha0, ha1, ha2, he0[3], he1[3], hg0[3], hg1[3], hp3[3], hp4[3], hp5[3];
     // NONLOCAL == 1: gd[c] = grad(div P~_d^n) at each dispersive cell of component c
     // (k_nonlocal_gd, before the box update); E^{n+1} -= nl_e gd, J~_d^{n+1} += nl_p gd
     float *gd[3];

@@ -77,15 +77,15 @@ def test_dimer_20000_steps_against_fp64_fixture():
 @pytest.mark.timeout(600)
 @pytest.mark.parametrize("exact", [0, 1])
 def test_hydrodynamic_96_40000_steps_against_fp64_fixture(exact):
-    """exact = 0: split-BF16 MFMA contraction (two parts); 1: FP64 contraction.
-    The absorption is the small difference of six face fluxes that each carry
-    the incident power (some 30 to 100 times larger), so it amplifies their
-    relative error: with FP32 fields the face fluxes sit within 5e-6 of FP64
-    and the absorption and extinction spectra within 6e-4, the same with
-    either contraction (the contraction is not the source).  The field left
-    at step 40,000 is a residual about 100 times below the incident amplitude,
-    with FP32 differences up to 3e-3 of that amplitude.  Bars: face fluxes
-    1e-4, spectra 1e-3, field lines 5e-3 of MAX_AMP."""
+    """The hydrodynamic workload's 96^3 copy over its 40,000 steps with the
+    configured pulse, against the FP64 fixture; exact = 0: split-BF16 MFMA
+    contraction (two parts), 1: FP64 contraction.  The absorption is the
+    small difference of six face fluxes that each carry the incident power
+    (some 30 to 100 times larger), so it amplifies their error; the
+    hydrodynamic box computes in FP64 for that reason (in FP32 arithmetic the
+    spectra sat at 5.4e-4).  Bars: spectra 1e-4 (measured 3.9e-5 and 8e-6),
+    face fluxes 1e-5, field lines 5e-3 of MAX_AMP (the field left at step
+    40,000 is a residual about 100 times below the incident amplitude)."""
     ref = load("hydro96_fp64.npz")
     p = W.hydro(G, n=96, dx=1e-9)
     p.dft_exact = exact
@@ -107,6 +107,6 @@ def test_hydrodynamic_96_40000_steps_against_fp64_fixture(exact):
     lines = line_errors(s, ref, (T // 2, T // 2, T // 2), scale=p.MAX_AMP)
     print("hydro 96^3, %d steps, dft_exact=%d:" % (steps, exact), {k: "%.2e" % v for k, v in errs.items()},
           "field lines", ["%.1e" % x for x in lines], flush=True)
-    assert errs["raw"] < 1e-4
+    assert errs["raw"] < 1e-5
     assert max(lines) < 5e-3
-    assert max(errs.values()) < 1e-3
+    assert max(errs.values()) < 1e-4
