@@ -68,6 +68,9 @@ struct fdtd_batch {
     // edges in a captured period
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_line = nullptr;
+    // y-slab and general-kernel launches on streams of their own (FDTD_SIDE_STREAMS=3; else all on stream2)
+    hipStream_t stream3 = nullptr, stream4 = nullptr;
+    hipEvent_t ev_join3 = nullptr, ev_join4 = nullptr;
     // TFSF fix-ups (every row-path simulation's planes in its clean box) and
     // dispersive boxes (every simulation on the row path, every box in its
     // clean box) follow the clean boxes without waiting for the side pieces
@@ -389,7 +392,8 @@ int upload_items(fdtd_batch *b) {
 // One step for every simulation: par = parity of its index, off = its offset
 // from dyn's step, sidx = ring row offset of its sample or -1.  The lines,
 // the side slabs and the general-kernel launches (corner columns, or whole
-// grids off the row path) run on the second stream beside the clean boxes:
+// grids off the row path) run on the second stream beside the clean boxes
+// (with FDTD_SIDE_STREAMS=3 the y slabs and the general kernel on two more):
 // each writes only its own cells and reads the previous step's fields.  The
 // TFSF fix-ups and the dispersive boxes follow the clean boxes on the main
 // stream, ahead of the join where tf_early and dp_early allow (the same
@@ -397,18 +401,29 @@ int upload_items(fdtd_batch *b) {
 int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     const int n = (int)b->sims.size();
     const bool fork = b->stream2 != nullptr;
+    const bool three = fork && b->stream3 && b->stream4;
     hipStream_t side = fork ? b->stream2 : b->stream;
+    hipStream_t side_y = three ? b->stream3 : side, side_g = three ? b->stream4 : side;
     const bool tf_early = fork && b->tf_early, dp_early = tf_early && b->dp_early;
     if (fork) {
         BCHK(b, hipEventRecord(b->ev_fork, b->stream));
         BCHK(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+        if (three) {
+            BCHK(b, hipStreamWaitEvent(b->stream3, b->ev_fork, 0));
+            BCHK(b, hipStreamWaitEvent(b->stream4, b->ev_fork, 0));
+        }
     }
     BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, side));
     if (tf_early) BCHK(b, hipEventRecord(b->ev_line, side));
     for (const Launch &l : b->launches) {
         const int nb = (int)l.map.size();
-        if (l.type == Launch::Rows && l.c != 0) BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, side));
-        if (l.type == Launch::Fused) BCHK(b, launch_fused_batch(b->fused[par], l.d, nb, l.a, side));
+        if (l.type == Launch::Rows && l.c != 0)
+            BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, l.c == 2 ? side_y : side));
+        if (l.type == Launch::Fused) BCHK(b, launch_fused_batch(b->fused[par], l.d, nb, l.a, side_g));
+    }
+    if (three) {
+        BCHK(b, hipEventRecord(b->ev_join3, b->stream3));
+        BCHK(b, hipEventRecord(b->ev_join4, b->stream4));
     }
     for (const Launch &l : b->launches)
         if (l.type == Launch::Rows && l.c == 0)
@@ -430,6 +445,10 @@ int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     if (fork) {
         BCHK(b, hipEventRecord(b->ev_join, b->stream2));
         BCHK(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    }
+    if (three) {
+        BCHK(b, hipStreamWaitEvent(b->stream, b->ev_join3, 0));
+        BCHK(b, hipStreamWaitEvent(b->stream, b->ev_join4, 0));
     }
     return post(false);
 }
@@ -540,6 +559,15 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
             hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_line, hipEventDisableTiming) != hipSuccess)
             return fail(berr(b, FDTD_ERR_DEVICE, "second stream or events could not be created"));
+        // one side stream by default: the batch's launches already fill the device
+        // (32-rod sweep: 51.6-52.0 k Mcell-updates/s with one, 51.0-51.2 k with three)
+        const char *ss = std::getenv("FDTD_SIDE_STREAMS");
+        if (ss && std::atoi(ss) == 3 &&
+            (hipStreamCreateWithFlags(&b->stream3, hipStreamNonBlocking) != hipSuccess ||
+             hipStreamCreateWithFlags(&b->stream4, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&b->ev_join4, hipEventDisableTiming) != hipSuccess))
+            return fail(berr(b, FDTD_ERR_DEVICE, "side streams or events could not be created"));
     }
     int rc;
     for (int par = 0; par < 2; par++) {
@@ -635,13 +663,19 @@ int fdtd_batch_destroy(fdtd_batch *b) {
     if (!b) return FDTD_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+    for (hipStream_t q : {b->stream3, b->stream4})
+        if (q) (void)hipStreamSynchronize(q);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     if (b->ev_line) (void)hipEventDestroy(b->ev_line);
+    if (b->ev_join3) (void)hipEventDestroy(b->ev_join3);
+    if (b->ev_join4) (void)hipEventDestroy(b->ev_join4);
     if (b->stream2) (void)hipStreamDestroy(b->stream2);
+    if (b->stream3) (void)hipStreamDestroy(b->stream3);
+    if (b->stream4) (void)hipStreamDestroy(b->stream4);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return FDTD_OK;
