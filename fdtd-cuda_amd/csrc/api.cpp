@@ -1092,7 +1092,7 @@ namespace {
 // One transposed step: adjoint level pa -> 1 - pa, psi_E pe, psi_H ph.  On
 // the row path the clean box runs on the row-blocked kernel (variant 3) with
 // the dispersive-cell correction after it, and the side slabs and corner
-// columns on the general transposed kernel beside it on the second stream.
+// columns on the general transposed kernel beside it on the side streams.
 int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
     const Setup &S = s->S;
     AdjDev av{};
@@ -1115,17 +1115,32 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
         return FDTD_OK;
     }
     const bool fork = s->stream2 != nullptr;
+    // with three side streams the x slabs, the y slabs and the corner columns each get one
+    const bool three = fork && s->stream3 && s->stream4;
     hipStream_t side = fork ? s->stream2 : s->stream;
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+        if (three) {
+            HIPCHK(s, hipStreamWaitEvent(s->stream3, s->ev_fork, 0));
+            HIPCHK(s, hipStreamWaitEvent(s->stream4, s->ev_fork, 0));
+        }
     }
-    // the slabs are thin: one launch for all four, short x chunks so they fill the device
-    int slabs[4][4], ns = 0;
-    for (int q = 0; q < s->nxbox; q++) std::memcpy(slabs[ns++], s->xbox[q], sizeof(slabs[0]));
-    for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
-    if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
-    if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, side));
+    // the slabs are thin: short x chunks so they fill the device (one launch for
+    // all four on a single side stream)
+    if (three) {
+        if (s->nxbox) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, s->xbox, s->nxbox, s->stream2));
+        if (s->nybox) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, s->ybox, s->nybox, s->stream3));
+        if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, s->stream4));
+        HIPCHK(s, hipEventRecord(s->ev_join3, s->stream3));
+        HIPCHK(s, hipEventRecord(s->ev_join4, s->stream4));
+    } else {
+        int slabs[4][4], ns = 0;
+        for (int q = 0; q < s->nxbox; q++) std::memcpy(slabs[ns++], s->xbox[q], sizeof(slabs[0]));
+        for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
+        if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
+        if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, side));
+    }
     if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     RowsArgs r = make_rows(s);
     const long fb = s->fbase;
@@ -1145,6 +1160,10 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
     HIPCHK(s, launch_step_rows(r, s->rows, 3, s->stream));
     if (s->nbox) HIPCHK(s, launch_adj_box_fix(av, s->stream));
     if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    if (three) {
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join3, 0));
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join4, 0));
+    }
     return FDTD_OK;
 }
 
