@@ -458,6 +458,7 @@ int plan_rows(fdtd_sim *s) {
             hipEventCreateWithFlags(&s->ev_line, hipEventDisableTiming) != hipSuccess)
             return set_err(s, FDTD_ERR_DEVICE, "second stream or events could not be created");
         if (const char *v = std::getenv("FDTD_SIDE_STREAMS")) s->side_streams = std::atoi(v) == 1 ? 1 : 3;
+        if (const char *v = std::getenv("FDTD_CHAIN_JOIN")) s->chain_join = std::atoi(v) != 0;
         if (s->side_streams == 3 &&
             (hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking) != hipSuccess ||
              hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking) != hipSuccess ||
@@ -624,11 +625,16 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     }
     // thin columns: short x chunks, 4 rows per block (tile rows 4, 8, 16 measured 0.381, 0.386, 0.386 ms per step)
     if (s->ncorner) HIPCHK(s, launch_fused_step(d, s->ctj, 2, s->cbox4, s->ncorner, side_c));
-    if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     if (three) {
         HIPCHK(s, hipEventRecord(s->ev_join3, s->stream3));
         HIPCHK(s, hipEventRecord(s->ev_join4, s->stream4));
+        if (s->chain_join) {
+            // the y slabs and corners join the x-slab stream, so the main stream waits once
+            HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_join3, 0));
+            HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_join4, 0));
+        }
     }
+    if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
     r.tiles = s->tiles * rows_waves(s->g.NZ) <= 16 ? s->tiles : 1;
@@ -649,7 +655,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
         *samp_done = merge_samp;
     }
     if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
-    if (three) {
+    if (three && !s->chain_join) {
         HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join3, 0));
         HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join4, 0));
     }

@@ -102,6 +102,7 @@ struct fdtd_sim {
     // side pieces of fdtd_step on separate streams (FDTD_SIDE_STREAMS: 1 = all on stream2, 3 = x
     // slabs after the line on stream2, y slabs on stream3, corner columns on stream4)
     int side_streams = 3;
+    bool chain_join = true;         // side streams join through stream2 (FDTD_CHAIN_JOIN=0: each joins the main stream)
     hipStream_t stream3 = nullptr, stream4 = nullptr;
     hipEvent_t ev_join3 = nullptr, ev_join4 = nullptr;
     bool early = true;              // TFSF fix and dispersive box ahead of the join when their cells allow (FDTD_EARLY=0 to keep them after)
