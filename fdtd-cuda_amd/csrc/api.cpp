@@ -459,6 +459,7 @@ int plan_rows(fdtd_sim *s) {
             return set_err(s, FDTD_ERR_DEVICE, "second stream or events could not be created");
         if (const char *v = std::getenv("FDTD_SIDE_STREAMS")) s->side_streams = std::atoi(v) == 1 ? 1 : 3;
         if (const char *v = std::getenv("FDTD_CHAIN_JOIN")) s->chain_join = std::atoi(v) != 0;
+        if (const char *v = std::getenv("FDTD_POST_AFTER_JOIN")) s->post_after_join = std::atoi(v) != 0;
         if (s->side_streams == 3 &&
             (hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking) != hipSuccess ||
              hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking) != hipSuccess ||
@@ -641,7 +642,11 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     if (ev_clean0) HIPCHK(s, hipEventRecord(ev_clean0, s->stream));
     HIPCHK(s, launch_step_rows(r, s->rows, 0, s->stream));
     if (ev_clean1) HIPCHK(s, hipEventRecord(ev_clean1, s->stream));
-    if (tf_early) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
+    // with the side streams chained, the post-update launch follows the join (one
+    // barrier on the main stream; the side pieces end before the clean box anyway)
+    const bool after_join = three && s->chain_join && s->post_after_join;
+    if (after_join) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    else if (tf_early) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
     const bool merge_disp = disp_early && tf_early && !s->S.hydro;
     if (disp_early && !merge_disp) {
         HIPCHK(s, s->S.hydro ? launch_hydro(*dd, s->stream) : launch_dispersive(*dd, s->stream));
@@ -654,7 +659,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
         *disp_done = *disp_done || merge_disp;
         *samp_done = merge_samp;
     }
-    if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    if (fork && !after_join) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     if (three && !s->chain_join) {
         HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join3, 0));
         HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join4, 0));

@@ -103,6 +103,7 @@ struct fdtd_sim {
     // slabs after the line on stream2, y slabs on stream3, corner columns on stream4)
     int side_streams = 3;
     bool chain_join = true;         // side streams join through stream2 (FDTD_CHAIN_JOIN=0: each joins the main stream)
+    bool post_after_join = true;    // with chained joins, the post-update launch after the join (FDTD_POST_AFTER_JOIN)
     hipStream_t stream3 = nullptr, stream4 = nullptr;
     hipEvent_t ev_join3 = nullptr, ev_join4 = nullptr;
     bool early = true;              // TFSF fix and dispersive box ahead of the join when their cells allow (FDTD_EARLY=0 to keep them after)
