@@ -1005,6 +1005,9 @@ int adj_alloc(fdtd_sim *s) {
     // cells inside the clean box and free of z CPML (launch_adj_box_fix)
     s->adj.rows = false;
     if (const char *v = std::getenv("FDTD_ADJ_SCHUNK")) s->adj_schunk = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_ADJ_RCHUNK")) s->adj_rchunk = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_ADJ_SCHED")) s->adj_sched = std::min(3, std::max(0, std::atoi(v)));
+    if (const char *v = std::getenv("FDTD_ADJ_FUSE_INJECT")) s->adj_fuse_inject = std::atoi(v) != 0;
     if (const char *v = std::getenv("FDTD_ADJ_STJ")) {
         const int tj = std::atoi(v);
         if (tj == 4 || tj == 8 || tj == 16) s->adj_stj = tj;
@@ -1033,6 +1036,15 @@ int adj_alloc(fdtd_sim *s) {
             std::memcpy(&img[1548], ptrs, sizeof(ptrs));
             if ((rc = upload(s, &s->adj.aux_d, img))) return rc;
             s->adj.rows = true;
+            // x and y slabs on the row-blocked transposed variants (FDTD_ADJ_SLABS=general:
+            // the general transposed kernel)
+            const char *sl = std::getenv("FDTD_ADJ_SLABS");
+            s->adj.rows_slabs = !(sl && std::string(sl) == "general");
+            // amounts of the dispersive-cell correction (schedule 3)
+            if (s->nbox > 0) {
+                const long nc = (long)(s->box_n[0] + 2) * (s->box_n[1] + 2) * (s->box_n[2] + 2);
+                if ((rc = dalloc(s, &s->adj.corr, 6 * nc))) return rc;
+            }
         }
     }
     s->adj.ready = true;
@@ -1102,9 +1114,16 @@ namespace {
 
 // One transposed step: adjoint level pa -> 1 - pa, psi_E pe, psi_H ph.  On
 // the row path the clean box runs on the row-blocked kernel (variant 3) with
-// the dispersive-cell correction after it, and the side slabs and corner
-// columns on the general transposed kernel beside it on the side streams.
-int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
+// the dispersive-cell correction after it, the x and y slabs on its slab
+// variants (4, 5) and the corner columns on the general transposed kernel,
+// beside it on the side streams.
+//
+// ad: the transposed dispersive update of this step (before the correction of
+// the dispersive cells, which reads the box state it writes).  next: the
+// injection of the following step, folded into this step's correction when
+// it is on (the caller turns it on only where the objective cell lies in the
+// clean box, which no side launch writes); *injected reports whether it was.
+int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, const AdjInject &next, bool *injected) {
     const Setup &S = s->S;
     AdjDev av{};
     const StepDev sd = make_step(s, false);
@@ -1120,8 +1139,10 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
         av.psiH_in.p[x] = s->adj.psiH[ph][x]; av.psiH_out.p[x] = s->adj.psiH[1 - ph][x];
     }
     for (int x = 0; x < 3; x++) { av.blo[x] = s->box_lo[x]; av.bn[x] = s->box_n[x]; }
+    *injected = false;
     if (!s->adj.rows) {
         const int full[1][4] = {{0, s->g.NX, 0, s->g.NY}};
+        HIPCHK(s, launch_adj_disp(ad, s->stream));
         HIPCHK(s, launch_adj_step(av, s->tj, s->ichunk, full, 1, s->stream));
         return FDTD_OK;
     }
@@ -1129,6 +1150,14 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
     // with three side streams the x slabs, the y slabs and the corner columns each get one
     const bool three = fork && s->stream3 && s->stream4;
     hipStream_t side = fork ? s->stream2 : s->stream;
+    // schedules 1-3: the transposed dispersive update beside the clean box,
+    // first on the corner stream (it reads the input level only, and the row
+    // kernels do not read the box state); the correction of the dispersive
+    // cells after the clean box on that stream (1) or on the main stream (2),
+    // or its amounts computed on the corner stream after the update and only
+    // added to the outputs on the main stream after the clean box (3)
+    const int sched = three && s->nbox > 0 && (s->adj_sched != 3 || s->adj.corr) ? s->adj_sched : 0;
+    if (sched == 0) HIPCHK(s, launch_adj_disp(ad, s->stream));
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
@@ -1137,22 +1166,11 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
             HIPCHK(s, hipStreamWaitEvent(s->stream4, s->ev_fork, 0));
         }
     }
-    // the slabs are thin: short x chunks so they fill the device (one launch for
-    // all four on a single side stream)
-    if (three) {
-        if (s->nxbox) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, s->xbox, s->nxbox, s->stream2));
-        if (s->nybox) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, s->ybox, s->nybox, s->stream3));
-        if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, s->stream4));
-        HIPCHK(s, hipEventRecord(s->ev_join3, s->stream3));
-        HIPCHK(s, hipEventRecord(s->ev_join4, s->stream4));
-    } else {
-        int slabs[4][4], ns = 0;
-        for (int q = 0; q < s->nxbox; q++) std::memcpy(slabs[ns++], s->xbox[q], sizeof(slabs[0]));
-        for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
-        if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
-        if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, side));
+    if (sched) {
+        HIPCHK(s, launch_adj_disp(ad, s->stream4));
+        if (sched == 3) HIPCHK(s, launch_adj_box_corr(av, s->adj.corr, s->stream4));
+        if (sched >= 2) HIPCHK(s, hipEventRecord(s->ev_line, s->stream4));
     }
-    if (fork) HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
     RowsArgs r = make_rows(s);
     const long fb = s->fbase;
     for (int c = 0; c < 3; c++) {
@@ -1166,14 +1184,61 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph) {
     r.Ceh = (float)-S.Che;
     r.pp = ph;
     r.pe = pe;
+    // x and y slabs on the row-blocked transposed variants (two rows, one row
+    // tile, short x chunks so the thin slabs fill the device), or all four
+    // slabs on the general transposed kernel
+    auto slab_rows = [&](const int (*b)[4], int n, int variant, hipStream_t st) {
+        RowsArgs q = r;
+        q.ichunk = s->adj_rchunk;
+        q.tiles = 1;
+        q.npiece = n;
+        std::memcpy(q.box, b, sizeof(q.box[0]) * n);
+        return launch_step_rows(q, 2, variant, st);
+    };
+    const bool rs = s->adj.rows_slabs;
+    hipStream_t sx = three ? s->stream2 : side, sy = three ? s->stream3 : side, sc = three ? s->stream4 : side;
+    if (rs) {
+        if (s->nxbox) HIPCHK(s, slab_rows(s->xbox, s->nxbox, 4, sx));
+        if (s->nybox) HIPCHK(s, slab_rows(s->ybox, s->nybox, 5, sy));
+    } else if (three) {
+        if (s->nxbox) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, s->xbox, s->nxbox, sx));
+        if (s->nybox) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, s->ybox, s->nybox, sy));
+    } else {
+        int slabs[4][4], ns = 0;
+        for (int q = 0; q < s->nxbox; q++) std::memcpy(slabs[ns++], s->xbox[q], sizeof(slabs[0]));
+        for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
+        if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
+    }
+    if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, sc));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
     HIPCHK(s, launch_step_rows(r, s->rows, 3, s->stream));
-    if (s->nbox) HIPCHK(s, launch_adj_box_fix(av, s->stream));
-    if (fork) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    AdjInject none{};
+    const AdjInject &inj = s->nbox > 0 && next.on ? next : none;
+    if (s->nbox) {
+        if (sched == 1) {
+            HIPCHK(s, hipEventRecord(s->ev_line, s->stream));
+            HIPCHK(s, hipStreamWaitEvent(s->stream4, s->ev_line, 0));
+            HIPCHK(s, launch_adj_box_fix(av, inj, s->stream4));
+        } else if (sched == 3) {
+            HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
+            HIPCHK(s, launch_adj_box_add(av, inj, s->adj.corr, s->stream));
+        } else {
+            if (sched == 2) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
+            HIPCHK(s, launch_adj_box_fix(av, inj, s->stream));
+        }
+        *injected = inj.on != 0;
+    }
+    // the y-slab and corner streams join through the x-slab stream: one wait on the main stream
     if (three) {
-        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join3, 0));
-        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join4, 0));
+        HIPCHK(s, hipEventRecord(s->ev_join3, s->stream3));
+        HIPCHK(s, hipEventRecord(s->ev_join4, s->stream4));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_join3, 0));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_join4, 0));
+    }
+    if (fork) {
+        HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     }
     return FDTD_OK;
 }
@@ -1254,6 +1319,27 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
     s->sampling = false;
     int pa = 0, pe = 0, ph = 0;
     const long p0 = gidx(s->g, r0[0], r0[1], r0[2]);
+    // dJ/dE^{t+1}(r0) as the reverse sweep adds it before the transposed step of t
+    auto inject_at = [&](int t) {
+        AdjInject v{};
+        double a[3] = {0, 0, 0};
+        for (int f = 0; f < nf; f++) {
+            const std::complex<double> w = std::exp(std::complex<double>(0, 2 * fdtd::kPI * freq[f] * (t + 1.0) * S.dt));
+            for (int c = 0; c < 3; c++) a[c] += 2.0 * std::real(std::conj(gsum[(size_t)f * 3 + c]) * w);
+        }
+        v.p0 = p0;
+        for (int c = 0; c < 3; c++) v.a[c] = (float)a[c];
+        v.on = 1;
+        v.inside = 1;
+        for (int x = 0; x < 3; x++)
+            if (r0[x] < s->box_lo[x] - 1 || r0[x] >= s->box_lo[x] + s->box_n[x] + 1) v.inside = 0;
+        return v;
+    };
+    // the injection of step t-1 rides on the dispersive-cell correction of step t
+    // where the objective cell lies in the clean box (written there by the clean box alone)
+    const bool fuse = s->adj.rows && s->nbox > 0 && s->adj_fuse_inject && r0[0] >= s->cbox[0] &&
+                      r0[0] < s->cbox[1] && r0[1] >= s->cbox[2] && r0[1] < s->cbox[3];
+    bool injected = false;
     for (int seg = nseg - 1; seg >= 0; seg--) {
         int seg_t0 = t0, len = nsteps;
         if (!store) {
@@ -1273,12 +1359,10 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
         }
         for (int st = len - 1; st >= 0; st--) {
             const int t = seg_t0 + st;
-            double a[3] = {0, 0, 0};
-            for (int f = 0; f < nf; f++) {
-                const std::complex<double> w = std::exp(std::complex<double>(0, 2 * fdtd::kPI * freq[f] * (t + 1.0) * S.dt));
-                for (int c = 0; c < 3; c++) a[c] += 2.0 * std::real(std::conj(gsum[(size_t)f * 3 + c]) * w);
+            if (!injected) {
+                const AdjInject v = inject_at(t);
+                HIPCHK(s, launch_adj_inject(s->adj.E[pa], p0, v.a[0], v.a[1], v.a[2], s->stream));
             }
-            HIPCHK(s, launch_adj_inject(s->adj.E[pa], p0, (float)a[0], (float)a[1], (float)a[2], s->stream));
             AdjDispDev ad{};
             ad.d = make_disp(s, s->p);
             for (int c = 0; c < 3; c++) {
@@ -1291,8 +1375,8 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
                 ad.tp1[c] = traj + ((size_t)(st + 2) * 3 + c) * nbox;
             }
             ad.grad = s->adj.grad;
-            HIPCHK(s, launch_adj_disp(ad, s->stream));
-            if ((rc = adjoint_step(s, pa, pe, ph))) return rc;
+            const AdjInject nx = fuse && t > t0 ? inject_at(t - 1) : AdjInject{};
+            if ((rc = adjoint_step(s, pa, pe, ph, ad, nx, &injected))) return rc;
             pa ^= 1; pe ^= 1; ph ^= 1;
         }
     }

@@ -148,6 +148,16 @@ struct AdjDev {
     const float *Qb[3], *Eb[3];
 };
 
+// The objective's injection dJ/dE at one cell, added to the E adjoint by
+// k_adj_inject, or folded into the dispersive-cell correction of the step
+// before (after its own update of that cell).
+struct AdjInject {
+    long p0;                        // cell (field layout)
+    float a[3];
+    int on;
+    int inside;                     // p0 lies in the cells of the correction launch
+};
+
 // Adjoint of the dispersive update on the box (run before k_adj_step).
 struct AdjDispDev {
     DispDev d;                      // forward coefficients, geometry, rho
@@ -225,7 +235,9 @@ int rows_waves(int NZ);
 // boundaries per plane), y clean; 2: y general (per row), x clean; 3: the
 // transposed step of the adjoint on clean boxes (coefficients passed as
 // Chh = 1, Che = -Ceh, Cee, Ceh = -Che; the table image holds both adjoint
-// psi_E sets; dispersive cells are corrected afterwards by launch_adj_box_fix).
+// psi_E sets; dispersive cells are corrected afterwards by launch_adj_box_fix);
+// 4, 5: the transposed step on x slabs (as 1) and y slabs (as 2), two rows,
+// one row tile, no dispersive cell in the boxes.
 hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st);
 
 // box[q] = {ia, ib, ja, jb}; one to four disjoint boxes.
@@ -317,8 +329,15 @@ hipError_t launch_adj_inject(float *const E[3], long p0, float ax, float ay, flo
 hipError_t launch_adj_disp(const AdjDispDev &a, hipStream_t st);
 // box[q] = {ia, ib, ja, jb}: one to four disjoint x-y boxes of cells (all z).
 hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, const int (*box)[4], int nbox, hipStream_t st);
-// Dispersive-cell correction after the adjoint row kernel (variant 3 of launch_step_rows).
-hipError_t launch_adj_box_fix(const AdjDev &a, hipStream_t st);
+// Dispersive-cell correction after the adjoint row kernel (variant 3 of
+// launch_step_rows), then the injection inj (if on) into its E output.
+hipError_t launch_adj_box_fix(const AdjDev &a, const AdjInject &inj, hipStream_t st);
+// The same in two launches: the amounts into corr (6 x adj_box_cells floats;
+// reads the step's inputs and the transposed dispersive update only), then
+// their addition to the outputs after the row kernel, with the injection.
+long adj_box_cells(const AdjDev &a);
+hipError_t launch_adj_box_corr(const AdjDev &a, float *corr, hipStream_t st);
+hipError_t launch_adj_box_add(const AdjDev &a, const AdjInject &inj, const float *corr, hipStream_t st);
 // E at one cell for every step (objective probe).
 hipError_t launch_probe(const float *const E[3], long p0, float *out, hipStream_t st);
 

@@ -703,6 +703,14 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // double buffered (pp: psi_H set read, pe: psi_E set read).  The lanes at
 // wave edges must lie outside the z slabs (checked by the host).
 //
+// ADJ with XG (x slabs) or YG (y slabs): the weights along the slab's CPML
+// axis are formed per cell with their adjoint psi recursions (V of rows
+// 0..R+1 of the plane along y; V of plane i+1, carried to the next step, and
+// W of plane i-1 along x) and the differences are taken of them; the other
+// in-plane axis is clean and keeps the swapped-coefficient form.  The slabs
+// hold no dispersive cell (the host keeps the dispersive box and its margin
+// inside the clean box).
+//
 // The body takes the block's index within its simulation's cell set (L), so
 // batched launches run it over many simulations.
 //
@@ -714,7 +722,8 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // Tiles past the last row tile of the box repeat its loads and store nothing.
 template <int W, int R, bool XG, bool YG, bool ADJ, int T = 1>
 __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
-    static_assert(!ADJ || (!XG && !YG), "the adjoint variant runs clean boxes only");
+    static_assert(!ADJ || !XG || !YG, "the adjoint variant runs clean boxes, x slabs or y slabs");
+    constexpr bool AS = ADJ && (XG || YG);   // adjoint on a side slab
     static_assert(T >= 1 && W * T <= 16, "at most 16 waves per block");
     __shared__ float sK[2][2][W * T][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
     __shared__ float sE[2][2][W * T][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
@@ -819,10 +828,12 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     // when XG.  The y and x loads are issued unconditionally (indices clamped
     // into the slab arrays, values unused where no slab applies) so the wait
     // counts after them stay exact.
+    // Adjoint slabs: exy, ezy hold rows 0..R+1 (V along y) and eyx, ezx rows
+    // 0..R of plane i+1 (V along x of the next plane).
     struct PSet {
         float hxz[R + 1], hyz[R + 1], exz[R + 1], eyz[R + 1];
-        float hxy[R + 1], hzy[R + 1], exy[R], ezy[R];
-        float hyx[R + 1], hzx[R + 1], eyx[R], ezx[R];
+        float hxy[R + 1], hzy[R + 1], exy[ADJ ? R + 2 : R], ezy[ADJ ? R + 2 : R];
+        float hyx[R + 1], hzx[R + 1], eyx[ADJ ? R + 1 : R], ezx[ADJ ? R + 1 : R];
     };
     auto load_psi = [&](int ip, PSet &P) {
 #pragma unroll
@@ -846,6 +857,35 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
                     const long q = ((long)ip * NY + jb + r) * Lzh + lzh;
                     P.hxz[r] = pa[q];
                     P.hyz[r] = pb[q];
+                }
+            }
+            if (YG) {
+                // psi_E along y of rows 0..R+1, psi_H along y of rows 0..R (clamped into the slab arrays)
+                const gfloat *e1 = psi(ein + 1), *e4 = psi(ein + 4), *h1 = psi(pp * 6 + 1), *h5 = psi(pp * 6 + 5);
+#pragma unroll
+                for (int r = 0; r <= R + 1; r++) {
+                    const long q = ((long)ip * Lye + max(lye(r), 0)) * NZ + kq;
+                    P.exy[r] = e1[q];
+                    P.ezy[r] = e4[q];
+                }
+#pragma unroll
+                for (int r = 0; r <= R; r++) {
+                    const long q = ((long)ip * Lyh + max(lyh(r), 0)) * NZ + kq;
+                    P.hxy[r] = h1[q];
+                    P.hzy[r] = h5[q];
+                }
+            }
+            if (XG) {
+                // psi_E along x of plane ip+1, psi_H along x of plane ip
+                const gfloat *e3 = psi(ein + 3), *e5 = psi(ein + 5), *h3 = psi(pp * 6 + 3), *h4 = psi(pp * 6 + 4);
+                const int le = max(lxe(ip + 1), 0), lh = max(lxh(ip), 0);
+#pragma unroll
+                for (int r = 0; r <= R; r++) {
+                    const long qe = ((long)le * NY + jq(r)) * NZ + kq, qh = ((long)lh * NY + jq(r)) * NZ + kq;
+                    P.eyx[r] = e3[qe];
+                    P.ezx[r] = e5[qe];
+                    P.hyx[r] = h3[qh];
+                    P.hzx[r] = h4[qh];
                 }
             }
             return;
@@ -910,45 +950,152 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     auto zweights = [&](int ip, const float *x, const float *y, const PSet &P, bool wr, float *zx, float *zy) {
 #pragma unroll
         for (int r = 0; r <= R; r++) {
+            // x and y parts of the ex and ey domains (always true on a clean box)
+            const bool dx = (!XG || ip < NX - 1) && jC(r), dy = (!XG || (ip > 0 && ip < NX - 1)) && jA(r);
             float ax = kC ? x[r] * ikz_e : 0.f, ay = kC ? y[r] * ikz_e : 0.f;
             if (ze && lze >= 0) {
                 ax -= cze * (P.exz[r] * icE - x[r]);
                 ay += cze * (P.eyz[r] * icE + y[r]);
                 if (wr && r > 0 && kown && jb + r < jbox) {
                     const long q = ((long)ip * NY + jb + r) * Lze + lze;
-                    psi(eout + 0)[q] = bze * (P.exz[r] - cE * x[r]);
-                    psi(eout + 2)[q] = bze * (P.eyz[r] + cE * y[r]);
+                    if (dx) psi(eout + 0)[q] = bze * (P.exz[r] - cE * x[r]);
+                    if (dy) psi(eout + 2)[q] = bze * (P.eyz[r] + cE * y[r]);
                 }
             }
-            zx[r] = ax;
-            zy[r] = ay;
+            zx[r] = dx ? ax : 0.f;
+            zy[r] = dy ? ay : 0.f;
         }
     };
     auto hweights = [&](int ip, const float *hx, const float *hy, const PSet &P, float *wx, float *wy) {
 #pragma unroll
         for (int r = 1; r <= R; r++) {
+            const bool dx = jA(r), dy = (!XG || ip < NX - 1) && jB(r);   // x and y parts of the hx, hy domains
             float ax = kA ? hx[r] * ikz_h : 0.f, ay = kA ? hy[r] * ikz_h : 0.f;
             if (zh && lzh >= 0) {
                 ax += czh * (P.hxz[r] * icH + hx[r]);
                 ay -= czh * (P.hyz[r] * icH - hy[r]);
                 if (kown && jb + r < jbox) {
                     const long q = ((long)ip * NY + jb + r) * Lzh + lzh;
-                    psi(hout + 0)[q] = bzh * (P.hxz[r] + cH * hx[r]);
-                    psi(hout + 2)[q] = bzh * (P.hyz[r] - cH * hy[r]);
+                    if (dx) psi(hout + 0)[q] = bzh * (P.hxz[r] + cH * hx[r]);
+                    if (dy) psi(hout + 2)[q] = bzh * (P.hyz[r] - cH * hy[r]);
                 }
             }
-            wx[r] = ax;
-            wy[r] = ay;
+            wx[r] = dx ? ax : 0.f;
+            wy[r] = dy ? ay : 0.f;
+        }
+    };
+    // Adjoint slabs, weights along the slab axis (k_adj_step's adj_v, adj_w
+    // with Q = cE E, G = cH Hhat_tot; zero outside the component domains; the
+    // own cells write the psi adjoints).
+    // y, V of rows 0..R+1 of plane ip: Vxy = Q ik + c (psi + Q), Vzy = Q ik - c (psi - Q)
+    auto vyweights = [&](int ip, const float *ex, const float *ez, float exj, float ezj, const PSet &P, bool wr,
+                         float *vxy, float *vzy) {
+#pragma unroll
+        for (int r = 0; r <= R + 1; r++) {
+            const float Qx = cE * (r <= R ? ex[r] : exj), Qz = cE * (r <= R ? ez[r] : ezj);
+            const float iky = ik(1, true, jq(r), NY);
+            float ax = Qx * iky, az = Qz * iky;
+            const int ly = lye(r);
+            if (ly >= 0) {
+                const float by = coef(1, 0, ly), cy = coef(1, 1, ly);
+                const float tx = P.exy[r] + Qx, tz = P.ezy[r] - Qz;
+                ax += cy * tx;
+                az -= cy * tz;
+                if (wr && r > 0 && r <= R && kown && jb + r < jbox) {
+                    const long q = ((long)ip * Lye + ly) * NZ + k;
+                    if (kC) psi(eout + 1)[q] = by * tx;
+                    if (kA) psi(eout + 4)[q] = by * tz;
+                }
+            }
+            vxy[r] = jC(r) && kC ? ax : 0.f;
+            vzy[r] = jC(r) && kA ? az : 0.f;
+        }
+    };
+    // x, V of rows 0..R of plane ip: Vyx = Q ik - c (psi - Q), Vzx = Q ik + c (psi + Q)
+    auto vxweights = [&](int ip, const float *ey, const float *ez, const float *pyx, const float *pzx, bool wr,
+                         float *vyx, float *vzx) {
+        const bool iC = ip > 0 && ip < NX - 1;
+        const float ikx = ik(0, true, min(max(ip, 0), NX - 1), NX);
+        const int lx = lxe(ip);
+        const float bx = lx >= 0 ? coef(0, 0, lx) : 0.f, cx = lx >= 0 ? coef(0, 1, lx) : 0.f;
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            const float Qy = cE * ey[r], Qz = cE * ez[r];
+            float ay = Qy * ikx, az = Qz * ikx;
+            if (lx >= 0) {
+                const float ty = pyx[r] - Qy, tz = pzx[r] + Qz;
+                ay -= cx * ty;
+                az += cx * tz;
+                if (wr && r > 0 && kown && jb + r < jbox) {
+                    const long q = ((long)lx * NY + jb + r) * NZ + k;
+                    if (kC) psi(eout + 3)[q] = bx * ty;
+                    if (kA) psi(eout + 5)[q] = bx * tz;
+                }
+            }
+            vyx[r] = iC && jA(r) && kC ? ay : 0.f;
+            vzx[r] = iC && jC(r) && kA ? az : 0.f;
+        }
+    };
+    // y, W of rows 0..R of plane ip: Wxy = G ik - c (psi - G), Wzy = G ik + c (psi + G)
+    auto wyweights = [&](int ip, const float *hx, const float *hz, const PSet &P, float *wxy, float *wzy) {
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            const float Gx = cH * hx[r], Gz = cH * hz[r], iky = ik(1, false, jq(r), NY);
+            float ax = Gx * iky, az = Gz * iky;
+            const int ly = lyh(r);
+            if (ly >= 0 && kA) {
+                const float by = coef(1, 2, ly), cy = coef(1, 3, ly);
+                const float tx = P.hxy[r] - Gx, tz = P.hzy[r] + Gz;
+                ax -= cy * tx;
+                az += cy * tz;
+                if (r > 0 && kown && jb + r < jbox) {
+                    const long q = ((long)ip * Lyh + ly) * NZ + k;
+                    psi(hout + 1)[q] = by * tx;
+                    psi(hout + 5)[q] = by * tz;
+                }
+            }
+            wxy[r] = jA(r) && kA ? ax : 0.f;
+            wzy[r] = jA(r) && kB ? az : 0.f;
+        }
+    };
+    // x, W of rows 1..R of plane ip: Wyx = G ik + c (psi + G), Wzx = G ik - c (psi - G)
+    auto wxweights = [&](int ip, const float *hy, const float *hz, const PSet &P, bool wr, float *wyx, float *wzx) {
+        const bool iA = ip < NX - 1;
+        const float ikx = ik(0, false, min(max(ip, 0), NX - 1), NX);
+        const int lx = lxh(ip);
+        const float bx = lx >= 0 ? coef(0, 2, lx) : 0.f, cx = lx >= 0 ? coef(0, 3, lx) : 0.f;
+        wyx[0] = 0.f;
+        wzx[0] = 0.f;
+#pragma unroll
+        for (int r = 1; r <= R; r++) {
+            const float Gy = cH * hy[r], Gz = cH * hz[r];
+            float ay = Gy * ikx, az = Gz * ikx;
+            if (lx >= 0 && kA) {
+                const float ty = P.hyx[r] + Gy, tz = P.hzx[r] - Gz;
+                ay += cx * ty;
+                az -= cx * tz;
+                if (wr && kown && jb + r < jbox) {
+                    const long q = ((long)lx * NY + jb + r) * NZ + k;
+                    psi(hout + 3)[q] = bx * ty;
+                    psi(hout + 4)[q] = bx * tz;
+                }
+            }
+            wyx[r] = iA && jB(r) && kA ? ay : 0.f;
+            wzx[r] = iA && jA(r) && kB ? az : 0.f;
         }
     };
 
     // H of plane ip for rows 0..R from E^n of plane ip (rows 0..R, ex and ez of
     // row R+1, ex and ey at k+1), ey and ez of plane ip+1, H^{n-1/2} and psi_H
     // of plane ip.  wr: write psi_H of the own cells.
-    // Adjoint: exk, eyk are the z weights of the k+1 neighbours and zx, zy the own ones.
+    // Adjoint: exk, eyk are the z weights of the k+1 neighbours and zx, zy the
+    // own ones; on the slabs vxy, vzy are the V weights along y of rows 0..R+1
+    // and vyxc, vzxc (plane ip), vyxn, vzxn (plane ip+1) those along x.
     auto hplane = [&](int ip, const float *ex, const float *ey, const float *ez, float exj, float ezj,
                       const float *exk, const float *eyk, const float *ny, const float *nz, const float (*ho)[R + 1],
-                      const PSet &P, bool wr, const float *zx, const float *zy, float *hx, float *hy, float *hz) {
+                      const PSet &P, bool wr, const float *zx, const float *zy, const float *vxy, const float *vzy,
+                      const float *vyxc, const float *vzxc, const float *vyxn, const float *vzxn, float *hx, float *hy,
+                      float *hz) {
         const bool iA = !XG || ip < NX - 1;
         const float ikx_h = XG ? ik(0, false, ip, NX) : ra.ik0[1][0];
         const int lx = lxh(ip);
@@ -961,15 +1108,25 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
             const float dEy_z = eyk[r] - ey[r], dEz_y = zj - ez[r];
             const float dEz_x = nz[r] - ez[r], dEx_z = exk[r] - ex[r];
             const float dEx_y = xj - ex[r], dEy_x = ny[r] - ey[r];
-            float vx, vy;
-            if constexpr (ADJ) {
+            float vx, vy, vz;
+            if constexpr (AS) {
+                // Hhat_tot = Hhat + forward differences of V (V along z from the lane weights)
+                vx = Chh * ho[0][r] + Che * (eyk[r] - zy[r]);
+                vy = Chh * ho[1][r] - Che * (exk[r] - zx[r]);
+                vz = Chh * ho[2][r];
+                if constexpr (YG) { vx += vzy[r + 1] - vzy[r]; vz += vxy[r] - vxy[r + 1]; }
+                else { vx -= Che * dEz_y * iky_h; vz += Che * dEx_y * iky_h; }
+                if constexpr (XG) { vy += vzxc[r] - vzxn[r]; vz += vyxn[r] - vyxc[r]; }
+                else { vy += Che * dEz_x * ikx_h; vz -= Che * dEy_x * ikx_h; }
+            } else if constexpr (ADJ) {
                 vx = Chh * ho[0][r] + Che * ((eyk[r] - zy[r]) - dEz_y * iky_h);
                 vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - (exk[r] - zx[r]));
+                vz = Chh * ho[2][r] + Che * (dEx_y * iky_h - dEy_x * ikx_h);
             } else {
                 vx = Chh * ho[0][r] + Che * (dEy_z * ikz_h - dEz_y * iky_h);
                 vy = Chh * ho[1][r] + Che * (dEz_x * ikx_h - dEx_z * ikz_h);
+                vz = Chh * ho[2][r] + Che * (dEx_y * iky_h - dEy_x * ikx_h);
             }
-            float vz = Chh * ho[2][r] + Che * (dEx_y * iky_h - dEy_x * ikx_h);
             // per component the psi terms add in the reference's order:
             // hx +z -y, hy -z +x, hz -x +y (main.cu:834-1037)
             if (!ADJ && zh && iA && jA(r) && lzh >= 0) {
@@ -982,7 +1139,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
             }
             const int ly = lyh(r);
             float pyz = 0.f;
-            if (YG && iA && ly >= 0 && kA) {
+            if (!ADJ && YG && iA && ly >= 0 && kA) {
                 const long q = ((long)ip * Lyh + ly) * NZ + k;
                 const float by = coef(1, 2, ly), cyv = coef(1, 3, ly);
                 const float a = by * P.hxy[r] + cyv * dEz_y;
@@ -990,7 +1147,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
                 vx -= Che * a;
                 if (own) { psi(hout + 1)[q] = a; psi(hout + 5)[q] = pyz; }
             }
-            if (XG && lx >= 0 && jA(r) && kA) {
+            if (!ADJ && XG && lx >= 0 && jA(r) && kA) {
                 const long q = ((long)lx * NY + j) * NZ + k;
                 const float bx = coef(0, 2, lx), cxv = coef(0, 3, lx);
                 const float a = bx * P.hyx[r] + cxv * dEz_x;
@@ -999,7 +1156,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
                 vz -= Che * b;
                 if (own) { psi(hout + 3)[q] = a; psi(hout + 4)[q] = b; }
             }
-            if (YG && iA && ly >= 0 && kA) vz += Che * pyz;
+            if (!ADJ && YG && iA && ly >= 0 && kA) vz += Che * pyz;
             hx[r] = jA(r) && kA ? vx : ho[0][r];
             hy[r] = iA && jB(r) && kA ? vy : ho[1][r];
             hz[r] = iA && jA(r) && kB ? vz : ho[2][r];
@@ -1014,7 +1171,8 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     // the top of step i (one set).
     struct ESet { float x[R + 1], y[R + 1], z[R + 1], xj, zj; };
     struct HSet { float h[3][R + 1]; };
-    float hyP[R + 1], hzP[R + 1];                         // H^{n+1/2} of plane i-1
+    float hyP[R + 1], hzP[R + 1];                         // H^{n+1/2} of plane i-1 (adjoint x slabs: Wyx, Wzx)
+    float vyxC[R + 1], vzxC[R + 1];                       // adjoint x slabs: Vyx, Vzx of plane i
     auto load_e = [&](int ip, ESet &e) {
         const unsigned pl = (unsigned)(ip + 1) * sxy4;
 #pragma unroll
@@ -1033,14 +1191,26 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
     // prologue: H^{n+1/2} of plane i0-1 (not needed at i0 = 0: ey, ez are not
     // updated there), E^n of planes i0 and i0+1, H and psi of plane i0
 #pragma unroll
-    for (int r = 0; r <= R; r++) { hyP[r] = 0.f; hzP[r] = 0.f; }
+    for (int r = 0; r <= R; r++) { hyP[r] = 0.f; hzP[r] = 0.f; vyxC[r] = 0.f; vzxC[r] = 0.f; }
     load_e(i0, EA);
     if (i0 > 0) {
         ESet P;
         float pxk[R + 1], pyk[R + 1], hx[R + 1], hy[R + 1], hz[R + 1];
+        float vxy[R + 2], vzy[R + 2], vyx[R + 1], vzx[R + 1], pyx[R + 1], pzx[R + 1];
         load_e(i0 - 1, P);
         load_h(i0 - 1, H1);
         load_psi(i0 - 1, P1);
+        if constexpr (AS && XG) {
+            // psi_E along x of the prologue plane itself (load_psi brings plane i0's)
+            const gfloat *e3 = psi(ein + 3), *e5 = psi(ein + 5);
+            const int le = max(lxe(i0 - 1), 0);
+#pragma unroll
+            for (int r = 0; r <= R; r++) {
+                const long q = ((long)le * NY + jq(r)) * NZ + kq;
+                pyx[r] = e3[q];
+                pzx[r] = e5[q];
+            }
+        }
         e_publish(P.x, P.y, 1);
         lds_sync();
         float zx[R + 1], zy[R + 1];
@@ -1050,9 +1220,19 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
         } else {
             e_up(P.x, P.y, 1, pxk, pyk);
         }
-        hplane(i0 - 1, P.x, P.y, P.z, P.xj, P.zj, pxk, pyk, EA.y, EA.z, H1.h, P1, false, zx, zy, hx, hy, hz);
+        if constexpr (AS && YG) vyweights(i0 - 1, P.x, P.z, P.xj, P.zj, P1, false, vxy, vzy);
+        if constexpr (AS && XG) {
+            vxweights(i0 - 1, P.y, P.z, pyx, pzx, false, vyx, vzx);
+            vxweights(i0, EA.y, EA.z, P1.eyx, P1.ezx, true, vyxC, vzxC);   // plane i0 is this chunk's: psi written
+        }
+        hplane(i0 - 1, P.x, P.y, P.z, P.xj, P.zj, pxk, pyk, EA.y, EA.z, H1.h, P1, false, zx, zy, vxy, vzy, vyx, vzx,
+               vyxC, vzxC, hx, hy, hz);
+        if constexpr (AS && XG) {
+            wxweights(i0 - 1, hy, hz, P1, false, hyP, hzP);
+        } else {
 #pragma unroll
-        for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
+            for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
+        }
     }
     e_publish(EA.x, EA.y, 0);
     load_e(i0 + 1, EB);
@@ -1079,10 +1259,16 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
         } else {
             e_up(C.x, C.y, buf, exk, eyk);
         }
-        hplane(i, C.x, C.y, C.z, C.xj, C.zj, exk, eyk, N.y, N.z, Hc.h, Pc, true, zx, zy, hx, hy, hz);
+        float vxy[R + 2], vzy[R + 2], vyxN[R + 1], vzxN[R + 1];
+        if constexpr (AS && YG) vyweights(i, C.x, C.z, C.xj, C.zj, Pc, true, vxy, vzy);
+        if constexpr (AS && XG) vxweights(i + 1, N.y, N.z, Pc.eyx, Pc.ezx, more, vyxN, vzxN);
+        hplane(i, C.x, C.y, C.z, C.xj, C.zj, exk, eyk, N.y, N.z, Hc.h, Pc, true, zx, zy, vxy, vzy, vyxC, vzxC, vyxN,
+               vzxN, hx, hy, hz);
         // adjoint: the z differences of the E half are taken of the H-side weights
-        float wx[R + 1], wy[R + 1];
+        float wx[R + 1], wy[R + 1], wxy[R + 1], wzy[R + 1], wyx[R + 1], wzx[R + 1];
         if constexpr (ADJ) hweights(i, hx, hy, Pc, wx, wy);
+        if constexpr (AS && YG) wyweights(i, hx, hz, Pc, wxy, wzy);
+        if constexpr (AS && XG) wxweights(i, hy, hz, Pc, true, wyx, wzx);
         const float *sy = ADJ ? wy : hy, *sx = ADJ ? wx : hx;
 #pragma unroll
         for (int r = 1; r <= R; r++) {
@@ -1109,15 +1295,25 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
             const float dHz_y = hz[r] - hz[r - 1], dHy_z = hy[r] - hyk[r];
             const float dHx_z = hx[r] - hxk[r], dHz_x = hz[r] - hzP[r];
             const float dHy_x = hy[r] - hyP[r], dHx_y = hx[r] - hx[r - 1];
-            float vx, vy;
-            if constexpr (ADJ) {
+            float vx, vy, vz;
+            if constexpr (AS) {
+                // Ehat = Cee Ehat + backward differences of W (W along z from the lane weights)
+                vx = Cee * C.x[r] - Ceh * (wy[r] - hyk[r]);
+                vy = Cee * C.y[r] + Ceh * (wx[r] - hxk[r]);
+                vz = Cee * C.z[r];
+                if constexpr (YG) { vx += wzy[r - 1] - wzy[r]; vz += wxy[r] - wxy[r - 1]; }
+                else { vx += Ceh * dHz_y * iky_e; vz -= Ceh * dHx_y * iky_e; }
+                if constexpr (XG) { vy += wzx[r] - hzP[r]; vz += hyP[r] - wyx[r]; }
+                else { vy -= Ceh * dHz_x * ikx_e; vz += Ceh * dHy_x * ikx_e; }
+            } else if constexpr (ADJ) {
                 vx = Cee * C.x[r] + Ceh * (dHz_y * iky_e - (wy[r] - hyk[r]));
                 vy = Cee * C.y[r] + Ceh * ((wx[r] - hxk[r]) - dHz_x * ikx_e);
+                vz = Cee * C.z[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
             } else {
                 vx = Cee * C.x[r] + Ceh * (dHz_y * iky_e - dHy_z * ikz_e);
                 vy = Cee * C.y[r] + Ceh * (dHx_z * ikz_e - dHz_x * ikx_e);
+                vz = Cee * C.z[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
             }
-            float vz = Cee * C.z[r] + Ceh * (dHy_x * ikx_e - dHx_y * iky_e);
             const bool ux = iA && jC(r) && kC, uy = iC && jA(r) && kC, uz = iC && jC(r) && kA;
             // ex -z +y, ey +z -x, ez -y +x (main.cu:1172-1906)
             if (!ADJ && ze && iA && lze >= 0) {
@@ -1126,13 +1322,13 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
                 if (iC && jA(r)) { const float a = bze * Pc.eyz[r - 1] + cze * dHx_z; vy += Ceh * a; if (own) psi(12 + 2)[q] = a; }
             }
             const int ly = lye(r);
-            if (YG && iA && ly >= 0) {
+            if (!ADJ && YG && iA && ly >= 0) {
                 const long q = ((long)i * Lye + ly) * NZ + k;
                 const float by = coef(1, 0, ly), cyv = coef(1, 1, ly);
                 if (kC) { const float a = by * Pc.exy[r - 1] + cyv * dHz_y; vx += Ceh * a; if (own) psi(12 + 1)[q] = a; }
                 if (iC && kA) { const float a = by * Pc.ezy[r - 1] + cyv * dHx_y; vz -= Ceh * a; if (own) psi(12 + 4)[q] = a; }
             }
-            if (XG && lx >= 0) {
+            if (!ADJ && XG && lx >= 0) {
                 const long q = ((long)lx * NY + j) * NZ + k;
                 const float bx = coef(0, 0, lx), cxv = coef(0, 1, lx);
                 if (jA(r) && kC) { const float a = bx * Pc.eyx[r - 1] + cxv * dHz_x; vy -= Ceh * a; if (own) psi(12 + 3)[q] = a; }
@@ -1144,8 +1340,13 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
                 stn<true>(oEz, pl + ro[r], uz ? vz : C.z[r]);
             }
         }
+        if constexpr (AS && XG) {
 #pragma unroll
-        for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
+            for (int r = 0; r <= R; r++) { hyP[r] = wyx[r]; hzP[r] = wzx[r]; vyxC[r] = vyxN[r]; vzxC[r] = vzxN[r]; }
+        } else {
+#pragma unroll
+            for (int r = 0; r <= R; r++) { hyP[r] = hy[r]; hzP[r] = hz[r]; }
+        }
     };
     for (int i = i0;;) {
         step(i, EA, EB, EC, H0, H1, P0); if (++i >= i1) break;
@@ -2314,19 +2515,11 @@ __device__ inline void adj_dh(const AdjDev &a, int i, int j, int k, float &hx, f
     hy = -(xz - xz1) + (zx - zx1);
     hz = (xy - xy1) - (yx - yx1);
 }
-__global__ void k_adj_box_fix(AdjDev a) {
-    const int nx = a.bn[0] + 2, ny = a.bn[1] + 2, nz = a.bn[2] + 2;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)nx * ny * nz) return;
-    const int i = a.blo[0] - 1 + (int)(idx / ((long)ny * nz));
-    const int j = a.blo[1] - 1 + (int)((idx / nz) % ny);
-    const int k = a.blo[2] - 1 + (int)(idx % nz);
-    const long p = gidx(a.g, i, j, k);
+// Correction of one cell (i, j, k) of the box grown by one cell: the H and E
+// amounts the dispersive cells add to the vacuum transposed step.
+__device__ inline void box_corr(const AdjDev &a, int i, int j, int k, long p, float *h, float *e) {
     float hx, hy, hz;
     adj_dh(a, i, j, k, hx, hy, hz);
-    if (hx != 0.f) a.Hout[0][p] += a.c.Chh * hx;
-    if (hy != 0.f) a.Hout[1][p] += a.c.Chh * hy;
-    if (hz != 0.f) a.Hout[2][p] += a.c.Chh * hz;
     float xm, ym, zm, t0, t1;
     const float Che = a.c.Che;
     const float hx_i = a.cp[0].ik_h[i], hy_i = a.cp[1].ik_h[j], hz_i = a.cp[2].ik_h[k];
@@ -2354,9 +2547,67 @@ __global__ void k_adj_box_fix(AdjDev a) {
             else ez += db;
         }
     }
-    if (ex != 0.f) a.Eout[0][p] += ex;
-    if (ey != 0.f) a.Eout[1][p] += ey;
-    if (ez != 0.f) a.Eout[2][p] += ez;
+    h[0] = hx; h[1] = hy; h[2] = hz;
+    e[0] = ex; e[1] = ey; e[2] = ez;
+}
+// Adds a cell's correction to the step's outputs, then the injection where it lands.
+__device__ inline void box_apply(const AdjDev &a, const AdjInject &inj, long p, const float *h, const float *e) {
+    for (int c = 0; c < 3; c++)
+        if (h[c] != 0.f) a.Hout[c][p] += a.c.Chh * h[c];
+    // the injection lands after this cell's correction, as a separate k_adj_inject would
+    const bool at = inj.on && p == inj.p0;
+    for (int c = 0; c < 3; c++)
+        if (e[c] != 0.f || at) {
+            float v = a.Eout[c][p];
+            if (e[c] != 0.f) v += e[c];
+            if (at) v += inj.a[c];
+            a.Eout[c][p] = v;
+        }
+}
+// Cell of thread idx in the grown box, or the injection outside it (idx == n).
+__device__ inline bool box_cell(const AdjDev &a, const AdjInject &inj, long idx, int &i, int &j, int &k) {
+    const int nx = a.bn[0] + 2, ny = a.bn[1] + 2, nz = a.bn[2] + 2;
+    const long n = (long)nx * ny * nz;
+    if (idx == n && inj.on && !inj.inside) {
+        // injection outside the corrected cells: as k_adj_inject
+        for (int c = 0; c < 3; c++) a.Eout[c][inj.p0] += inj.a[c];
+    }
+    if (idx >= n) return false;
+    i = a.blo[0] - 1 + (int)(idx / ((long)ny * nz));
+    j = a.blo[1] - 1 + (int)((idx / nz) % ny);
+    k = a.blo[2] - 1 + (int)(idx % nz);
+    return true;
+}
+__global__ void k_adj_box_fix(AdjDev a, AdjInject inj) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!box_cell(a, inj, idx, i, j, k)) return;
+    const long p = gidx(a.g, i, j, k);
+    float h[3], e[3];
+    box_corr(a, i, j, k, p, h, e);
+    box_apply(a, inj, p, h, e);
+}
+// The correction in two launches: its amounts (from the step's inputs and the
+// transposed dispersive update alone, so beside the row kernel), then their
+// addition after it.  corr: 6 x n floats (H then E, component-major).
+__global__ void k_adj_box_corr(AdjDev a, float *corr) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const AdjInject none{};
+    int i, j, k;
+    if (!box_cell(a, none, idx, i, j, k)) return;
+    const long n = (long)(a.bn[0] + 2) * (a.bn[1] + 2) * (a.bn[2] + 2);
+    float h[3], e[3];
+    box_corr(a, i, j, k, gidx(a.g, i, j, k), h, e);
+    for (int c = 0; c < 3; c++) { corr[c * n + idx] = h[c]; corr[(3 + c) * n + idx] = e[c]; }
+}
+__global__ void k_adj_box_add(AdjDev a, AdjInject inj, const float *corr) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!box_cell(a, inj, idx, i, j, k)) return;
+    const long n = (long)(a.bn[0] + 2) * (a.bn[1] + 2) * (a.bn[2] + 2);
+    float h[3], e[3];
+    for (int c = 0; c < 3; c++) { h[c] = corr[c * n + idx]; e[c] = corr[(3 + c) * n + idx]; }
+    box_apply(a, inj, gidx(a.g, i, j, k), h, e);
 }
 
 // Adjoint of the (P, J) dispersive update at every dispersive box cell:
@@ -2661,7 +2912,12 @@ static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_
     }
     switch (R) {
         case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
-        case 3: hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
+        case 3:
+            if constexpr (!ADJ || (!XG && !YG)) {
+                hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra);
+                break;
+            }
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2689,19 +2945,21 @@ int rows_waves(int NZ) {
 }
 
 long rows_prepare(RowsArgs &ra, int R, int variant) {
-    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 3) return -1;
+    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 5) return -1;
+    if (variant >= 4 && R != 2) return -1;   // the adjoint slab variants are built for two rows
     const int W = rows_waves(ra.NZ);
     if (ra.tiles < 1) ra.tiles = 1;
-    if (ra.tiles != 1 && (ra.tiles < 2 || ra.tiles > 3 || ra.tiles * W > 16 || variant == 3 || (variant != 0 && ra.tiles != 2)))
+    if (ra.tiles != 1 && (ra.tiles < 2 || ra.tiles > 3 || ra.tiles * W > 16 || variant >= 3 || (variant != 0 && ra.tiles != 2)))
         return -1;
+    const bool xg = variant == 1 || variant == 4, yg = variant == 2 || variant == 5;
     const int T = ra.tiles;
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
     long total = 0;
     for (int q = 0; q < ra.npiece; q++) {
         const int *b = ra.box[q];
         if (b[0] < 0 || b[1] > ra.NX || b[0] >= b[1] || b[2] < 0 || b[3] > ra.NY || b[2] >= b[3]) return -1;
-        if (variant != 1 && (b[0] < 1 || b[1] > ra.NX - 1)) return -1;   // x must be clean
-        if (variant != 2 && (b[2] < 1 || b[3] > ra.NY - 1)) return -1;   // y must be clean
+        if (!xg && (b[0] < 1 || b[1] > ra.NX - 1)) return -1;   // x must be clean
+        if (!yg && (b[2] < 1 || b[3] > ra.NY - 1)) return -1;   // y must be clean
         ra.start[q] = (int)total;
         total += (long)(((b[3] - b[2] + R - 1) / R + T - 1) / T) * ra.nseg * ((b[1] - b[0] + ra.ichunk - 1) / ra.ichunk);
     }
@@ -2719,7 +2977,9 @@ hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
         case 0: return launch_rows_g<false, false>(ra, R, W, grid, st);
         case 1: return launch_rows_g<true, false>(ra, R, W, grid, st);
         case 2: return launch_rows_g<false, true>(ra, R, W, grid, st);
-        default: return launch_rows_g<false, false, true>(ra, R, W, grid, st);
+        case 3: return launch_rows_g<false, false, true>(ra, R, W, grid, st);
+        case 4: return launch_rows_g<true, false, true>(ra, R, W, grid, st);
+        default: return launch_rows_g<false, true, true>(ra, R, W, grid, st);
     }
 }
 
@@ -2749,10 +3009,29 @@ hipError_t launch_adj_step(const AdjDev &a, int tj, int ichunk, const int (*box)
     return hipGetLastError();
 }
 
-hipError_t launch_adj_box_fix(const AdjDev &a, hipStream_t st) {
-    const long n = (long)(a.bn[0] + 2) * (a.bn[1] + 2) * (a.bn[2] + 2);
-    if (a.bn[0] <= 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_box_fix, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+long adj_box_cells(const AdjDev &a) {
+    return a.bn[0] <= 0 ? 0 : (long)(a.bn[0] + 2) * (a.bn[1] + 2) * (a.bn[2] + 2);
+}
+
+hipError_t launch_adj_box_fix(const AdjDev &a, const AdjInject &inj, hipStream_t st) {
+    const long n = adj_box_cells(a);
+    if (n <= 0) return inj.on ? hipErrorInvalidValue : hipSuccess;
+    // one thread beyond the cells for an injection outside them
+    hipLaunchKernelGGL(k_adj_box_fix, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, st, a, inj);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_box_corr(const AdjDev &a, float *corr, hipStream_t st) {
+    const long n = adj_box_cells(a);
+    if (n <= 0 || !corr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_adj_box_corr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, corr);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_box_add(const AdjDev &a, const AdjInject &inj, const float *corr, hipStream_t st) {
+    const long n = adj_box_cells(a);
+    if (n <= 0 || !corr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_adj_box_add, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, st, a, inj, corr);
     return hipGetLastError();
 }
 

@@ -72,6 +72,8 @@ struct fdtd_sim {
         float *Qb[3] = {}, *Eb[3] = {};
         double *grad = nullptr;
         bool rows = false;          // transposed step on the clean box by the row-blocked kernel
+        bool rows_slabs = false;    // and on the x and y slabs by its slab variants
+        float *corr = nullptr;      // amounts of the dispersive-cell correction (6 per cell of the box grown by one)
         unsigned *aux_d = nullptr;  // its table image (adjoint psi pointers)
     } adj;
     // misc
@@ -98,6 +100,8 @@ struct fdtd_sim {
     std::vector<uint32_t> aux_img;  // host copy of it
     bool concurrent = true;         // side slabs on a second stream beside the clean box (FDTD_CONCURRENT=0 to serialise)
     hipStream_t stream2 = nullptr;
+    // ev_line: the 1D line's completion in a step; the transposed step reuses it
+    // for the dispersive update or the clean box (FDTD_ADJ_SCHED 2, 1)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_line = nullptr;
     // side pieces of fdtd_step on separate streams (FDTD_SIDE_STREAMS: 1 = all on stream2, 3 = x
     // slabs after the line on stream2, y slabs on stream3, corner columns on stream4)
@@ -116,7 +120,11 @@ struct fdtd_sim {
     // at most 5 waves, else one (see rows_tiles)
     int tiles = 0, stiles = 0;
     int schunk = 16;                // x chunk of the side slabs
-    int adj_schunk = 8;             // x chunk of the side slabs in the transposed step (FDTD_ADJ_SCHUNK)
+    int adj_schunk = 8;             // x chunk of the side slabs in the transposed step, general kernel (FDTD_ADJ_SCHUNK)
+    int adj_rchunk = 8;             // x chunk of the side slabs in the transposed step, row-blocked variants (FDTD_ADJ_RCHUNK)
+    int adj_sched = 3;              // transposed dispersive update and correction: 0 main stream, 1 corner stream, 2 update there, correction on main,
+                                    // 3 update and correction amounts there, their addition on main (FDTD_ADJ_SCHED)
+    bool adj_fuse_inject = true;    // the next step's injection in the correction launch (FDTD_ADJ_FUSE_INJECT)
     int adj_stj = 16;               // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ; 4, 8, 16 measured ratios 3.14, 3.16, 3.07 over 2000 steps)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only

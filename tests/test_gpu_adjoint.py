@@ -7,7 +7,8 @@ the largest gradient entry.  Checkpoint spacing, and keeping the trajectory
 instead of recomputing, must not change a single bit of the gradient (every
 segment is recomputed by the same kernels), and the state left behind is the
 plain forward state after nsteps.  The transposed step on the row-blocked
-kernel (clean box) agrees with the general transposed kernel to rounding.
+kernel (clean box, and its variants on the x and y slabs) agrees with the
+general transposed kernel to rounding.
 """
 import os
 
@@ -105,19 +106,21 @@ def test_gpu_adjoint_autograd_wrapper():
 
 
 
-def _adjoint_with_path(path, ck=40, shape=(5, 4, 3)):
-    old = os.environ.get("FDTD_ADJ_PATH")
-    os.environ["FDTD_ADJ_PATH"] = path
+def _adjoint_with_path(path, ck=40, shape=(5, 4, 3), slabs="rows", r0=None, extra=None):
+    env = {"FDTD_ADJ_PATH": path, "FDTD_ADJ_SLABS": slabs, **(extra or {})}
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
-        p, rho, lo, r0 = problem(G, shape)
+        p, rho, lo, r0d = problem(G, shape)
         sim = G.Simulation(p)
         sim.set_design(lo, rho)
-        return sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=ck)
+        return sim.adjoint(STEPS, r0 or r0d, C0 / LAMS, checkpoint=ck)
     finally:
-        if old is None:
-            del os.environ["FDTD_ADJ_PATH"]
-        else:
-            os.environ["FDTD_ADJ_PATH"] = old
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 @pytest.mark.gpu
@@ -128,6 +131,60 @@ def test_gpu_adjoint_row_kernel_matches_general_kernel():
     print("row vs general transposed step: max |dg| / max |g| = %.2e" % err, flush=True)
     assert Jr == Jg
     assert err < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("where", ["x slab", "y slab"])
+def test_gpu_adjoint_slab_variants_match_general_kernel_and_oracle(where):
+    """Objective cell inside the x-slab (or y-slab) CPML: the adjoint source
+    starts in the slab, so the gradient on the block is carried there by the
+    transposed slab kernels.  The row-blocked slab variants against the
+    general transposed kernel on the slabs (same clean box) and on the whole
+    grid, and against the FP64 oracle.  The objective there is the block's
+    scattered field alone (about 3.5e-5 of the field at the default cell, so
+    the FP32 TFSF leakage weighs more in it): 2e-2 against the oracle, 1e-5
+    between the paths, whose forward passes are the same."""
+    c = (20 + 2 * 5 + 1) // 2
+    r0 = (3, c + 1, c) if where == "x slab" else (c, 3, c)
+    Jr, gr = _adjoint_with_path("rows", r0=r0)
+    Js, gs = _adjoint_with_path("rows", slabs="general", r0=r0)
+    Jg, gg = _adjoint_with_path("general", r0=r0)
+    po, rhoo, loo, _ = problem(O)
+    o = O.Oracle(po)
+    o.set_design(loo, rhoo.astype(np.float32).astype(np.float64))
+    o.set_objective(r0, C0 / LAMS)
+    Jo, go = o.adjoint(STEPS)
+    scale = np.abs(gg).max()
+    es, eg, eo = (np.abs(gr - x).max() / scale for x in (gs, gg, go))
+    print("%s: slab rows vs slab general %.2e, vs general %.2e, vs oracle %.2e (max|g| %.3e)"
+          % (where, es, eg, eo, scale), flush=True)
+    assert scale > 0
+    assert Jr == Js == Jg
+    assert es < 1e-5 and eg < 1e-5
+    assert Jr == pytest.approx(Jo, rel=2e-2)
+    assert eo < 2e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("where", ["above the box", "in the box margin"])
+def test_gpu_adjoint_schedules_give_the_same_bits(where):
+    """The transposed dispersive update beside the clean box (schedules 1-3,
+    3 with the correction amounts computed there and added after the clean
+    box) and the next step's injection folded into the dispersive-cell correction
+    (for an objective cell outside and inside the corrected cells) change no
+    bit against the plain order: update, injection and correction on the
+    main stream."""
+    shape = (5, 4, 3)
+    _, _, lo, r0 = problem(G, shape)
+    if where == "in the box margin":
+        r0 = (r0[0], r0[1], lo[2] + shape[2])
+    J0, g0 = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_SCHED": "0", "FDTD_ADJ_FUSE_INJECT": "0"})
+    for sched, fuse in (("3", "1"), ("3", "0"), ("2", "1"), ("1", "1"), ("1", "0"), ("0", "1")):
+        J, g = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_SCHED": sched, "FDTD_ADJ_FUSE_INJECT": fuse})
+        print(where, "schedule", sched, "fused injection", fuse, "max |dg|", np.abs(g - g0).max(), flush=True)
+        assert J == J0
+        assert np.array_equal(g, g0)
+    assert np.abs(g0).max() > 0
 
 
 @pytest.mark.gpu
