@@ -1007,6 +1007,10 @@ int adj_alloc(fdtd_sim *s) {
     if (const char *v = std::getenv("FDTD_ADJ_SCHUNK")) s->adj_schunk = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_RCHUNK")) s->adj_rchunk = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_SCHED")) s->adj_sched = std::min(3, std::max(0, std::atoi(v)));
+    if (const char *v = std::getenv("FDTD_ADJ_TILES")) s->adj_tiles = std::atoi(v) == 2 ? 2 : 1;
+    if (const char *v = std::getenv("FDTD_ADJ_ROWS")) s->adj_rows = std::atoi(v) == 3 ? 3 : 2;
+    if (const char *v = std::getenv("FDTD_ADJ_CCHUNK")) s->adj_cchunk = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_ADJ_STILES")) s->adj_stiles = std::atoi(v) == 2 ? 2 : 1;
     if (const char *v = std::getenv("FDTD_ADJ_FUSE_INJECT")) s->adj_fuse_inject = std::atoi(v) != 0;
     if (const char *v = std::getenv("FDTD_ADJ_STJ")) {
         const int tj = std::atoi(v);
@@ -1190,7 +1194,7 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
     auto slab_rows = [&](const int (*b)[4], int n, int variant, hipStream_t st) {
         RowsArgs q = r;
         q.ichunk = s->adj_rchunk;
-        q.tiles = 1;
+        q.tiles = s->adj_stiles * rows_waves(s->g.NZ) <= 16 ? s->adj_stiles : 1;
         q.npiece = n;
         std::memcpy(q.box, b, sizeof(q.box[0]) * n);
         return launch_step_rows(q, 2, variant, st);
@@ -1212,7 +1216,10 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
     if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, sc));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
-    HIPCHK(s, launch_step_rows(r, s->rows, 3, s->stream));
+    const int arows = s->adj_rows ? s->adj_rows : s->rows;
+    r.tiles = arows == 2 && s->adj_tiles * rows_waves(s->g.NZ) <= 16 ? s->adj_tiles : 1;
+    if (s->adj_cchunk) r.ichunk = s->adj_cchunk;
+    HIPCHK(s, launch_step_rows(r, arows, 3, s->stream));
     AdjInject none{};
     const AdjInject &inj = s->nbox > 0 && next.on ? next : none;
     if (s->nbox) {

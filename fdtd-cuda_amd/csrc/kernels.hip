@@ -2884,11 +2884,11 @@ hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st) {
 template <int W, bool XG, bool YG, bool ADJ>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
     if (ra.tiles == 2) {
-        if constexpr (W * 2 <= 16 && !ADJ) {
+        if constexpr (W * 2 <= 16) {
             switch (R) {
-                case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, false, 2>), grid, dim3(64, 2 * W), 0, st, ra); break;
+                case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, ADJ, 2>), grid, dim3(64, 2 * W), 0, st, ra); break;
                 case 3:
-                    if constexpr (!XG && !YG) {
+                    if constexpr (!XG && !YG && !ADJ) {
                         hipLaunchKernelGGL((k_step_rows<W, 3, false, false, false, 2>), grid, dim3(64, 2 * W), 0, st, ra);
                         break;
                     }
@@ -2949,7 +2949,8 @@ long rows_prepare(RowsArgs &ra, int R, int variant) {
     if (variant >= 4 && R != 2) return -1;   // the adjoint slab variants are built for two rows
     const int W = rows_waves(ra.NZ);
     if (ra.tiles < 1) ra.tiles = 1;
-    if (ra.tiles != 1 && (ra.tiles < 2 || ra.tiles > 3 || ra.tiles * W > 16 || variant >= 3 || (variant != 0 && ra.tiles != 2)))
+    if (ra.tiles != 1 && (ra.tiles < 2 || ra.tiles > 3 || ra.tiles * W > 16 || (variant != 0 && ra.tiles != 2) ||
+                          (variant >= 3 && R != 2)))
         return -1;
     const bool xg = variant == 1 || variant == 4, yg = variant == 2 || variant == 5;
     const int T = ra.tiles;

@@ -125,6 +125,10 @@ struct fdtd_sim {
     int adj_sched = 3;              // transposed dispersive update and correction: 0 main stream, 1 corner stream, 2 update there, correction on main,
                                     // 3 update and correction amounts there, their addition on main (FDTD_ADJ_SCHED)
     bool adj_fuse_inject = true;    // the next step's injection in the correction launch (FDTD_ADJ_FUSE_INJECT)
+    int adj_tiles = 1;              // row tiles per block of the transposed clean box, two rows (FDTD_ADJ_TILES: 1, 2)
+    int adj_stiles = 1;             // the same for its x and y slabs (FDTD_ADJ_STILES: 1, 2)
+    int adj_rows = 0;               // rows per thread of the transposed clean box (FDTD_ADJ_ROWS; 0: as the forward step)
+    int adj_cchunk = 0;             // its x chunk (FDTD_ADJ_CCHUNK; 0: as the forward clean box)
     int adj_stj = 16;               // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ; 4, 8, 16 measured ratios 3.14, 3.16, 3.07 over 2000 steps)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
