@@ -996,7 +996,9 @@ int adj_alloc(fdtd_sim *s) {
     for (int c = 0; c < 3; c++) {
         for (int q = 0; q < 7; q++)
             if ((rc = dalloc(s, &s->adj.st[c][q], s->nbox))) return rc;
-        if ((rc = dalloc(s, &s->adj.Qb[c], s->nbox)) || (rc = dalloc(s, &s->adj.Eb[c], s->nbox))) return rc;
+        if ((rc = dalloc(s, &s->adj.Qb[c], s->nbox)) || (rc = dalloc(s, &s->adj.Eb[c], s->nbox)) ||
+            (rc = dalloc(s, &s->adj.dq[c], s->nbox)) || (rc = dalloc(s, &s->adj.db[c], s->nbox)))
+            return rc;
     }
     if ((rc = dalloc(s, &s->adj.grad, s->nbox))) return rc;
     // transposed step on the clean box with the row-blocked kernel (variant 3):
@@ -1009,6 +1011,11 @@ int adj_alloc(fdtd_sim *s) {
     if (const char *v = std::getenv("FDTD_ADJ_SCHED")) s->adj_sched = std::min(3, std::max(0, std::atoi(v)));
     if (const char *v = std::getenv("FDTD_ADJ_TILES")) s->adj_tiles = std::atoi(v) == 2 ? 2 : 1;
     if (const char *v = std::getenv("FDTD_ADJ_ROWS")) s->adj_rows = std::atoi(v) == 3 ? 3 : 2;
+    if (const char *v = std::getenv("FDTD_ADJ_CTJ")) {
+        const int tj = std::atoi(v);
+        if (tj == 4 || tj == 8 || tj == 16) s->adj_ctj = tj;
+    }
+    if (const char *v = std::getenv("FDTD_ADJ_CICH")) s->adj_cich = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_CCHUNK")) s->adj_cchunk = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_STILES")) s->adj_stiles = std::atoi(v) == 2 ? 2 : 1;
     if (const char *v = std::getenv("FDTD_ADJ_FUSE_INJECT")) s->adj_fuse_inject = std::atoi(v) != 0;
@@ -1137,6 +1144,7 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
         av.Ein[c] = s->adj.E[pa][c]; av.Hin[c] = s->adj.H[pa][c];
         av.Eout[c] = s->adj.E[1 - pa][c]; av.Hout[c] = s->adj.H[1 - pa][c];
         av.bid[c] = s->disp_id[c]; av.Qb[c] = s->adj.Qb[c]; av.Eb[c] = s->adj.Eb[c];
+        av.dq[c] = s->adj.dq[c]; av.db[c] = s->adj.db[c];
     }
     for (int x = 0; x < 6; x++) {
         av.psiE_in.p[x] = s->adj.psiE[pe][x]; av.psiE_out.p[x] = s->adj.psiE[1 - pe][x];
@@ -1213,7 +1221,7 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
         for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
         if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
     }
-    if (s->ncorner) HIPCHK(s, launch_adj_step(av, 4, 2, s->cbox4, s->ncorner, sc));
+    if (s->ncorner) HIPCHK(s, launch_adj_step(av, s->adj_ctj, s->adj_cich, s->cbox4, s->ncorner, sc));
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
     const int arows = s->adj_rows ? s->adj_rows : s->rows;
@@ -1377,11 +1385,15 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
                 for (int q = 0; q < 7; q++) ad.a[c][q] = s->adj.st[c][q];
                 ad.Qb[c] = s->adj.Qb[c];
                 ad.Eb[c] = s->adj.Eb[c];
+                ad.dq[c] = s->adj.dq[c];
+                ad.db[c] = s->adj.db[c];
                 ad.tm1[c] = traj + ((size_t)st * 3 + c) * nbox;
                 ad.t0[c] = traj + ((size_t)(st + 1) * 3 + c) * nbox;
                 ad.tp1[c] = traj + ((size_t)(st + 2) * 3 + c) * nbox;
             }
             ad.grad = s->adj.grad;
+            ad.Ceh = (float)S.Ceh;
+            ad.Cee = (float)S.Cee;
             const AdjInject nx = fuse && t > t0 ? inject_at(t - 1) : AdjInject{};
             if ((rc = adjoint_step(s, pa, pe, ph, ad, nx, &injected))) return rc;
             pa ^= 1; pe ^= 1; ph ^= 1;

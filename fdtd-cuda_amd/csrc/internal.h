@@ -146,6 +146,8 @@ struct AdjDev {
     int blo[3], bn[3];
     const uint8_t *bid[3];
     const float *Qb[3], *Eb[3];
+    // per box cell, Qb - Ceh Ehat and Eb - Cee Ehat (zero at cells that are not dispersive)
+    const float *dq[3], *db[3];
 };
 
 // The objective's injection dJ/dE at one cell, added to the E adjoint by
@@ -164,6 +166,8 @@ struct AdjDispDev {
     const float *Ehat[3];           // adjoint of E^{n+1} (field layout)
     float *a[3][7];                 // adjoint state: 0 E^{n-1}, 1+2q P_q, 2+2q J_q
     float *Qb[3], *Eb[3];
+    float *dq[3], *db[3];           // Qb - Ceh Ehat, Eb - Cee Ehat (the correction after the row kernel), or null
+    float Ceh, Cee;
     const float *tm1[3], *t0[3], *tp1[3];   // E^{n-1}, E^n, E^{n+1} on the box (gradient)
     double *grad;                   // dJ/drho per box cell, or null
 };

@@ -2495,12 +2495,18 @@ __global__ __launch_bounds__(64 * TJ) void k_adj_step(AdjDev a, int ichunk, int 
 // Q = Qb instead of Ceh Ehat and the base Eb instead of Cee Ehat; the step is
 // linear, so the outputs take dV = (Qb - Ceh Ehat) 1/(kappa d) through the
 // forward differences of the H half, dH, and then Eb - Cee Ehat plus the
-// backward differences of dW = Che dH 1/(kappa d) in the E half.  One thread
-// per cell of the box grown by one cell on every side; the region must be
-// free of CPML (checked by the host).
+// backward differences of dW = Che dH 1/(kappa d) in the E half (both
+// differences per box cell formed by k_adj_disp: dq, db).  One thread per cell
+// of the box grown by one cell on every side; the region must be free of CPML
+// (checked by the host).
+__device__ inline long adj_box_any(const AdjDev &a, int i, int j, int k) {
+    const int bi = i - a.blo[0], bj = j - a.blo[1], bk = k - a.blo[2];
+    if (bi < 0 || bj < 0 || bk < 0 || bi >= a.bn[0] || bj >= a.bn[1] || bk >= a.bn[2]) return -1;
+    return ((long)bi * a.bn[1] + bj) * a.bn[2] + bk;
+}
 __device__ inline float adj_dq(const AdjDev &a, int c, int i, int j, int k) {
-    const long b = adj_box(a, c, i, j, k);
-    return b < 0 ? 0.f : a.Qb[c][b] - a.c.Ceh * a.Ein[c][gidx(a.g, i, j, k)];
+    const long b = adj_box_any(a, i, j, k);
+    return b < 0 ? 0.f : a.dq[c][b];
 }
 __device__ inline void adj_dh(const AdjDev &a, int i, int j, int k, float &hx, float &hy, float &hz) {
     const float ix = a.cp[0].ik_e[i], iy = a.cp[1].ik_e[j], iz = a.cp[2].ik_e[k];
@@ -2538,15 +2544,14 @@ __device__ inline void box_corr(const AdjDev &a, int i, int j, int k, long p, fl
     ey += Che * (hz * hx_i - zm * ikxm);
     ez -= Che * (hy * hx_i - ym * ikxm);
     (void)t1;
-    for (int c = 0; c < 3; c++) {
-        const long b = adj_box(a, c, i, j, k);
-        if (b >= 0) {
-            const float db = a.Eb[c][b] - a.c.Cee * a.Ein[c][p];
-            if (c == 0) ex += db;
-            else if (c == 1) ey += db;
-            else ez += db;
-        }
+    const long b = adj_box_any(a, i, j, k);
+    if (b >= 0) {
+        // zero at cells that are not dispersive
+        ex += a.db[0][b];
+        ey += a.db[1][b];
+        ez += a.db[2][b];
     }
+    (void)p;
     h[0] = hx; h[1] = hy; h[2] = hz;
     e[0] = ex; e[1] = ey; e[2] = ez;
 }
@@ -2625,7 +2630,10 @@ __global__ void k_adj_disp(AdjDispDev A) {
     const long p = gidx(d.g, d.lo[0] + bi, d.lo[1] + bj, d.lo[2] + bk);
     double gsum = 0.0;
     for (int c = 0; c < 3; c++) {
-        if (!d.id[c][idx]) continue;
+        if (!d.id[c][idx]) {
+            if (A.dq[c]) { A.dq[c][idx] = 0.f; A.db[c][idx] = 0.f; }
+            continue;
+        }
         float a0 = d.a0, a1 = d.a1, a2 = d.a2, r = 1.f, inv = 0.f;
         float e0[3] = {d.e0[0], d.e0[1], d.e0[2]}, e1[3] = {d.e1[0], d.e1[1], d.e1[2]};
         if (d.rho) {
@@ -2638,7 +2646,8 @@ __global__ void k_adj_disp(AdjDispDev A) {
         }
         float *const *s = A.a[c];
         float Jt[3];
-        float et = A.Ehat[c][p];
+        const float er = A.Ehat[c][p];
+        float et = er;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             Jt[q] = s[2 + 2 * q][idx] + s[1 + 2 * q][idx];   // J' feeds P' = P + J'
@@ -2654,8 +2663,13 @@ __global__ void k_adj_disp(AdjDispDev A) {
             s[2 + 2 * q][idx] = d.g1[q] * Jt[q] - e1[q] * et;
         }
         s[0][idx] = e1n;
-        A.Qb[c][idx] = a0 * et;
+        const float qb = a0 * et;
+        A.Qb[c][idx] = qb;
         A.Eb[c][idx] = eb;
+        if (A.dq[c]) {
+            A.dq[c][idx] = qb - A.Ceh * er;
+            A.db[c][idx] = eb - A.Cee * er;
+        }
         if (A.grad && d.rho) {
             const float Ep = A.tp1[c][idx], En = A.t0[c][idx], Em = A.tm1[c][idx];
             const float dE = (d.u2 * En - d.u3 * Em - d.u1 * Ep) * inv;

@@ -70,6 +70,7 @@ struct fdtd_sim {
         float *psiE[2][6] = {}, *psiH[2][6] = {};
         float *st[3][7] = {};
         float *Qb[3] = {}, *Eb[3] = {};
+        float *dq[3] = {}, *db[3] = {};   // Qb - Ceh Ehat, Eb - Cee Ehat per box cell
         double *grad = nullptr;
         bool rows = false;          // transposed step on the clean box by the row-blocked kernel
         bool rows_slabs = false;    // and on the x and y slabs by its slab variants
@@ -129,6 +130,7 @@ struct fdtd_sim {
     int adj_stiles = 1;             // the same for its x and y slabs (FDTD_ADJ_STILES: 1, 2)
     int adj_rows = 0;               // rows per thread of the transposed clean box (FDTD_ADJ_ROWS; 0: as the forward step)
     int adj_cchunk = 0;             // its x chunk (FDTD_ADJ_CCHUNK; 0: as the forward clean box)
+    int adj_ctj = 8, adj_cich = 2;  // tile rows and x chunk of the transposed corner columns (FDTD_ADJ_CTJ, FDTD_ADJ_CICH)
     int adj_stj = 16;               // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ; 4, 8, 16 measured ratios 3.14, 3.16, 3.07 over 2000 steps)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only
