@@ -150,6 +150,25 @@ def test_batch_graph_direct_and_alone_agree_bitwise(general):
         assert a[i].get_info().sample_count == c[i].get_info().sample_count
 
 
+@pytest.mark.parametrize("ctj", [8, 16])
+def test_batch_corner_tiles_agree_bitwise(ctj, monkeypatch):
+    """Corner columns of 8 and 16 tile rows (FDTD_CTJ; the batch groups its
+    corner launches by tile rows): graph replay and each simulation alone
+    agree bit for bit, as with the default 4."""
+    monkeypatch.setenv("FDTD_CTJ", str(ctj))
+    steps = 61
+    a = sims_for(CASES)
+    c = sims_for(CASES)
+    ba = G.Batch(a, flags=G.BATCH_NO_TUNE)
+    ba.run(steps)
+    for s in c:
+        s.step(steps)
+    for i in range(len(CASES)):
+        fa, fc = fields(a[i]), fields(c[i])
+        for k in range(6):
+            assert np.array_equal(fa[k], fc[k]), (ctj, i, k)
+
+
 @pytest.mark.parametrize("general", [False, True])
 def test_batch_runs_interleaved_with_single_steps(general):
     """Batch runs split by fdtd_step calls on every simulation (odd counts,
