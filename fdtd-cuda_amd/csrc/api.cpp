@@ -186,10 +186,41 @@ DispDev make_disp(fdtd_sim *s, int parity) {
     return d;
 }
 
+// Frees the adjoint's buffers (sized by the grid and the dispersive box),
+// so a new design or box allocates them afresh instead of beside them.
+void adj_release(fdtd_sim *s) {
+    (void)hipDeviceSynchronize();
+    for (int q = 0; q < 2; q++)
+        for (int c = 0; c < 3; c++) {
+            if (s->adj.E[q][c]) { float *b = s->adj.E[q][c] - s->fbase; dfree(s, b); s->adj.E[q][c] = nullptr; }
+            if (s->adj.H[q][c]) { float *b = s->adj.H[q][c] - s->fbase; dfree(s, b); s->adj.H[q][c] = nullptr; }
+        }
+    for (int q = 0; q < 2; q++)
+        for (int a = 0; a < 6; a++) { dfree(s, s->adj.psiE[q][a]); dfree(s, s->adj.psiH[q][a]); }
+    for (int c = 0; c < 3; c++) {
+        for (int q = 0; q < 7; q++) dfree(s, s->adj.st[c][q]);
+        dfree(s, s->adj.Qb[c]); dfree(s, s->adj.Eb[c]); dfree(s, s->adj.dq[c]); dfree(s, s->adj.db[c]);
+    }
+    dfree(s, s->adj.grad);
+    dfree(s, s->adj.corr);
+    dfree(s, s->adj.aux_d);
+    s->adj.ready = false;
+    s->adj.rows = s->adj.rows_slabs = false;
+}
+
 // Builds the dispersive box from the masks and uploads ids; state zeroed.
+// The buffers of an earlier box (and the adjoint's, sized by it) are freed first.
 int build_dispersive(fdtd_sim *s) {
     const Setup &S = s->S;
     const int NX = S.N[0], NY = S.N[1], NZ = S.N[2];
+    adj_release(s);
+    for (int c = 0; c < 3; c++) {
+        dfree(s, s->disp_id[c]);
+        for (int q = 0; q < 8; q++) dfree(s, s->disp_st[c][q]);
+        dfree(s, s->h_vnew[c]); dfree(s, s->h_nd[c]); dfree(s, s->h_ndp[c]);
+        dfree(s, s->nl_gd[c]);
+    }
+    dfree(s, s->rho_box);
     int lo[3] = {NX, NY, NZ}, hi[3] = {-1, -1, -1};
     for (int c = 0; c < 3; c++) s->n_disp[c] = 0;
     for (int c = 0; c < 3; c++) {
@@ -982,6 +1013,7 @@ void ckpt_free(Checkpoint &ck) {
 
 int adj_alloc(fdtd_sim *s) {
     if (s->adj.ready) return FDTD_OK;
+    adj_release(s);
     int rc;
     for (int q = 0; q < 2; q++)
         for (int c = 0; c < 3; c++) {

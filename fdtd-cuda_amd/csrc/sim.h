@@ -165,6 +165,20 @@ int dalloc(fdtd_sim *s, T **ptr, size_t count) {
     return FDTD_OK;
 }
 
+// Frees a device buffer of the handle (one dalloc'd or uploaded) and clears
+// the pointer; the caller has synchronised the device.
+template <typename T>
+void dfree(fdtd_sim *s, T *&ptr) {
+    if (!ptr) return;
+    for (size_t n = 0; n < s->allocs.size(); n++)
+        if (s->allocs[n] == (void *)ptr) {
+            s->allocs.erase(s->allocs.begin() + (long)n);
+            (void)hipFree((void *)ptr);
+            break;
+        }
+    ptr = nullptr;
+}
+
 // Allocation zero-fills run on the solver stream, so uploads are queued on the
 // same stream (and waited for, the host vector being temporary) to keep them
 // ordered after the fill.

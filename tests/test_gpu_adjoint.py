@@ -204,3 +204,48 @@ def test_gpu_adjoint_larger_block_against_oracle():
     print("J", J, Jo, "err", np.abs(g - go).max() / np.abs(go).max(), flush=True)
     assert J == pytest.approx(Jo, rel=1e-3)
     assert np.abs(g - go).max() < 2e-3 * np.abs(go).max()
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_repeated_designs_keep_device_memory_flat():
+    """An optimisation loop on one handle: a new design block (two sizes in
+    turn, so the dispersive box is rebuilt) before every adjoint.  The box and
+    adjoint buffers of the last design are freed before the next ones are
+    allocated, so the device memory in use stays flat (before, each round
+    added the twelve adjoint field arrays, about 17 MB on this 71^3 grid)."""
+    import ctypes
+
+    def in_use():
+        # the HIP runtime the library itself mapped (another copy of the runtime,
+        # such as one bundled with torch, would see a different device context)
+        paths = [ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln and "torch" not in ln]
+        assert paths, "the library's HIP runtime is not mapped"
+        hip = ctypes.CDLL(paths[0])
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+        return total.value - free.value
+
+    p = small_params(G, n=60, ncpml=5, sphere=0, steps=120)
+    p.inc_phi = 90.0
+    T = 60 + 2 * 5 + 1
+    c = T // 2
+    sim = G.Simulation(p)
+    rng = np.random.default_rng(5)
+    used = []
+    try:
+        for it in range(6):
+            shape = (5, 4, 3) if it % 2 == 0 else (6, 6, 4)
+            lo = (c - shape[0] // 2, c - shape[1] // 2, c - shape[2] // 2)
+            r0 = (c, c + 1, lo[2] + shape[2] + 1)
+            sim.set_design(lo, rng.uniform(0.1, 0.9, size=shape).astype(np.float32))
+            J, g = sim.adjoint(40, r0, C0 / LAMS, checkpoint=20)
+            assert np.isfinite(J) and np.all(np.isfinite(g))
+            used.append(in_use())
+        growth = max(used[2:]) - used[1]
+        print("device memory in use per round (MB):", [round(u / 2**20, 1) for u in used],
+              "growth after round 2: %.1f MB" % (growth / 2**20), flush=True)
+        assert growth < 8 * 2**20
+    finally:
+        sim.close()
+
