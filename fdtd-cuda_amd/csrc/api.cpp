@@ -1613,6 +1613,10 @@ int fdtd_set_point_monitor(fdtd_sim *s, const int r[3], int capacity) {
     for (int a = 0; a < 3; a++)
         if (r[a] < 0 || r[a] >= s->S.N[a]) return set_err(s, FDTD_ERR_INVALID, "point monitor outside the grid");
     if (capacity > s->mon_cap_alloc) {
+        // the old buffer goes once the launches queued into it have run
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        dfree(s, s->mon);
+        s->mon_cap_alloc = 0;
         float *m = nullptr;
         if (int rc = dalloc(s, &m, (size_t)3 * capacity)) return rc;
         s->mon = m;

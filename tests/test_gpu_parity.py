@@ -224,7 +224,8 @@ def test_point_monitor_gap_field_of_a_dimer():
     o.set_materials(s.get_materials(1), s.get_materials(2), s.get_materials(3))
     r0 = (c, c, c)
     freqs = np.array([3.8e14, 4.6e14, 5.5e14, 6.7e14])
-    s.set_point_monitor(r0, 600)
+    s.set_point_monitor(r0, 100)
+    s.set_point_monitor(r0, 600)   # a larger capacity replaces the first buffer
     o.set_objective(r0, freqs)
     s.step(500)
     o.step(500)
