@@ -171,7 +171,10 @@ int  fdtd_add_rod(fdtd_sim *sim, const double centre[3], int axis, double length
  * [lo, lo+n) of cells becomes dispersive for all three E components with the
  * ADE coefficients blended by the density rho in [0, 1] (i-major, k fastest):
  * eps_inf(rho) = 1 + rho (eps_inf - 1) and every pole drive scaled by rho,
- * so rho = 0 is vacuum and rho = 1 the configured metal. */
+ * so rho = 0 is vacuum and rho = 1 the configured metal.  Rebuilding the
+ * dispersive box frees the device buffers sized by the previous one (box
+ * state and the adjoint's), so a design loop on one handle keeps its device
+ * memory flat. */
 int  fdtd_set_design(fdtd_sim *sim, const int lo[3], const int n[3], const float *rho);
 /* Discrete adjoint (no reference counterpart).  From the current state (t = 0)
  * runs nsteps forward with a full-state checkpoint every ckpt steps, then the
