@@ -124,11 +124,24 @@ def _adjoint_with_path(path, ck=40, shape=(5, 4, 3), slabs="rows", r0=None, extr
 
 
 @pytest.mark.gpu
-def test_gpu_adjoint_row_kernel_matches_general_kernel():
-    Jr, gr = _adjoint_with_path("rows")
-    Jg, gg = _adjoint_with_path("general")
+@pytest.mark.parametrize("rows", ["2", "3"])
+def test_gpu_adjoint_row_kernel_matches_general_kernel(rows):
+    """Two or three rows per thread (FDTD_ROWS; the transposed clean box
+    follows the forward step's, the slab variants keep two)."""
+    old = os.environ.get("FDTD_ROWS")
+    os.environ["FDTD_ROWS"] = rows
+    try:
+        info = G.Simulation(problem(G)[0]).get_info()
+    finally:
+        if old is None:
+            del os.environ["FDTD_ROWS"]
+        else:
+            os.environ["FDTD_ROWS"] = old
+    assert info.fused_path == 1 and info.tile[0] == int(rows)
+    Jr, gr = _adjoint_with_path("rows", extra={"FDTD_ROWS": rows})
+    Jg, gg = _adjoint_with_path("general", extra={"FDTD_ROWS": rows})
     err = np.abs(gr - gg).max() / np.abs(gg).max()
-    print("row vs general transposed step: max |dg| / max |g| = %.2e" % err, flush=True)
+    print("rows %s: row vs general transposed step: max |dg| / max |g| = %.2e" % (rows, err), flush=True)
     assert Jr == Jg
     assert err < 1e-5
 
