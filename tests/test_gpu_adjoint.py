@@ -262,3 +262,41 @@ def test_gpu_adjoint_repeated_designs_keep_device_memory_flat():
     finally:
         sim.close()
 
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nz", [60, 130])
+def test_gpu_adjoint_taller_grids_row_kernel_matches_general_kernel(nz):
+    """z columns of two and three waves (NCELLZ 60 and 130 with a 5-cell
+    CPML): the row-blocked transposed step (clean box and slab variants)
+    against the general transposed kernel."""
+    res = {}
+    for path in ("rows", "general"):
+        old = os.environ.get("FDTD_ADJ_PATH")
+        os.environ["FDTD_ADJ_PATH"] = path
+        try:
+            p = small_params(G, n=20, ncpml=5, sphere=0, steps=STEPS)
+            p.NCELLZ = nz
+            p.inc_phi = 90.0
+            T = [20 + 11, 20 + 11, nz + 11]
+            shape = (5, 4, 3)
+            # the block at the height it has in the 31^3 problem, which the pulse reaches within the window
+            lo = (T[0] // 2 - 2, T[1] // 2 - 2, 14)
+            r0 = (T[0] // 2, T[1] // 2 + 1, lo[2] + shape[2] + 1)
+            sim = G.Simulation(p)
+            rho = np.random.default_rng(13).uniform(0.1, 0.9, size=shape).astype(np.float32)
+            sim.set_design(lo, rho)
+            res[path] = sim.adjoint(STEPS, r0, C0 / LAMS, checkpoint=40)
+            if path == "rows":
+                assert sim.get_info().fused_path == 1
+            sim.close()
+        finally:
+            if old is None:
+                del os.environ["FDTD_ADJ_PATH"]
+            else:
+                os.environ["FDTD_ADJ_PATH"] = old
+    (Jr, gr), (Jg, gg) = res["rows"], res["general"]
+    err = np.abs(gr - gg).max() / np.abs(gg).max()
+    print("NCELLZ %d: row vs general transposed step %.2e (max|g| %.3e)" % (nz, err, np.abs(gg).max()), flush=True)
+    assert Jr == Jg and np.abs(gg).max() > 1e-8
+    assert err < 1e-5
+
