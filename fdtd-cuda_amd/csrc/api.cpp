@@ -1048,6 +1048,8 @@ int adj_alloc(fdtd_sim *s) {
         if (tj == 4 || tj == 8 || tj == 16) s->adj_ctj = tj;
     }
     if (const char *v = std::getenv("FDTD_ADJ_CICH")) s->adj_cich = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("FDTD_ADJ_CORNER_ROWS")) s->adj_corner_rows = std::atoi(v) != 0;
+    if (const char *v = std::getenv("FDTD_ADJ_CRCHUNK")) s->adj_crchunk = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_CCHUNK")) s->adj_cchunk = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_STILES")) s->adj_stiles = std::atoi(v) == 2 ? 2 : 1;
     if (const char *v = std::getenv("FDTD_ADJ_FUSE_INJECT")) s->adj_fuse_inject = std::atoi(v) != 0;
@@ -1157,9 +1159,9 @@ namespace {
 
 // One transposed step: adjoint level pa -> 1 - pa, psi_E pe, psi_H ph.  On
 // the row path the clean box runs on the row-blocked kernel (variant 3) with
-// the dispersive-cell correction after it, the x and y slabs on its slab
-// variants (4, 5) and the corner columns on the general transposed kernel,
-// beside it on the side streams.
+// the dispersive-cell correction after it, the x and y slabs and the corner
+// columns on its slab and corner variants (4, 5, 6), beside it on the side
+// streams.
 //
 // ad: the transposed dispersive update of this step (before the correction of
 // the dispersive cells, which reads the box state it writes).  next: the
@@ -1253,7 +1255,18 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
         for (int q = 0; q < s->nybox; q++) std::memcpy(slabs[ns++], s->ybox[q], sizeof(slabs[0]));
         if (ns) HIPCHK(s, launch_adj_step(av, s->adj_stj, s->adj_schunk, slabs, ns, side));
     }
-    if (s->ncorner) HIPCHK(s, launch_adj_step(av, s->adj_ctj, s->adj_cich, s->cbox4, s->ncorner, sc));
+    if (s->ncorner) {
+        if (rs && s->adj_corner_rows) {
+            RowsArgs q = r;
+            q.ichunk = s->adj_crchunk;
+            q.tiles = 1;
+            q.npiece = s->ncorner;
+            std::memcpy(q.box, s->cbox4, sizeof(q.box[0]) * s->ncorner);
+            HIPCHK(s, launch_step_rows(q, 2, 6, sc));
+        } else {
+            HIPCHK(s, launch_adj_step(av, s->adj_ctj, s->adj_cich, s->cbox4, s->ncorner, sc));
+        }
+    }
     r.npiece = 1;
     std::memcpy(r.box[0], s->cbox, sizeof(s->cbox));
     const int arows = s->adj_rows ? s->adj_rows : s->rows;

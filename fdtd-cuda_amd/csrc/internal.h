@@ -241,7 +241,8 @@ int rows_waves(int NZ);
 // Chh = 1, Che = -Ceh, Cee, Ceh = -Che; the table image holds both adjoint
 // psi_E sets; dispersive cells are corrected afterwards by launch_adj_box_fix);
 // 4, 5: the transposed step on x slabs (as 1) and y slabs (as 2), two rows,
-// one row tile, no dispersive cell in the boxes.
+// one row tile, no dispersive cell in the boxes; 6: the transposed step on
+// the corner columns (x and y general), two rows, one row tile.
 hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st);
 
 // box[q] = {ia, ib, ja, jb}; one to four disjoint boxes.

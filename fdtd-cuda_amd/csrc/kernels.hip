@@ -703,8 +703,8 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // double buffered (pp: psi_H set read, pe: psi_E set read).  The lanes at
 // wave edges must lie outside the z slabs (checked by the host).
 //
-// ADJ with XG (x slabs) or YG (y slabs): the weights along the slab's CPML
-// axis are formed per cell with their adjoint psi recursions (V of rows
+// ADJ with XG (x slabs) or YG (y slabs), or both (corner columns): the weights
+// along each general axis are formed per cell with their adjoint psi recursions (V of rows
 // 0..R+1 of the plane along y; V of plane i+1, carried to the next step, and
 // W of plane i-1 along x) and the differences are taken of them; the other
 // in-plane axis is clean and keeps the swapped-coefficient form.  The slabs
@@ -722,8 +722,8 @@ __device__ inline void stn(char *b, unsigned o, float v) {
 // Tiles past the last row tile of the box repeat its loads and store nothing.
 template <int W, int R, bool XG, bool YG, bool ADJ, int T = 1>
 __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
-    static_assert(!ADJ || !XG || !YG, "the adjoint variant runs clean boxes, x slabs or y slabs");
-    constexpr bool AS = ADJ && (XG || YG);   // adjoint on a side slab
+    static_assert(ADJ || !XG || !YG, "corner columns of the forward step run on the general kernel");
+    constexpr bool AS = ADJ && (XG || YG);   // adjoint on a side slab or corner column
     static_assert(T >= 1 && W * T <= 16, "at most 16 waves per block");
     __shared__ float sK[2][2][W * T][R + 1];   // last lane's hy, hx per wave and row (H at k-1 of the next wave)
     __shared__ float sE[2][2][W * T][R + 1];   // first lane's ex, ey per wave and row (E at k+1 of the previous wave)
@@ -840,12 +840,15 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
         for (int r = 0; r <= R; r++) { P.hxz[r] = 0.f; P.hyz[r] = 0.f; P.exz[r] = 0.f; P.eyz[r] = 0.f; }
         const bool iA = !XG || ip < NX - 1, iC = !XG || (ip > 0 && ip < NX - 1);
         if (ADJ) {
-            // adjoint: psi_E of rows 0..R (z weights of the H half), psi_H of rows 1..R (E half)
+            // adjoint: psi_E of rows 0..R (z weights of the H half), psi_H of rows 1..R (E half).
+            // Rows clamped into the grid: on a y slab or corner column a tile's halo row
+            // can be j = -1 or j = NY (at planes 0 and NX-1 outside the arrays); their
+            // values are masked out by the domain tests
             if (ze && lze >= 0) {
                 const gfloat *pa = psi(ein + 0), *pb = psi(ein + 2);
 #pragma unroll
                 for (int r = 0; r <= R; r++) {
-                    const long q = ((long)ip * NY + jb + r) * Lze + lze;
+                    const long q = ((long)ip * NY + jq(r)) * Lze + lze;
                     P.exz[r] = pa[q];
                     P.eyz[r] = pb[q];
                 }
@@ -854,7 +857,7 @@ __device__ __forceinline__ void rows_body(const RowsArgs &ra, int L) {
                 const gfloat *pa = psi(pp * 6 + 0), *pb = psi(pp * 6 + 2);
 #pragma unroll
                 for (int r = 1; r <= R; r++) {
-                    const long q = ((long)ip * NY + jb + r) * Lzh + lzh;
+                    const long q = ((long)ip * NY + jq(r)) * Lzh + lzh;
                     P.hxz[r] = pa[q];
                     P.hyz[r] = pb[q];
                 }
@@ -2898,7 +2901,7 @@ hipError_t launch_batch_advance(int *dyn, int dt, int dslot, hipStream_t st) {
 template <int W, bool XG, bool YG, bool ADJ>
 static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_t st) {
     if (ra.tiles == 2) {
-        if constexpr (W * 2 <= 16) {
+        if constexpr (W * 2 <= 16 && !(XG && YG)) {
             switch (R) {
                 case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, ADJ, 2>), grid, dim3(64, 2 * W), 0, st, ra); break;
                 case 3:
@@ -2927,7 +2930,7 @@ static hipError_t launch_rows_w(const RowsArgs &ra, int R, dim3 grid, hipStream_
     switch (R) {
         case 2: hipLaunchKernelGGL((k_step_rows<W, 2, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra); break;
         case 3:
-            if constexpr (!ADJ || (!XG && !YG)) {
+            if constexpr (!(XG && YG) && (!ADJ || (!XG && !YG))) {
                 hipLaunchKernelGGL((k_step_rows<W, 3, XG, YG, ADJ>), grid, dim3(64, W), 0, st, ra);
                 break;
             }
@@ -2959,14 +2962,15 @@ int rows_waves(int NZ) {
 }
 
 long rows_prepare(RowsArgs &ra, int R, int variant) {
-    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 5) return -1;
-    if (variant >= 4 && R != 2) return -1;   // the adjoint slab variants are built for two rows
+    if (R < 2 || R > 3 || ra.ichunk < 1 || ra.npiece < 1 || ra.npiece > 4 || variant < 0 || variant > 6) return -1;
+    if (variant >= 4 && R != 2) return -1;   // the slab and corner variants of the transposed step are built for two rows
     const int W = rows_waves(ra.NZ);
     if (ra.tiles < 1) ra.tiles = 1;
     if (ra.tiles != 1 && (ra.tiles < 2 || ra.tiles > 3 || ra.tiles * W > 16 || (variant != 0 && ra.tiles != 2) ||
                           (variant >= 3 && R != 2)))
         return -1;
-    const bool xg = variant == 1 || variant == 4, yg = variant == 2 || variant == 5;
+    const bool xg = variant == 1 || variant == 4 || variant == 6, yg = variant == 2 || variant == 5 || variant == 6;
+    if (variant == 6 && ra.tiles != 1) return -1;   // the corner variant is built for one row tile
     const int T = ra.tiles;
     ra.nseg = (ra.NZ + 1 + 64 * W - 3) / (64 * W - 2);
     long total = 0;
@@ -2994,7 +2998,8 @@ hipError_t launch_step_rows(RowsArgs ra, int R, int variant, hipStream_t st) {
         case 2: return launch_rows_g<false, true>(ra, R, W, grid, st);
         case 3: return launch_rows_g<false, false, true>(ra, R, W, grid, st);
         case 4: return launch_rows_g<true, false, true>(ra, R, W, grid, st);
-        default: return launch_rows_g<false, true, true>(ra, R, W, grid, st);
+        case 5: return launch_rows_g<false, true, true>(ra, R, W, grid, st);
+        default: return launch_rows_g<true, true, true>(ra, R, W, grid, st);
     }
 }
 

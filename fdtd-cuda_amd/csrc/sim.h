@@ -130,7 +130,9 @@ struct fdtd_sim {
     int adj_stiles = 1;             // the same for its x and y slabs (FDTD_ADJ_STILES: 1, 2)
     int adj_rows = 0;               // rows per thread of the transposed clean box (FDTD_ADJ_ROWS; 0: as the forward step)
     int adj_cchunk = 0;             // its x chunk (FDTD_ADJ_CCHUNK; 0: as the forward clean box)
-    int adj_ctj = 8, adj_cich = 2;  // tile rows and x chunk of the transposed corner columns (FDTD_ADJ_CTJ, FDTD_ADJ_CICH)
+    int adj_ctj = 8, adj_cich = 2;  // tile rows and x chunk of the transposed corner columns on k_adj_step (FDTD_ADJ_CTJ, FDTD_ADJ_CICH)
+    bool adj_corner_rows = true;    // transposed corner columns on the row-blocked kernel (FDTD_ADJ_CORNER_ROWS; 0: k_adj_step)
+    int adj_crchunk = 4;            // their x chunk (FDTD_ADJ_CRCHUNK)
     int adj_stj = 16;               // tile rows of the side slabs in the transposed step (FDTD_ADJ_STJ; 4, 8, 16 measured ratios 3.14, 3.16, 3.07 over 2000 steps)
     int xcd = 0;                    // XCD-contiguous block order in the row-blocked kernel (FDTD_XCD=1)
     int timing_what = 1;            // 1: all field-update kernels of a step, 2: the row-blocked kernel only

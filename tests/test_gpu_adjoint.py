@@ -300,3 +300,29 @@ def test_gpu_adjoint_taller_grids_row_kernel_matches_general_kernel(nz):
     assert Jr == Jg and np.abs(gg).max() > 1e-8
     assert err < 1e-5
 
+
+@pytest.mark.gpu
+def test_gpu_adjoint_corner_variant_matches_general_kernel_and_oracle():
+    """FDTD_ADJ_CORNER_ROWS=1, objective cell inside a corner column of the
+    CPML (x and y both in the layer), so the gradient on the block is carried
+    there by the transposed corner kernel: the row-blocked corner variant
+    against the general transposed kernel on the corners and on the whole
+    grid, and against the FP64 oracle (bars as for the slab variants)."""
+    r0 = (3, 3, (20 + 2 * 5 + 1) // 2)
+    Jr, gr = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_CORNER_ROWS": "1"})
+    Jc, gc = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_CORNER_ROWS": "0"})
+    Jg, gg = _adjoint_with_path("general", r0=r0)
+    po, rhoo, loo, _ = problem(O)
+    o = O.Oracle(po)
+    o.set_design(loo, rhoo.astype(np.float32).astype(np.float64))
+    o.set_objective(r0, C0 / LAMS)
+    Jo, go = o.adjoint(STEPS)
+    scale = np.abs(gg).max()
+    ec, eg, eo = (np.abs(gr - x).max() / scale for x in (gc, gg, go))
+    print("corner: rows vs general corners %.2e, vs general %.2e, vs oracle %.2e (max|g| %.3e)" % (ec, eg, eo, scale), flush=True)
+    assert scale > 0
+    assert Jr == Jc == Jg
+    assert ec < 1e-5 and eg < 1e-5
+    assert Jr == pytest.approx(Jo, rel=2e-2)
+    assert eo < 2e-2
+
