@@ -491,11 +491,13 @@ int plan_rows(fdtd_sim *s) {
         if (const char *v = std::getenv("FDTD_SIDE_STREAMS")) s->side_streams = std::atoi(v) == 1 ? 1 : 3;
         if (const char *v = std::getenv("FDTD_CHAIN_JOIN")) s->chain_join = std::atoi(v) != 0;
         if (const char *v = std::getenv("FDTD_POST_AFTER_JOIN")) s->post_after_join = std::atoi(v) != 0;
+        if (const char *v = std::getenv("FDTD_EARLY_SIDE")) s->early_side = std::atoi(v) != 0;
         if (s->side_streams == 3 &&
             (hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking) != hipSuccess ||
              hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking) != hipSuccess ||
              hipEventCreateWithFlags(&s->ev_join3, hipEventDisableTiming) != hipSuccess ||
-             hipEventCreateWithFlags(&s->ev_join4, hipEventDisableTiming) != hipSuccess))
+             hipEventCreateWithFlags(&s->ev_join4, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&s->ev_early, hipEventDisableTiming) != hipSuccess))
             return set_err(s, FDTD_ERR_DEVICE, "side streams or events could not be created");
     }
     s->use_rows = s->has_clean;
@@ -538,6 +540,16 @@ bool tfsf_in_clean(const fdtd_sim *s) {
     const Setup &S = s->S;
     for (int a = 0; a < 2; a++)
         if (S.tf0[a] - 1 < s->cbox[2 * a] || S.tf1[a] + 1 >= s->cbox[2 * a + 1]) return false;
+    return true;
+}
+
+// Whether the cells the TFSF fix-up writes ([t0-1, t1+1] on x and y) keep off
+// the clean box's outer planes and rows, which the side pieces of the next
+// step read.
+bool tfsf_off_clean_edges(const fdtd_sim *s) {
+    const Setup &S = s->S;
+    for (int a = 0; a < 2; a++)
+        if (S.tf0[a] - 1 <= s->cbox[2 * a] || S.tf1[a] + 1 >= s->cbox[2 * a + 1] - 1) return false;
     return true;
 }
 
@@ -633,10 +645,13 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
         if (three) {
-            HIPCHK(s, hipStreamWaitEvent(s->stream3, s->ev_fork, 0));
-            HIPCHK(s, hipStreamWaitEvent(s->stream4, s->ev_fork, 0));
+            // the y slabs and corner columns may start at the last step's join (ev_early)
+            hipEvent_t e = s->early_ready ? s->ev_early : s->ev_fork;
+            HIPCHK(s, hipStreamWaitEvent(s->stream3, e, 0));
+            HIPCHK(s, hipStreamWaitEvent(s->stream4, e, 0));
         }
     }
+    s->early_ready = false;
     HIPCHK(s, launch_line(l, side));
     if (tf_early) HIPCHK(s, hipEventRecord(s->ev_line, side));
     if (s->nxbox) {
@@ -678,6 +693,19 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     const bool after_join = three && s->chain_join && s->post_after_join;
     if (after_join) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     else if (tf_early) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
+    // The next step's y slabs and corner columns (stream3, stream4) may start
+    // here, beside this step's post-update work, when that work writes no cell
+    // they read: they read their own cells and the clean box's outer planes and
+    // rows (one cell deep) of this step's outputs, and write the other
+    // ping-pong buffers; the TFSF fix-up writes [t0-1, t1+1] on x and y (kept
+    // off those planes and rows here), the dispersive box and the monitor
+    // sample stay inside the clean box with a 2-cell margin.  The next step's
+    // line and x slabs keep the fork after the post-update launch (the line
+    // would overwrite the incident field the fix-up reads).
+    if (after_join && s->early_side && tf_early && (s->nbox <= 0 || disp_early) && tfsf_off_clean_edges(s)) {
+        HIPCHK(s, hipEventRecord(s->ev_early, s->stream));
+        s->early_ready = true;
+    }
     const bool merge_disp = disp_early && tf_early && !s->S.hydro;
     if (disp_early && !merge_disp) {
         HIPCHK(s, s->S.hydro ? launch_hydro(*dd, s->stream) : launch_dispersive(*dd, s->stream));
@@ -838,6 +866,7 @@ int fdtd_destroy(fdtd_sim *s) {
         if (q) (void)hipStreamSynchronize(q);
     if (s->ev_join3) (void)hipEventDestroy(s->ev_join3);
     if (s->ev_join4) (void)hipEventDestroy(s->ev_join4);
+    if (s->ev_early) (void)hipEventDestroy(s->ev_early);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->status_h) (void)hipHostFree(s->status_h);
     if (s->stream2) (void)hipStreamDestroy(s->stream2);
@@ -874,6 +903,9 @@ int fdtd_get_info(const fdtd_sim *s, fdtd_grid_info *g) {
     const bool two = s->use_rows && s->stream2 && s->early;
     const bool tf = two && tfsf_in_clean(s), dp = two && disp_in_clean(s);
     g->schedule = (tf ? 1 : 0) | (dp ? 2 : 0) | (tf && (s->nbox <= 0 || dp) && sample_in_clean(s) ? 4 : 0);
+    // bit 3: the next step's y slabs and corner columns start at the join (launch_rows_update)
+    const bool three = two && s->stream3 && s->stream4 && s->chain_join && s->post_after_join;
+    if (three && s->early_side && tf && (s->nbox <= 0 || dp) && tfsf_off_clean_edges(s)) g->schedule |= 8;
     if (s->use_rows) {
         const int W = rows_waves(s->g.NZ);
         g->tile[0] = s->rows;
@@ -1547,6 +1579,9 @@ static int periodic_step(fdtd_sim *s) {
 int fdtd_step(fdtd_sim *s, int nsteps) {
     if (!s || nsteps < 0) return FDTD_ERR_INVALID;
     if (int rc = check_finite(s)) return rc;
+    // an early start reaches only the steps of this call (anything the caller
+    // did on the main stream since the last one is ordered by the plain fork)
+    s->early_ready = false;
     if (s->S.P.periodic_xy) {
         for (int n = 0; n < nsteps; n++)
             if (int rc = periodic_step(s)) return rc;

@@ -109,6 +109,9 @@ struct fdtd_sim {
     int side_streams = 3;
     bool chain_join = true;         // side streams join through stream2 (FDTD_CHAIN_JOIN=0: each joins the main stream)
     bool post_after_join = true;    // with chained joins, the post-update launch after the join (FDTD_POST_AFTER_JOIN)
+    bool early_side = true;         // the next step's y slabs and corners start at this step's join (FDTD_EARLY_SIDE)
+    bool early_ready = false;       // ev_early holds that point for the next step of this fdtd_step call
+    hipEvent_t ev_early = nullptr;
     hipStream_t stream3 = nullptr, stream4 = nullptr;
     hipEvent_t ev_join3 = nullptr, ev_join4 = nullptr;
     bool early = true;              // TFSF fix and dispersive box ahead of the join when their cells allow (FDTD_EARLY=0 to keep them after)
@@ -204,6 +207,7 @@ RowsArgs make_rows(fdtd_sim *s);
 int rows_tiles(int NZ);
 bool tfsf_in_clean(const fdtd_sim *s);
 bool disp_in_clean(const fdtd_sim *s);
+bool tfsf_off_clean_edges(const fdtd_sim *s);
 // FDTD_ERR_NONFINITE (with a message) when a kernel has flagged a NaN or Inf.
 int check_finite(fdtd_sim *s);
 // DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.

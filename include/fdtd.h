@@ -121,7 +121,9 @@ typedef struct fdtd_grid_info {
                                   CPML-free variant of the row-blocked kernel; -1 when none       */
     int    schedule;           /* fdtd_step on two streams: bit 0 the TFSF fix-up, bit 1 the
                                   dispersive box, bit 2 the DFT sample follow the clean box without
-                                  waiting for the side pieces (no reference counterpart)          */
+                                  waiting for the side pieces; bit 3 the next step's y slabs and
+                                  corner columns start at the join, beside the post-update launch
+                                  (no reference counterpart)                                      */
     int    tile[7];            /* tiles fdtd_step uses (no reference counterpart): row-blocked kernel
                                   rows per thread, clean-box x chunk, side-slab x chunk, row tiles
                                   per block on the clean box and on the side slabs, tile rows of the
