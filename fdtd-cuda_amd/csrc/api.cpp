@@ -491,7 +491,7 @@ int plan_rows(fdtd_sim *s) {
         if (const char *v = std::getenv("FDTD_SIDE_STREAMS")) s->side_streams = std::atoi(v) == 1 ? 1 : 3;
         if (const char *v = std::getenv("FDTD_CHAIN_JOIN")) s->chain_join = std::atoi(v) != 0;
         if (const char *v = std::getenv("FDTD_POST_AFTER_JOIN")) s->post_after_join = std::atoi(v) != 0;
-        if (const char *v = std::getenv("FDTD_EARLY_SIDE")) s->early_side = std::atoi(v) != 0;
+        if (const char *v = std::getenv("FDTD_EARLY_SIDE")) s->early_side = std::min(2, std::max(0, std::atoi(v)));
         if (s->side_streams == 3 &&
             (hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking) != hipSuccess ||
              hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking) != hipSuccess ||
@@ -641,19 +641,23 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     // three side streams: the y slabs and the corner columns beside the line and x slabs
     const bool three = fork && s->stream3 && s->stream4;
     hipStream_t side_y = three ? s->stream3 : side, side_c = three ? s->stream4 : side;
+    // early_side 2 (three side streams): the line goes first on the corner
+    // stream, which keeps the plain fork, and the x slabs may start early too
+    const bool line_c = three && s->early_side == 2;
+    hipStream_t side_l = line_c ? s->stream4 : side;
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
-        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+        // the side pieces may start at the last step's join (ev_early); the stream with the line may not
+        hipEvent_t e = s->early_ready ? s->ev_early : s->ev_fork;
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, line_c ? e : s->ev_fork, 0));
         if (three) {
-            // the y slabs and corner columns may start at the last step's join (ev_early)
-            hipEvent_t e = s->early_ready ? s->ev_early : s->ev_fork;
             HIPCHK(s, hipStreamWaitEvent(s->stream3, e, 0));
-            HIPCHK(s, hipStreamWaitEvent(s->stream4, e, 0));
+            HIPCHK(s, hipStreamWaitEvent(s->stream4, line_c ? s->ev_fork : e, 0));
         }
     }
     s->early_ready = false;
-    HIPCHK(s, launch_line(l, side));
-    if (tf_early) HIPCHK(s, hipEventRecord(s->ev_line, side));
+    HIPCHK(s, launch_line(l, side_l));
+    if (tf_early) HIPCHK(s, hipEventRecord(s->ev_line, side_l));
     if (s->nxbox) {
         RowsArgs rx = r;
         rx.ichunk = s->schunk;

@@ -109,7 +109,8 @@ struct fdtd_sim {
     int side_streams = 3;
     bool chain_join = true;         // side streams join through stream2 (FDTD_CHAIN_JOIN=0: each joins the main stream)
     bool post_after_join = true;    // with chained joins, the post-update launch after the join (FDTD_POST_AFTER_JOIN)
-    bool early_side = true;         // the next step's y slabs and corners start at this step's join (FDTD_EARLY_SIDE)
+    int early_side = 2;             // the next step's y slabs and corners (1), and x slabs (2: the line moves to the
+                                    // corner stream), start at this step's join (FDTD_EARLY_SIDE)
     bool early_ready = false;       // ev_early holds that point for the next step of this fdtd_step call
     hipEvent_t ev_early = nullptr;
     hipStream_t stream3 = nullptr, stream4 = nullptr;

@@ -248,9 +248,10 @@ def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypa
     ahead of the side pieces, in one launch with the Drude metal, against
     them after the join (FDTD_EARLY=0) and on one stream (FDTD_CONCURRENT=0):
     the cells each kernel touches are disjoint, so every schedule gives the
-    same bits, DFT spectra included; the default starts the next step's y
-    slabs and corners at the join (schedule bit 3 where the cells allow it),
-    FDTD_EARLY_SIDE=0 at the fork.  The sample joins only when its faces
+    same bits, DFT spectra included; the default starts the next step's x
+    and y slabs and corners at the join (schedule bit 3 where the cells allow
+    it; the line then runs first on the corner stream), FDTD_EARLY_SIDE=1
+    only the y slabs and corners, 0 none.  The sample joins only when its faces
     clear the TFSF planes (scattering offset 2 puts the far face on plane
     t1+1; 3 clears it) and the dispersive box with its margin (the 4 nm
     sphere's box clears absorption faces at offset 1)."""
@@ -261,7 +262,8 @@ def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypa
         p.hydro_beta = 1.27e6
     runs = {}
     for name, env in (("early", {}), ("late", {"FDTD_EARLY": "0"}), ("one side stream", {"FDTD_SIDE_STREAMS": "1"}),
-                      ("no early side", {"FDTD_EARLY_SIDE": "0"}), ("serial", {"FDTD_CONCURRENT": "0"})):
+                      ("no early side", {"FDTD_EARLY_SIDE": "0"}), ("early y slabs only", {"FDTD_EARLY_SIDE": "1"}),
+                      ("serial", {"FDTD_CONCURRENT": "0"})):
         for k in ("FDTD_EARLY", "FDTD_CONCURRENT", "FDTD_SIDE_STREAMS", "FDTD_EARLY_SIDE"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
@@ -276,7 +278,8 @@ def test_step_schedules_bit_identical(hydro, radius, offsets, schedule, monkeypa
     assert runs["early"][0] & 8, runs["early"][0]
     assert runs["no early side"][0] == schedule, runs["no early side"][0]
     _, ref, ref_dft = runs["serial"]
-    for name in ("early", "late", "one side stream", "no early side"):
+    assert runs["early y slabs only"][0] & 8
+    for name in ("early", "late", "one side stream", "no early side", "early y slabs only"):
         _, fields, dft = runs[name]
         for c in range(6):
             assert np.array_equal(fields[c], ref[c]), (name, c)
