@@ -1089,6 +1089,7 @@ int adj_alloc(fdtd_sim *s) {
     if (const char *v = std::getenv("FDTD_ADJ_CCHUNK")) s->adj_cchunk = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("FDTD_ADJ_STILES")) s->adj_stiles = std::atoi(v) == 2 ? 2 : 1;
     if (const char *v = std::getenv("FDTD_ADJ_FUSE_INJECT")) s->adj_fuse_inject = std::atoi(v) != 0;
+    if (const char *v = std::getenv("FDTD_ADJ_EARLY")) s->adj_early = std::atoi(v) != 0;
     if (const char *v = std::getenv("FDTD_ADJ_STJ")) {
         const int tj = std::atoi(v);
         if (tj == 4 || tj == 8 || tj == 16) s->adj_stj = tj;
@@ -1240,11 +1241,17 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
     // added to the outputs on the main stream after the clean box (3)
     const int sched = three && s->nbox > 0 && (s->adj_sched != 3 || s->adj.corr) ? s->adj_sched : 0;
     if (sched == 0) HIPCHK(s, launch_adj_disp(ad, s->stream));
+    // early start (schedule 3): the x and y slabs of this step may start at the
+    // last step's join (ev_early, recorded before its correction's addition);
+    // the corner stream, whose dispersive update and correction amounts read
+    // what the addition writes, keeps the fork
+    const bool early_in = sched == 3 && s->adj_early_ready;
+    s->adj_early_ready = false;
     if (fork) {
         HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
-        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, early_in ? s->ev_early : s->ev_fork, 0));
         if (three) {
-            HIPCHK(s, hipStreamWaitEvent(s->stream3, s->ev_fork, 0));
+            HIPCHK(s, hipStreamWaitEvent(s->stream3, early_in ? s->ev_early : s->ev_fork, 0));
             HIPCHK(s, hipStreamWaitEvent(s->stream4, s->ev_fork, 0));
         }
     }
@@ -1311,6 +1318,24 @@ int adjoint_step(fdtd_sim *s, int pa, int pe, int ph, const AdjDispDev &ad, cons
     HIPCHK(s, launch_step_rows(r, arows, 3, s->stream));
     AdjInject none{};
     const AdjInject &inj = s->nbox > 0 && next.on ? next : none;
+    // early start of the next step's slabs: the join first, then the addition
+    // (its cells, the box grown by one and the injection cell, lie off the
+    // clean box's outer planes and rows, which the slabs read; the caller
+    // allows it only then and only when the next injection rides on it)
+    const bool early_out = sched == 3 && s->adj_early_allowed && s->nbox > 0;
+    if (early_out) {
+        HIPCHK(s, hipEventRecord(s->ev_join3, s->stream3));
+        HIPCHK(s, hipEventRecord(s->ev_join4, s->stream4));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_join3, 0));
+        HIPCHK(s, hipStreamWaitEvent(s->stream2, s->ev_join4, 0));
+        HIPCHK(s, hipEventRecord(s->ev_join, s->stream2));
+        HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+        HIPCHK(s, hipEventRecord(s->ev_early, s->stream));
+        s->adj_early_ready = true;
+        HIPCHK(s, launch_adj_box_add(av, inj, s->adj.corr, s->stream));
+        *injected = inj.on != 0;
+        return FDTD_OK;
+    }
     if (s->nbox) {
         if (sched == 1) {
             HIPCHK(s, hipEventRecord(s->ev_line, s->stream));
@@ -1371,10 +1396,13 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
     s->probe = probe;
     s->probe_p = gidx(s->g, r0[0], r0[1], r0[2]);
     s->probe_t0 = t0;
+    // the recorded loops step one at a time (record() only moves host pointers),
+    // keeping the early start of the side pieces across the steps of a loop
     if (store) {
+        s->early_ready = false;
         for (int st = 0; st < nsteps; st++) {
             record(st);
-            if ((rc = fdtd_step(s, 1))) return rc;
+            if ((rc = step_steps(s, 1))) return rc;
         }
         unrecord();
     } else {
@@ -1386,9 +1414,10 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
             } else {
                 // the last segment records its box trajectory on the way, so the
                 // reverse sweep starts on it without a checkpoint or a recompute
+                s->early_ready = false;
                 for (int st = 0; st < len; st++) {
                     record(st);
-                    if ((rc = fdtd_step(s, 1))) return rc;
+                    if ((rc = step_steps(s, 1))) return rc;
                 }
                 unrecord();
             }
@@ -1435,6 +1464,12 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
     // where the objective cell lies in the clean box (written there by the clean box alone)
     const bool fuse = s->adj.rows && s->nbox > 0 && s->adj_fuse_inject && r0[0] >= s->cbox[0] &&
                       r0[0] < s->cbox[1] && r0[1] >= s->cbox[2] && r0[1] < s->cbox[3];
+    // the next step's slabs may start before this step's correction is added
+    // when every injection rides on it and its cell is off the clean box's
+    // outer planes and rows (which the slabs read)
+    s->adj_early_allowed = fuse && s->adj_early && r0[0] > s->cbox[0] && r0[0] < s->cbox[1] - 1 &&
+                           r0[1] > s->cbox[2] && r0[1] < s->cbox[3] - 1;
+    s->adj_early_ready = false;
     bool injected = false;
     for (int seg = nseg - 1; seg >= 0; seg--) {
         int seg_t0 = t0, len = nsteps;
@@ -1446,11 +1481,14 @@ int adjoint_run(fdtd_sim *s, AdjScratch &W, int nsteps, int ckpt, const int r0[3
                 Checkpoint &ck = cks[seg];
                 s->t = ck.t; s->p = ck.p; s->pp = ck.pp; s->le = ck.le;
                 if ((rc = ckpt_copy(s, ck, false))) return rc;
+                s->early_ready = false;
                 for (int st = 0; st < len; st++) {
                     record(st);
-                    if ((rc = fdtd_step(s, 1))) return rc;
+                    if ((rc = step_steps(s, 1))) return rc;
                 }
                 unrecord();
+                s->early_ready = false;
+                s->adj_early_ready = false;
             }
         }
         for (int st = len - 1; st >= 0; st--) {
@@ -1582,10 +1620,16 @@ static int periodic_step(fdtd_sim *s) {
 
 int fdtd_step(fdtd_sim *s, int nsteps) {
     if (!s || nsteps < 0) return FDTD_ERR_INVALID;
-    if (int rc = check_finite(s)) return rc;
     // an early start reaches only the steps of this call (anything the caller
     // did on the main stream since the last one is ordered by the plain fork)
     s->early_ready = false;
+    return fdtd_host::step_steps(s, nsteps);
+}
+
+// fdtd_step without clearing the early start: for loops of single steps that
+// launch nothing else between them (the adjoint's recorded steps).
+int fdtd_host::step_steps(fdtd_sim *s, int nsteps) {
+    if (int rc = check_finite(s)) return rc;
     if (s->S.P.periodic_xy) {
         for (int n = 0; n < nsteps; n++)
             if (int rc = periodic_step(s)) return rc;

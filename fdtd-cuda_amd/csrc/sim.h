@@ -130,6 +130,9 @@ struct fdtd_sim {
     int adj_sched = 3;              // transposed dispersive update and correction: 0 main stream, 1 corner stream, 2 update there, correction on main,
                                     // 3 update and correction amounts there, their addition on main (FDTD_ADJ_SCHED)
     bool adj_fuse_inject = true;    // the next step's injection in the correction launch (FDTD_ADJ_FUSE_INJECT)
+    bool adj_early = false;         // the next transposed step's slabs start at the join (FDTD_ADJ_EARLY; measured slower)
+    bool adj_early_allowed = false; // ... for this sweep's objective cell
+    bool adj_early_ready = false;   // ev_early holds that point for the next transposed step
     int adj_tiles = 1;              // row tiles per block of the transposed clean box, two rows (FDTD_ADJ_TILES: 1, 2)
     int adj_stiles = 1;             // the same for its x and y slabs (FDTD_ADJ_STILES: 1, 2)
     int adj_rows = 0;               // rows per thread of the transposed clean box (FDTD_ADJ_ROWS; 0: as the forward step)
@@ -209,6 +212,8 @@ int rows_tiles(int NZ);
 bool tfsf_in_clean(const fdtd_sim *s);
 bool disp_in_clean(const fdtd_sim *s);
 bool tfsf_off_clean_edges(const fdtd_sim *s);
+// nsteps steps keeping an early start from the step before (fdtd_step clears it first)
+int step_steps(fdtd_sim *s, int nsteps);
 // FDTD_ERR_NONFINITE (with a message) when a kernel has flagged a NaN or Inf.
 int check_finite(fdtd_sim *s);
 // DFT bookkeeping: a sampling step, the ring slot just written, the K-sample contraction.

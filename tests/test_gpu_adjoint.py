@@ -183,7 +183,9 @@ def test_gpu_adjoint_slab_variants_match_general_kernel_and_oracle(where):
 def test_gpu_adjoint_schedules_give_the_same_bits(where):
     """The transposed dispersive update beside the clean box (schedules 1-3,
     3 with the correction amounts computed there and added after the clean
-    box) and the next step's injection folded into the dispersive-cell correction
+    box), the next step's slabs starting at the join (FDTD_ADJ_EARLY; the
+    forward recompute's side pieces likewise, FDTD_EARLY_SIDE) and the next
+    step's injection folded into the dispersive-cell correction
     (for an objective cell outside and inside the corrected cells) change no
     bit against the plain order: update, injection and correction on the
     main stream."""
@@ -191,10 +193,14 @@ def test_gpu_adjoint_schedules_give_the_same_bits(where):
     _, _, lo, r0 = problem(G, shape)
     if where == "in the box margin":
         r0 = (r0[0], r0[1], lo[2] + shape[2])
-    J0, g0 = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_SCHED": "0", "FDTD_ADJ_FUSE_INJECT": "0"})
-    for sched, fuse in (("3", "1"), ("3", "0"), ("2", "1"), ("1", "1"), ("1", "0"), ("0", "1")):
-        J, g = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_SCHED": sched, "FDTD_ADJ_FUSE_INJECT": fuse})
-        print(where, "schedule", sched, "fused injection", fuse, "max |dg|", np.abs(g - g0).max(), flush=True)
+    J0, g0 = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_SCHED": "0", "FDTD_ADJ_FUSE_INJECT": "0",
+                                                      "FDTD_ADJ_EARLY": "0", "FDTD_EARLY_SIDE": "0"})
+    for sched, fuse, early in (("3", "1", "1"), ("3", "1", "0"), ("3", "0", "1"), ("2", "1", "1"), ("1", "1", "1"),
+                               ("1", "0", "1"), ("0", "1", "1")):
+        J, g = _adjoint_with_path("rows", r0=r0, extra={"FDTD_ADJ_SCHED": sched, "FDTD_ADJ_FUSE_INJECT": fuse,
+                                                        "FDTD_ADJ_EARLY": early, "FDTD_EARLY_SIDE": "2" if early == "1" else "0"})
+        print(where, "schedule", sched, "fused injection", fuse, "early slabs", early, "max |dg|", np.abs(g - g0).max(),
+              flush=True)
         assert J == J0
         assert np.array_equal(g, g0)
     assert np.abs(g0).max() > 0
