@@ -638,7 +638,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     const bool disp_early = fork && s->early && dd && disp_in_clean(s);
     *disp_done = false;
     *samp_done = false;
-    // three side streams: the y slabs and the corner columns beside the line and x slabs
+    // three side streams: x slabs, y slabs, and the corner columns (with the line first under early_side 2)
     const bool three = fork && s->stream3 && s->stream4;
     hipStream_t side_y = three ? s->stream3 : side, side_c = three ? s->stream4 : side;
     // early_side 2 (three side streams): the line goes first on the corner
@@ -697,15 +697,16 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     const bool after_join = three && s->chain_join && s->post_after_join;
     if (after_join) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     else if (tf_early) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_line, 0));
-    // The next step's y slabs and corner columns (stream3, stream4) may start
-    // here, beside this step's post-update work, when that work writes no cell
-    // they read: they read their own cells and the clean box's outer planes and
-    // rows (one cell deep) of this step's outputs, and write the other
-    // ping-pong buffers; the TFSF fix-up writes [t0-1, t1+1] on x and y (kept
-    // off those planes and rows here), the dispersive box and the monitor
-    // sample stay inside the clean box with a 2-cell margin.  The next step's
-    // line and x slabs keep the fork after the post-update launch (the line
-    // would overwrite the incident field the fix-up reads).
+    // The next step's side pieces may start here, beside this step's
+    // post-update work, when that work writes no cell they read: they read
+    // their own cells and the clean box's outer planes and rows (one cell deep)
+    // of this step's outputs, and write the other ping-pong buffers; the TFSF
+    // fix-up writes [t0-1, t1+1] on x and y (kept off those planes and rows
+    // here), the dispersive box and the monitor sample stay inside the clean
+    // box with a 2-cell margin.  The stream that runs the next step's line
+    // keeps the fork after the post-update launch (the line would overwrite
+    // the incident field the fix-up reads): the x-slab stream with
+    // early_side 1, the corner stream with 2.
     if (after_join && s->early_side && tf_early && (s->nbox <= 0 || disp_early) && tfsf_off_clean_edges(s)) {
         HIPCHK(s, hipEventRecord(s->ev_early, s->stream));
         s->early_ready = true;
