@@ -75,6 +75,13 @@ struct fdtd_batch {
     // dispersive boxes (every simulation on the row path, every box in its
     // clean box) follow the clean boxes without waiting for the side pieces
     bool tf_early = false, dp_early = false;
+    // early side start (FDTD_BATCH_EARLY=1; off by default, the 32-rod sweep
+    // ran 1-9 % slower with it): the next step's x and y slabs wait for this
+    // step's join (ev_early) instead of its post-update launches; early_ok
+    // when every simulation allows it, early_ready when the step enqueued
+    // just before, in the same run or capture, recorded ev_early
+    hipEvent_t ev_early = nullptr;
+    bool early = false, early_ok = false, early_ready = false;
 };
 
 namespace {
@@ -376,6 +383,9 @@ int upload_items(fdtd_batch *b) {
         if (b->rowpath[i] && !tfsf_in_clean(s)) b->tf_early = false;
         if (!b->rowpath[i] || (s->nbox > 0 && !disp_in_clean(s))) b->dp_early = false;
     }
+    b->early_ok = b->tf_early && b->dp_early;
+    for (int i = 0; i < n; i++)
+        if (!tfsf_off_clean_edges(b->sims[i])) b->early_ok = false;
     bool same = ls.size() == b->launches.size();
     for (size_t q = 0; same && q < ls.size(); q++) same = ls[q].same(b->launches[q]);
     if (same) return FDTD_OK;
@@ -386,6 +396,64 @@ int upload_items(fdtd_batch *b) {
         BCHK(b, hipMemcpy(l.d, l.map.data(), l.map.size() * sizeof(unsigned), hipMemcpyHostToDevice));
     }
     b->launches = std::move(ls);
+    return FDTD_OK;
+}
+
+// One step for every simulation under the early side start: the x and y
+// slabs (and on one side stream the corner columns) wait for the last step's
+// join only, when that step was enqueued just before (early_ready), and so
+// run beside its post-update launches, which write no cell they read: every
+// simulation is on the row path, its TFSF fix-up cells keep off the clean
+// box's outer planes and rows, its dispersive box with a 2-cell margin lies
+// in the clean box, the samples read that step's outputs, and the side
+// pieces write the other ping-pong buffers.  The lines keep the fork (a line
+// overwrites the incident field the last step's TFSF fix-ups read): first on
+// the corner stream, ahead of the corner columns, with three side streams,
+// after the side pieces with one.  Every side stream joins the x-slab stream,
+// the main stream waits once, records ev_early and runs the TFSF fix-ups, the
+// dispersive boxes and the samples.
+int enqueue_step_early(fdtd_batch *b, int par, int off, int sidx) {
+    const int n = (int)b->sims.size();
+    const bool three = b->stream3 && b->stream4;
+    hipStream_t side_y = three ? b->stream3 : b->stream2, side_c = three ? b->stream4 : b->stream2;
+    BCHK(b, hipEventRecord(b->ev_fork, b->stream));
+    hipEvent_t e = b->early_ready ? b->ev_early : b->ev_fork;
+    b->early_ready = false;
+    BCHK(b, hipStreamWaitEvent(b->stream2, e, 0));
+    if (three) {
+        BCHK(b, hipStreamWaitEvent(b->stream3, e, 0));
+        BCHK(b, hipStreamWaitEvent(b->stream4, b->ev_fork, 0));
+        BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, b->stream4));
+    }
+    for (const Launch &l : b->launches) {
+        const int nb = (int)l.map.size();
+        if (l.type == Launch::Rows && l.c != 0)
+            BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, nb, l.a, l.b, l.c, l.c == 2 ? side_y : b->stream2));
+        if (l.type == Launch::Fused) BCHK(b, launch_fused_batch(b->fused[par], l.d, nb, l.a, side_c));
+    }
+    if (three) {
+        BCHK(b, hipEventRecord(b->ev_join3, b->stream3));
+        BCHK(b, hipEventRecord(b->ev_join4, b->stream4));
+        BCHK(b, hipStreamWaitEvent(b->stream2, b->ev_join3, 0));
+        BCHK(b, hipStreamWaitEvent(b->stream2, b->ev_join4, 0));
+    } else {
+        // one side stream: the lines follow the side pieces there, behind the fork
+        if (e != b->ev_fork) BCHK(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+        BCHK(b, launch_line_batch(b->line[par], n, b->dyn, off, b->stream2));
+    }
+    BCHK(b, hipEventRecord(b->ev_join, b->stream2));
+    for (const Launch &l : b->launches)
+        if (l.type == Launch::Rows && l.c == 0)
+            BCHK(b, launch_rows_batch(b->rows[par][l.c], l.d, (int)l.map.size(), l.a, l.b, l.c, b->stream));
+    BCHK(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    BCHK(b, hipEventRecord(b->ev_early, b->stream));
+    for (const Launch &l : b->launches) {
+        const int nb = (int)l.map.size();
+        if (l.type == Launch::Tfsf) BCHK(b, launch_tfsf_fix_batch(b->fused[par], l.d, nb, b->stream));
+        if (l.type == Launch::Disp) BCHK(b, launch_disp_batch(b->disp[par], l.d, nb, b->stream));
+        if (l.type == Launch::Sample && sidx >= 0) BCHK(b, launch_sample_batch(b->samp[par], l.d, nb, b->dyn, sidx, b->stream));
+    }
+    b->early_ready = true;
     return FDTD_OK;
 }
 
@@ -402,6 +470,7 @@ int enqueue_step(fdtd_batch *b, int par, int off, int sidx) {
     const int n = (int)b->sims.size();
     const bool fork = b->stream2 != nullptr;
     const bool three = fork && b->stream3 && b->stream4;
+    if (fork && b->early && b->early_ok && b->ev_early) return enqueue_step_early(b, par, off, sidx);
     hipStream_t side = fork ? b->stream2 : b->stream;
     hipStream_t side_y = three ? b->stream3 : side, side_g = three ? b->stream4 : side;
     const bool tf_early = fork && b->tf_early, dp_early = tf_early && b->dp_early;
@@ -497,6 +566,9 @@ int capture(fdtd_batch *b) {
     const fdtd_sim *s0 = b->sims[0];
     const int P = period_of(s0), stride = s0->S.P.dft_stride;
     const bool samples = s0->S.P.StaticField == 0;
+    // a captured period starts from the plain fork, and no later step may
+    // wait on an event recorded only inside the capture
+    b->early_ready = false;
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int sidx = 0, rc = FDTD_OK;
     for (int o = 0; o < P && !rc; o++) {
@@ -505,6 +577,7 @@ int capture(fdtd_batch *b) {
         if (samp) sidx++;
     }
     if (!rc && launch_batch_advance(b->dyn, P, sidx, b->stream) != hipSuccess) rc = FDTD_ERR_DEVICE;
+    b->early_ready = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess) {
@@ -557,7 +630,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
         if (hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b->ev_line, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&b->ev_line, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_early, hipEventDisableTiming) != hipSuccess)
             return fail(berr(b, FDTD_ERR_DEVICE, "second stream or events could not be created"));
         // one side stream by default: the batch's launches already fill the device
         // (32-rod sweep: 51.6-52.0 k Mcell-updates/s with one, 51.0-51.2 k with three)
@@ -568,6 +642,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int n, int flags, fdtd_batch **out)
              hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming) != hipSuccess ||
              hipEventCreateWithFlags(&b->ev_join4, hipEventDisableTiming) != hipSuccess))
             return fail(berr(b, FDTD_ERR_DEVICE, "side streams or events could not be created"));
+        const char *ee = std::getenv("FDTD_BATCH_EARLY");
+        b->early = ee && std::atoi(ee) != 0;
     }
     int rc;
     for (int par = 0; par < 2; par++) {
@@ -623,6 +699,7 @@ int fdtd_batch_run(fdtd_batch *b, int nsteps) {
     const bool samples = s0->S.P.StaticField == 0;
     const int per_period = samples ? P / stride : 0;
     const bool use_graph = !(b->flags & FDTD_BATCH_NO_GRAPH);
+    b->early_ready = false;
     int done = 0;
     while (done < nsteps) {
         if (use_graph && s0->t % P == 0 && nsteps - done >= P && per_period <= K) {
@@ -631,6 +708,7 @@ int fdtd_batch_run(fdtd_batch *b, int nsteps) {
             if (!b->graph_valid)
                 if (int rc = capture(b)) return rc;
             BCHK(b, hipGraphLaunch(b->exec, b->stream));
+            b->early_ready = false;
             for (int o = 0; o < P; o++) mirror_step(b, samples && o % stride == 0);
             done += P;
         } else {
@@ -673,6 +751,7 @@ int fdtd_batch_destroy(fdtd_batch *b) {
     if (b->ev_line) (void)hipEventDestroy(b->ev_line);
     if (b->ev_join3) (void)hipEventDestroy(b->ev_join3);
     if (b->ev_join4) (void)hipEventDestroy(b->ev_join4);
+    if (b->ev_early) (void)hipEventDestroy(b->ev_early);
     if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream3) (void)hipStreamDestroy(b->stream3);
     if (b->stream4) (void)hipStreamDestroy(b->stream4);

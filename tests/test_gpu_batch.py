@@ -150,6 +150,39 @@ def test_batch_graph_direct_and_alone_agree_bitwise(general):
         assert a[i].get_info().sample_count == c[i].get_info().sample_count
 
 
+@pytest.mark.parametrize("side", ["1", "3"])
+def test_batch_early_side_start_agrees_bitwise(side, monkeypatch):
+    """The early side start (FDTD_BATCH_EARLY=1: the next step's x and y
+    slabs wait for the join only, and run beside the TFSF fix-ups,
+    dispersive boxes and samples), on one side stream (the lines after the
+    side pieces) and on three (FDTD_SIDE_STREAMS=3, the lines and corners on
+    their own): graph replay, direct launches and each simulation alone agree
+    bit for bit in the fields, spectra to rounding of the contraction.  Every
+    simulation allows the start (its own step sets schedule bit 3 on the same
+    cell conditions), so the batch takes it."""
+    steps = 203
+    a, b, c = sims_for(CASES), sims_for(CASES), sims_for(CASES)
+    sched = [s.get_info().schedule for s in a]
+    print("schedules", sched, flush=True)
+    assert all(x & 8 for x in sched), sched
+    monkeypatch.setenv("FDTD_SIDE_STREAMS", side)
+    monkeypatch.setenv("FDTD_BATCH_EARLY", "1")
+    ba = G.Batch(a, flags=G.BATCH_NO_TUNE)
+    bb = G.Batch(b, flags=G.BATCH_NO_TUNE | G.BATCH_NO_GRAPH)
+    ba.run(steps)
+    bb.run(steps)
+    for s in c:
+        s.step(steps)
+    for i in range(len(CASES)):
+        fa, fb, fc = fields(a[i]), fields(b[i]), fields(c[i])
+        for k in range(6):
+            assert np.array_equal(fa[k], fb[k]), (i, k)
+            assert np.array_equal(fa[k], fc[k]), (i, k)
+        ra, rb, rc = a[i].flux()["raw"], b[i].flux()["raw"], c[i].flux()["raw"]
+        assert np.max(np.abs(ra - rc)) <= 1e-12 * np.max(np.abs(rc))
+        assert np.max(np.abs(rb - rc)) <= 1e-12 * np.max(np.abs(rc))
+
+
 @pytest.mark.parametrize("ctj", [8, 16])
 def test_batch_corner_tiles_agree_bitwise(ctj, monkeypatch):
     """Corner columns of 8 and 16 tile rows (FDTD_CTJ; the batch groups its
