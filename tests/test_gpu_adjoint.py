@@ -262,8 +262,8 @@ def test_gpu_adjoint_repeated_designs_keep_device_memory_flat():
             assert np.isfinite(J) and np.all(np.isfinite(g))
             used.append(in_use())
         growth = max(used[2:]) - used[1]
-        print("device memory in use per round (MB):", [round(u / 2**20, 1) for u in used],
-              "growth after round 2: %.1f MB" % (growth / 2**20), flush=True)
+        print("device memory in use after each design (MB):", [round(u / 2**20, 1) for u in used],
+              "growth after the second design: %.1f MB" % (growth / 2**20), flush=True)
         assert growth < 8 * 2**20
     finally:
         sim.close()
