@@ -7,7 +7,8 @@
   tiles, graph replay, direct launches and each simulation run alone agree
   bit for bit in the fields, on the row-blocked path and on the general
   kernel (FDTD_PATH=general), including runs split by individual steps
-  between batch runs.
+  between batch runs, and on every stream layout (one stream, one or three
+  side streams, the early side start).
 * The autotuned tiles change only rounding (group-normalised 1e-5).
 """
 import os
@@ -181,6 +182,31 @@ def test_batch_early_side_start_agrees_bitwise(side, monkeypatch):
         ra, rb, rc = a[i].flux()["raw"], b[i].flux()["raw"], c[i].flux()["raw"]
         assert np.max(np.abs(ra - rc)) <= 1e-12 * np.max(np.abs(rc))
         assert np.max(np.abs(rb - rc)) <= 1e-12 * np.max(np.abs(rc))
+
+
+@pytest.mark.parametrize("env", [{"FDTD_CONCURRENT": "0"}, {"FDTD_SIDE_STREAMS": "3"}])
+def test_batch_stream_layouts_agree_bitwise(env, monkeypatch):
+    """Every batched launch on one stream (FDTD_CONCURRENT=0), and the side
+    launches on three streams (FDTD_SIDE_STREAMS=3: x slabs, y slabs, lines
+    and corner columns): graph replay, direct launches and each simulation
+    alone agree bit for bit in the fields."""
+    steps = 203
+    a, b, c = sims_for(CASES), sims_for(CASES), sims_for(CASES)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    ba = G.Batch(a, flags=G.BATCH_NO_TUNE)
+    bb = G.Batch(b, flags=G.BATCH_NO_TUNE | G.BATCH_NO_GRAPH)
+    ba.run(steps)
+    bb.run(steps)
+    for s in c:
+        s.step(steps)
+    for i in range(len(CASES)):
+        fa, fb, fc = fields(a[i]), fields(b[i]), fields(c[i])
+        for k in range(6):
+            assert np.array_equal(fa[k], fb[k]), (env, i, k)
+            assert np.array_equal(fa[k], fc[k]), (env, i, k)
+        ra, rc = a[i].flux()["raw"], c[i].flux()["raw"]
+        assert np.max(np.abs(ra - rc)) <= 1e-12 * np.max(np.abs(rc))
 
 
 @pytest.mark.parametrize("ctj", [8, 16])
