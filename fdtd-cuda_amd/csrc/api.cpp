@@ -543,6 +543,22 @@ bool tfsf_in_clean(const fdtd_sim *s) {
     return true;
 }
 
+// The step's update of the dispersive box: the local model, or the
+// hydrodynamic launches and then their level shift, a swap of the roles of the
+// velocity-state and density buffers (k_hydro_ce leaves v^{n+1} in h_vnew and
+// n~^{n+1} in h_nd, and reads neither the old velocity state nor h_ndp of the
+// component it updates at a neighbour after writing it).
+hipError_t launch_box(fdtd_sim *s, const DispDev &dd) {
+    if (!s->S.hydro) return launch_dispersive(dd, s->stream);
+    const hipError_t e = launch_hydro(dd, s->stream);
+    if (e == hipSuccess)
+        for (int c = 0; c < 3; c++) {
+            std::swap(s->disp_st[c][2], s->h_vnew[c]);
+            std::swap(s->h_nd[c], s->h_ndp[c]);
+        }
+    return e;
+}
+
 // Whether the cells the TFSF fix-up writes ([t0-1, t1+1] on x and y) keep off
 // the clean box's outer planes and rows, which the side pieces of the next
 // step read.
@@ -713,7 +729,7 @@ int launch_rows_update(fdtd_sim *s, const StepDev &d, const LineDev &l, const Di
     }
     const bool merge_disp = disp_early && tf_early && !s->S.hydro;
     if (disp_early && !merge_disp) {
-        HIPCHK(s, s->S.hydro ? launch_hydro(*dd, s->stream) : launch_dispersive(*dd, s->stream));
+        HIPCHK(s, launch_box(s, *dd));
         *disp_done = true;
     }
     const bool merge_samp = tf_early && samp && d.sample_row && (s->nbox <= 0 || disp_early) && sample_in_clean(s);
@@ -1581,7 +1597,7 @@ int fdtd_update_e(fdtd_sim *s) {
     HIPCHK(s, s->S.P.periodic_xy ? launch_periodic_e(d, s->stream) : launch_split_e(d, s->stream));
     if (s->nbox) {
         DispDev dd = make_disp(s, s->p);
-        HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
+        HIPCHK(s, launch_box(s, dd));
     }
     return FDTD_OK;
 }
@@ -1676,7 +1692,7 @@ int fdtd_host::step_steps(fdtd_sim *s, int nsteps) {
             if (step_timing) HIPCHK(s, hipEventRecord(ev1, s->stream));
         }
         if (step_timing) s->ev_used++;
-        if (s->nbox && !disp_done) HIPCHK(s, s->S.hydro ? launch_hydro(dd, s->stream) : launch_dispersive(dd, s->stream));
+        if (s->nbox && !disp_done) HIPCHK(s, launch_box(s, dd));
         if (s->probe) {
             const float *E1[3] = {s->E[1 - q][0], s->E[1 - q][1], s->E[1 - q][2]};
             HIPCHK(s, launch_probe(E1, s->probe_p, s->probe + 3 * (size_t)(s->t - s->probe_t0), s->stream));

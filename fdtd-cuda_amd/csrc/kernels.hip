@@ -1812,21 +1812,17 @@ __global__ void k_hydro_v(DispDev d) {
 #undef VN
 }
 
-// Continuity for component c (main.cu:1224-1239, 1537-1554, 1818-1837):
-// n~^{n+1} = n~^{n-1} - 2 dt/dx * (div(n~ v) + div v), the normal-axis terms
-// halved; earlier components are already at level n+1 (x -> y -> z order).
-// The z update keeps the reference's 0.5 (Vz1 - Vz2) in place of the
-// density-weighted normal term (main.cu:1837); that term is in absolute
-// density units there, so here it is scaled by 1/N_EQ.
-__global__ void k_hydro_cont(DispDev d, int c) {
-    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nb) return;
-    int bi, bj, bk;
-    if (!box_interior(d, idx, bi, bj, bk) || !d.id[c][idx]) return;
+// Continuity sum of component c at box cell idx (main.cu:1224-1239,
+// 1537-1554, 1818-1837): n~^{n+1} = n~^{n-1} - 2 dt/dx * (div(n~ v) + div v),
+// the normal-axis terms halved.  P and N are the velocity and density of each
+// component at the level the reference's x -> y -> z order reads: the
+// components before c at n+1, c and the later ones at n.  The z update keeps
+// the reference's 0.5 (Vz1 - Vz2) in place of the density-weighted normal
+// term (main.cu:1837); that term is in absolute density units there, so here
+// it is scaled by 1/N_EQ.
+__device__ __forceinline__ double hydro_cont_sum(const DispDev &d, int c, long idx, const float *Px, const float *Py,
+                                                 const float *Pz, const float *Nx, const float *Ny, const float *Nz) {
     const long oi = (long)d.n[1] * d.n[2], oj = d.n[2], ok = 1;
-    const float *Px = d.st[0][2], *Py = d.st[1][2], *Pz = d.st[2][2];
-    const float *Nx = d.nd_prev[0], *Ny = d.nd_prev[1], *Nz = d.nd_prev[2];
     double r;
     if (c == 0) {
         const double Vx1 = Px[idx + oi], Vx2 = Px[idx - oi], Nx1 = Nx[idx + oi], Nx2 = Nx[idx - oi];
@@ -1853,18 +1849,34 @@ __global__ void k_hydro_cont(DispDev d, int c) {
         r = (Nx1 * Vx1 - Nx2 * Vx2) + (Ny1 * Vy1 - Ny2 * Vy2) + 0.5 * (Vz1 - Vz2) * d.h_inv_neq
           + ((Vx1 - Vx2) + (Vy1 - Vy2) + 0.5 * (Vz1 - Vz2));
     }
-    d.nd[c][idx] = (float)((double)d.nd[c][idx] - d.h_cont * r);
+    return r;
 }
 
-// E update of component c with the hydrodynamic current (main.cu:1247-1271):
-// E^{n+1} = a0 curl + h_cur v^{n+1} (1 + n~^{n+1}) + a1 E^n - a2 E^{n-1} - CP terms;
-// then the CP poles advance, v and n~ shift one level.
-__global__ void k_hydro_e(DispDev d, int c) {
+// Continuity and E update of component c in one launch (main.cu:1224-1271
+// and the y, z equivalents): n~^{n+1} of the cell, then
+// E^{n+1} = a0 curl + h_cur v^{n+1} (1 + n~^{n+1}) + a1 E^n - a2 E^{n-1} - CP terms,
+// then the CP poles advance.  Nothing this launch reads at a neighbour is
+// written by it: the components before c are read at n+1 from vnew and nd
+// (written by k_hydro_v and the earlier launches), c and the later ones at n
+// from the velocity state and nd_prev, which no launch of the step writes;
+// the cell's own n~ goes from n-1 to n+1 in nd.  The level shift (velocity
+// state <- vnew, nd_prev <-> nd) is a swap of the buffers' roles on the host
+// after the step's launches.
+__global__ void k_hydro_ce(DispDev d, int c) {
     const long nb = (long)d.n[0] * d.n[1] * d.n[2];
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nb) return;
     int bi, bj, bk;
     if (!box_interior(d, idx, bi, bj, bk) || !d.id[c][idx]) return;
+    const float *V[3], *N[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        V[q] = q < c ? d.vnew[q] : d.st[q][2];
+        N[q] = q < c ? d.nd[q] : d.nd_prev[q];
+    }
+    const double r = hydro_cont_sum(d, c, idx, V[0], V[1], V[2], N[0], N[1], N[2]);
+    const float ndf = (float)((double)d.nd[c][idx] - d.h_cont * r);
+    d.nd[c][idx] = ndf;
     const int i = d.lo[0] + bi, j = d.lo[1] + bj, k = d.lo[2] + bk;
     const Geom &g = d.g;
     const long p = gidx(g, i, j, k);
@@ -1875,8 +1887,7 @@ __global__ void k_hydro_e(DispDev d, int c) {
     else curl = ((double)H[1][p] - H[1][p - g.sxy]) * d.ik_e[0][i] - ((double)H[0][p] - H[0][p - g.sz]) * d.ik_e[1][j];
     float *const *st = d.st[c];
     const double en = st[0][idx], en1 = st[1][idx];
-    const float vnf = d.vnew[c][idx], ndf = d.nd[c][idx];
-    const double vn = vnf, ndn = ndf;
+    const double vn = d.vnew[c][idx], ndn = ndf;
     double P[2], J[2];
 #pragma unroll
     for (int q = 1; q < 3; q++) { P[q - 1] = st[2 + 2 * q][idx]; J[q - 1] = st[3 + 2 * q][idx]; }
@@ -1890,10 +1901,6 @@ __global__ void k_hydro_e(DispDev d, int c) {
         st[3 + 2 * q][idx] = (float)jn;
     }
     st[3][idx] = (float)(vn - (double)st[2][idx]);
-    st[2][idx] = vnf;
-    const float hold = d.nd_prev[c][idx];
-    d.nd_prev[c][idx] = ndf;
-    d.nd[c][idx] = hold;
     st[1][idx] = (float)en;
     st[0][idx] = (float)e;
     d.E[c][p] = (float)e;
@@ -3159,10 +3166,7 @@ hipError_t launch_hydro(const DispDev &d, hipStream_t st) {
     if (nb <= 0) return hipSuccess;
     const dim3 grid((unsigned)((nb + 255) / 256)), block(256);
     hipLaunchKernelGGL(k_hydro_v, grid, block, 0, st, d);
-    for (int c = 0; c < 3; c++) {
-        hipLaunchKernelGGL(k_hydro_cont, grid, block, 0, st, d, c);
-        hipLaunchKernelGGL(k_hydro_e, grid, block, 0, st, d, c);
-    }
+    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_hydro_ce, grid, block, 0, st, d, c);
     return hipGetLastError();
 }
 
