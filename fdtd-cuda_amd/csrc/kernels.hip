@@ -1772,12 +1772,7 @@ __device__ inline bool box_interior(const DispDev &d, long idx, int &bi, int &bj
 // every neighbour at time level n.  With d1 + d2 = 1 the first two terms are
 // v^n + (d1 - 1)(v^n - v^{n-1}).  Convection and magnetic terms are off, as
 // in every configuration of the reference (setup.cu:992-999).
-__global__ void k_hydro_v(DispDev d) {
-    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nb) return;
-    int bi, bj, bk;
-    if (!box_interior(d, idx, bi, bj, bk)) return;
+__device__ __forceinline__ void hydro_v_cell(const DispDev &d, long idx) {
     const long oi = (long)d.n[1] * d.n[2], oj = d.n[2], ok = 1;
     const double I2 = d.h_inv_d * d.h_inv_d;
     const float *V0 = d.st[0][2], *V1 = d.st[1][2], *V2 = d.st[2][2];
@@ -1852,22 +1847,17 @@ __device__ __forceinline__ double hydro_cont_sum(const DispDev &d, int c, long i
     return r;
 }
 
-// Continuity and E update of component c in one launch (main.cu:1224-1271
+// Continuity and E update of component c at one cell (main.cu:1224-1271
 // and the y, z equivalents): n~^{n+1} of the cell, then
 // E^{n+1} = a0 curl + h_cur v^{n+1} (1 + n~^{n+1}) + a1 E^n - a2 E^{n-1} - CP terms,
-// then the CP poles advance.  Nothing this launch reads at a neighbour is
-// written by it: the components before c are read at n+1 from vnew and nd
-// (written by k_hydro_v and the earlier launches), c and the later ones at n
-// from the velocity state and nd_prev, which no launch of the step writes;
-// the cell's own n~ goes from n-1 to n+1 in nd.  The level shift (velocity
-// state <- vnew, nd_prev <-> nd) is a swap of the buffers' roles on the host
-// after the step's launches.
-__global__ void k_hydro_ce(DispDev d, int c) {
-    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nb) return;
-    int bi, bj, bk;
-    if (!box_interior(d, idx, bi, bj, bk) || !d.id[c][idx]) return;
+// then the CP poles advance.  Nothing a launch of it reads at a neighbour is
+// written by that launch: the components before c are read at n+1 from vnew
+// and nd (written by the earlier launches of the step), c and the later ones
+// at n from the velocity state and nd_prev, which no launch of the step
+// writes; the cell's own n~ goes from n-1 to n+1 in nd.  The level shift
+// (velocity state <- vnew, nd_prev <-> nd) is a swap of the buffers' roles on
+// the host after the step's launches.
+__device__ __forceinline__ void hydro_ce_cell(const DispDev &d, int c, long idx, int bi, int bj, int bk) {
     const float *V[3], *N[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
@@ -1904,6 +1894,30 @@ __global__ void k_hydro_ce(DispDev d, int c) {
     st[1][idx] = (float)en;
     st[0][idx] = (float)e;
     d.E[c][p] = (float)e;
+}
+
+// The momentum update of every component and the x continuity + E update in
+// one launch: the x update reads the new velocity only at its own cell (the
+// same thread computed it just before), and at neighbours only level-n values,
+// which no launch of the step writes.
+__global__ void k_hydro_v_ce0(DispDev d) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    int bi, bj, bk;
+    if (!box_interior(d, idx, bi, bj, bk)) return;
+    hydro_v_cell(d, idx);
+    if (d.id[0][idx]) hydro_ce_cell(d, 0, idx, bi, bj, bk);
+}
+
+// Continuity + E update of component c = 1 (y) or 2 (z).
+__global__ void k_hydro_ce(DispDev d, int c) {
+    const long nb = (long)d.n[0] * d.n[1] * d.n[2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    int bi, bj, bk;
+    if (!box_interior(d, idx, bi, bj, bk) || !d.id[c][idx]) return;
+    hydro_ce_cell(d, c, idx, bi, bj, bk);
 }
 
 // Copies E^n from the field into the box state (after a field overwrite) or
@@ -3165,8 +3179,8 @@ hipError_t launch_hydro(const DispDev &d, hipStream_t st) {
     const long nb = (long)d.n[0] * d.n[1] * d.n[2];
     if (nb <= 0) return hipSuccess;
     const dim3 grid((unsigned)((nb + 255) / 256)), block(256);
-    hipLaunchKernelGGL(k_hydro_v, grid, block, 0, st, d);
-    for (int c = 0; c < 3; c++) hipLaunchKernelGGL(k_hydro_ce, grid, block, 0, st, d, c);
+    hipLaunchKernelGGL(k_hydro_v_ce0, grid, block, 0, st, d);
+    for (int c = 1; c < 3; c++) hipLaunchKernelGGL(k_hydro_ce, grid, block, 0, st, d, c);
     return hipGetLastError();
 }
 
